@@ -1,0 +1,152 @@
+/*
+ * kete_hip.h -- C ABI of libkete_hip.so, the MI355X (gfx950) implementation of
+ * kete's batched orbit-propagation path.
+ *
+ * The reference (Caltech-IPAC/kete 1.1.3) is Rust with no FFI of its own; each
+ * entry point below replaces the per-object loop body of one of its batch
+ * entry points, called once per batch instead of once per object from rayon.
+ * Paths are relative to the kete source tree.
+ *
+ * Conventions
+ *  - Host-pointer entry points copy inputs to the device, run, copy results
+ *    back and synchronise before returning (kete's calls are synchronous).
+ *    The *_dev variants take device pointers and run on the context's stream;
+ *    they return once the work is enqueued unless `sync` is non-zero.
+ *  - States are structure-of-arrays: xyz[0..2] position (AU), xyz[3..5]
+ *    velocity (AU/day), one array per component, object index fastest.
+ *  - Frames use kete's SPK numbering: 1 = Equatorial (J2000), 17 = Ecliptic.
+ *    Centers are NAIF ids (0 = solar-system barycentre, 10 = Sun).
+ *  - Per-object status mirrors kete_core::errors::Error (errors.rs:14-32):
+ *    0 ok, 1 Convergence, 2 ValueError, 3 DAFLimits, 4 UnknownFrame,
+ *    5 IOError, 6 Impact.  A failed object's outputs are NaN, as the pyo3
+ *    bindings return State::new_nan (src/kete/rust/propagation.rs:115-147).
+ *  - The return value is an API status: KH_OK or a negative KH_E_* code; the
+ *    message is available from kh_last_error().
+ */
+#ifndef KETE_HIP_H
+#define KETE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kh_ctx kh_ctx;
+
+enum {
+    KH_OK = 0,
+    KH_E_ARG = -1,      /* invalid argument                            */
+    KH_E_HIP = -2,      /* HIP runtime failure                         */
+    KH_E_NO_SPK = -3,   /* no ephemeris uploaded / body missing        */
+    KH_E_NO_DEVICE = -4 /* no GPU visible                              */
+};
+
+enum { KH_FRAME_EQUATORIAL = 1, KH_FRAME_ECLIPTIC = 17 };
+
+/* Per-object work counters of the Radau path. */
+typedef struct {
+    int32_t n_force; /* force evaluations                          */
+    int32_t n_sweep; /* predictor-corrector sweeps (incl. failed)  */
+    int32_t n_acc;   /* accepted steps                             */
+    int32_t n_fail;  /* failed step attempts                       */
+} kh_counters;
+
+/* JPL comet / dust non-gravitational model (propagation/nongrav.rs:12-61).
+ * kind: 0 none, 1 JplComet (a1..k), 2 Dust (beta).  The JplComet time delay
+ * `dt` must be 0 (a non-zero delay is rejected with KH_E_ARG). */
+typedef struct {
+    int32_t kind;
+    double a1, a2, a3, alpha, r_0, m, n, k, dt, beta;
+} kh_nongrav;
+
+/* One SPK Type 2 segment (spice/spk_segments.rs:316-397).  `records` is the
+ * segment's DAF array without its 4-double trailer: n_rec records of rsize
+ * doubles [mid, radius, x[n], y[n], z[n]] (km, TDB seconds past J2000).
+ * jd_start/jd_end are the summary times converted with spice_jds_to_jd
+ * (spice/mod.rs:25-27); intlen_s is the trailer's INTLEN. */
+typedef struct {
+    int64_t obj_id;
+    int64_t center_id;
+    int32_t frame;
+    int32_t rsize;
+    int64_t n_rec;
+    double jd_start;
+    double jd_end;
+    double intlen_s;
+    const double *records;
+} kh_spk2_seg;
+
+/* Device selection and stream ownership.  Replaces nothing in kete (its
+ * state is the process-global LOADED_SPK, spice/spk.rs:368-377); one context
+ * per host thread. */
+int32_t kh_ctx_create(int32_t device, kh_ctx **out);
+int32_t kh_ctx_destroy(kh_ctx *ctx);
+const char *kh_last_error(void);
+/* hipStream_t of the context (as void*), for callers that enqueue around it. */
+void *kh_ctx_stream(kh_ctx *ctx);
+
+/* Upload the Type 2 ephemeris.  Replaces SpkCollection::load_file for the
+ * planet segments kete's spk_accel reads (spice/spk.rs:307-329,65-82): the
+ * host side walks SpkCollection.planet_segments and passes them here. */
+int32_t kh_spk_upload_type2(kh_ctx *ctx, const kh_spk2_seg *segs, int64_t nseg);
+
+/* State of `obj_id` relative to `center` in `frame` at m epochs, evaluated on
+ * the device (SpkCollection::try_get_state, spice/spk.rs:87-96).
+ * out: [6][m]; status: [m]. */
+int32_t kh_spk_states(kh_ctx *ctx, int64_t obj_id, int64_t center, int32_t frame,
+                      int64_t m, const double *jd, double *const out[6], int32_t *status);
+
+/* Two-body propagation of n states to m target epochs.  Replaces the loop
+ * body of propagation_kepler_py (src/kete/rust/kepler.rs:68-108): center to
+ * the Sun, analytic_2_body (src/kete_core/src/propagation/kepler.rs:254-322),
+ * optional one-step light delay against observer_pos, center back.
+ * center must be 10 or 0.  out: [6][m][n]; status: [m][n]. */
+int32_t kh_propagate_two_body(kh_ctx *ctx, int64_t n, const double *jd0,
+                              const double *const xyz[6], int64_t center, int32_t frame,
+                              int64_t m, const double *jd_targets,
+                              const double *observer_pos, double *const out[6],
+                              int32_t *status);
+int32_t kh_propagate_two_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
+                                  const double *const xyz[6], int64_t center, int32_t frame,
+                                  int64_t m, const double *jd_targets,
+                                  const double *observer_pos, double *const out[6],
+                                  int32_t *status, int32_t sync);
+
+/* N-body propagation with the Gauss-Radau 15 integrator and kete's SPK force
+ * model.  Replaces the loop body of propagation_n_body_spk_py
+ * (src/kete/rust/propagation.rs:70-151) around propagate_n_body_spk
+ * (src/kete_core/src/propagation/mod.rs:70-118) with include_asteroids=false.
+ * jd_final: n entries, or 1 entry broadcast when jd_final_stride is 0.
+ * epsilon: Radau EPSILON; 1e-6 reproduces kete (radau.rs:71).
+ * mode: 0 plain launch (one lane per object), 1 persistent launch with a
+ * device work queue, -1 let the library choose.
+ * ng, counters, impact_id, impact_jd are nullable. */
+int32_t kh_propagate_n_body(kh_ctx *ctx, int64_t n, const double *jd0,
+                            const double *const xyz[6], int64_t center, int32_t frame,
+                            const double *jd_final, int32_t jd_final_stride,
+                            const kh_nongrav *ng, double epsilon, int32_t mode,
+                            double *const out[6], int32_t *status, int64_t *impact_id,
+                            double *impact_jd, kh_counters *counters);
+/* Device-pointer variant; ng points to device memory (nullable), counters is
+ * a device array of n kh_counters (nullable), impact_id is int64 on device. */
+int32_t kh_propagate_n_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
+                                const double *const xyz[6], int64_t center, int32_t frame,
+                                const double *jd_final, int32_t jd_final_stride,
+                                const kh_nongrav *ng, double epsilon, int32_t mode,
+                                double *const out[6], int32_t *status, int64_t *impact_id,
+                                double *impact_jd, kh_counters *counters, int32_t sync);
+
+/* Launch geometry last used by kh_propagate_n_body*: threads per block,
+ * blocks, and mode actually run. */
+int32_t kh_last_launch(kh_ctx *ctx, int32_t *block, int32_t *grid, int32_t *mode);
+
+/* Average duration in milliseconds of the last n kernel launches of the
+ * N-body path, measured with HIP events on the context's stream. */
+double kh_last_kernel_ms(kh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

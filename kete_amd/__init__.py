@@ -1,0 +1,301 @@
+"""kete_amd -- kete's batched orbit-propagation path on one AMD MI355X.
+
+The Python surface mirrors kete's (``kete.propagate_n_body``,
+``kete.propagate_two_body``, ``kete.fov_state_check``, ``kete.State``,
+``kete.RectangleFOV``; ``src/kete/propagation.py:8-14``, ``src/kete/fov.py``)
+so that calling code can switch by changing the import.  Work is done by the
+HIP kernels of ``libkete_hip.so`` through its C ABI (``include/kete_hip.h``);
+batches can also be passed as structure-of-arrays (:class:`StateArray`) to skip
+per-object Python objects.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .spk import FRAME_ECLIPTIC, FRAME_EQUATORIAL, Segment, Type2Table, read_spk
+
+__all__ = [
+    "State", "StateArray", "NonGravModel", "Context", "propagate_n_body", "propagate_two_body",
+    "spk_state", "load_spk", "get_context", "Frames", "KeteHipError",
+]
+
+KeteHipError = _lib.KeteHipError
+
+STATUS_NAMES = {0: "ok", 1: "Convergence", 2: "ValueError", 3: "DAFLimits", 4: "UnknownFrame",
+                5: "IOError", 6: "Impact"}
+
+
+class Frames:
+    """kete's frame names with their SPK numbers."""
+    Equatorial = FRAME_EQUATORIAL
+    Ecliptic = FRAME_ECLIPTIC
+
+
+@dataclass
+class State:
+    """One object's state (``state.rs:76-95``): position (AU), velocity
+    (AU/day), TDB JD, frame, and center NAIF id."""
+    desig: object
+    jd: float
+    pos: np.ndarray
+    vel: np.ndarray
+    frame: int = FRAME_ECLIPTIC
+    center_id: int = 10
+
+    def __post_init__(self):
+        self.pos = np.asarray(self.pos, dtype=np.float64).reshape(3)
+        self.vel = np.asarray(self.vel, dtype=np.float64).reshape(3)
+
+    @property
+    def is_finite(self) -> bool:
+        return bool(np.all(np.isfinite(self.pos)) and np.all(np.isfinite(self.vel))
+                    and np.isfinite(self.jd))
+
+
+@dataclass
+class StateArray:
+    """A batch of states as structure-of-arrays: ``xyz[0:3]`` positions,
+    ``xyz[3:6]`` velocities, one frame and center for the batch."""
+    jd: np.ndarray
+    xyz: np.ndarray
+    frame: int = FRAME_ECLIPTIC
+    center_id: int = 10
+    desigs: list | None = field(default=None, repr=False)
+
+    def __post_init__(self):
+        self.xyz = np.ascontiguousarray(self.xyz, dtype=np.float64)
+        if self.xyz.ndim != 2 or self.xyz.shape[0] != 6:
+            raise ValueError("xyz must have shape (6, n)")
+        self.jd = np.ascontiguousarray(np.broadcast_to(np.asarray(self.jd, dtype=np.float64),
+                                                       (self.xyz.shape[1],)))
+
+    def __len__(self):
+        return self.xyz.shape[1]
+
+    @classmethod
+    def from_states(cls, states):
+        states = list(states)
+        if not states:
+            raise ValueError("empty state list")
+        frame, center = states[0].frame, states[0].center_id
+        if any(s.frame != frame or s.center_id != center for s in states):
+            raise ValueError("all states in a batch must share frame and center")
+        xyz = np.array([np.concatenate([s.pos, s.vel]) for s in states]).T
+        return cls(np.array([s.jd for s in states]), xyz, frame, center,
+                   [s.desig for s in states])
+
+    def to_states(self):
+        d = self.desigs or [None] * len(self)
+        return [State(d[i], float(self.jd[i]), self.xyz[:3, i].copy(), self.xyz[3:, i].copy(),
+                      self.frame, self.center_id) for i in range(len(self))]
+
+
+class NonGravModel:
+    """Non-gravitational models (``propagation/nongrav.rs:12-61``; Python
+    constructors ``src/kete/rust/nongrav.rs:72-161``)."""
+
+    def __init__(self, kind, a1=0.0, a2=0.0, a3=0.0, alpha=1.0, r_0=1.0, m=2.0, n=1.0, k=0.0,
+                 dt=0.0, beta=0.0):
+        self.params = dict(kind=kind, a1=a1, a2=a2, a3=a3, alpha=alpha, r_0=r_0, m=m, n=n, k=k,
+                           dt=dt, beta=beta)
+
+    @classmethod
+    def new_dust(cls, beta):
+        return cls(2, beta=beta)
+
+    @classmethod
+    def new_comet(cls, a1=0.0, a2=0.0, a3=0.0, alpha=0.1112620426, r_0=2.808, m=2.15, n=5.093,
+                  k=4.6142, dt=0.0):
+        return cls(1, a1, a2, a3, alpha, r_0, m, n, k, dt)
+
+    @classmethod
+    def new_asteroid(cls, a1=0.0, a2=0.0, a3=0.0, alpha=1.0, r_0=1.0, m=2.0, n=1.0, k=0.0,
+                     dt=0.0):
+        return cls(1, a1, a2, a3, alpha, r_0, m, n, k, dt)
+
+
+def nongrav_array(models, n):
+    arr = np.zeros(n, dtype=_lib.NONGRAV_DTYPE)
+    for i, m in enumerate(models):
+        if m is None:
+            continue
+        for k, v in m.params.items():
+            arr[k][i] = v
+    return arr
+
+
+class Context:
+    """A device, its stream and the uploaded ephemeris (kete's process-global
+    ``LOADED_SPK``, ``spice/spk.rs:368-377``, becomes per-context state)."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.L.kh_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.table: Type2Table | None = None
+
+    def close(self):
+        if self.h:
+            self.L.kh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ephemeris ---------------------------------------------------------
+    def load_spk(self, path_or_segments):
+        segs = read_spk(path_or_segments) if isinstance(path_or_segments, str) else \
+            list(path_or_segments)
+        table = Type2Table(segs)
+        arr = (_lib.KhSpk2Seg * len(table.segments))()
+        keep = []
+        for k, s in enumerate(table.segments):
+            rec = np.ascontiguousarray(s.records())
+            keep.append(rec)
+            arr[k] = _lib.KhSpk2Seg(s.obj_id, s.center_id, s.frame, s.rsize, s.n_rec, s.jd_start,
+                                    s.jd_end, s.intlen, _lib.dptr(rec.reshape(-1)))
+        _lib.check(self.L.kh_spk_upload_type2(self.h, arr, len(table.segments)))
+        self.table = table
+        return table
+
+    def spk_state(self, obj_id, jd, center=10, frame=FRAME_ECLIPTIC):
+        jd = np.ascontiguousarray(np.atleast_1d(jd), dtype=np.float64)
+        out = np.empty((6, len(jd)))
+        st = np.empty(len(jd), dtype=np.int32)
+        _lib.check(self.L.kh_spk_states(self.h, obj_id, center, frame, len(jd), _lib.dptr(jd),
+                                        _lib.ptr_array(list(out)), _lib.iptr(st, C.c_int32)))
+        return out, st
+
+    # -- propagation ---------------------------------------------------------
+    def two_body(self, batch: StateArray, targets, observer_pos=None):
+        """-> (out[6, m, n], status[m, n])"""
+        targets = np.ascontiguousarray(np.atleast_1d(targets), dtype=np.float64)
+        n, m = len(batch), len(targets)
+        out = np.empty((6, m, n))
+        st = np.empty((m, n), dtype=np.int32)
+        obs = None
+        if observer_pos is not None:
+            obs_arr = np.ascontiguousarray(observer_pos, dtype=np.float64).reshape(3)
+            obs = _lib.dptr(obs_arr)
+        _lib.check(self.L.kh_propagate_two_body(
+            self.h, n, _lib.dptr(batch.jd), _lib.ptr_array(list(batch.xyz)), batch.center_id,
+            batch.frame, m, _lib.dptr(targets), obs,
+            _lib.ptr_array([out[k] for k in range(6)]), _lib.iptr(st, C.c_int32)))
+        return out, st
+
+    def n_body(self, batch: StateArray, jd_final, non_gravs=None, epsilon=1e-6, mode=-1,
+               counters=False):
+        """-> dict(out[6, n], status[n], impact_id[n], impact_jd[n], counters[n, 4])"""
+        if self.table is None:
+            raise KeteHipError("no ephemeris loaded")
+        n = len(batch)
+        jf = np.ascontiguousarray(np.atleast_1d(jd_final), dtype=np.float64)
+        stride = 0 if len(jf) == 1 else 1
+        if stride and len(jf) != n:
+            raise ValueError("jd_final must have 1 or n entries")
+        out = np.empty((6, n))
+        st = np.empty(n, dtype=np.int32)
+        iid = np.empty(n, dtype=np.int64)
+        ijd = np.empty(n)
+        cnt = np.zeros((n, 4), dtype=np.int32) if counters else None
+        ng = None
+        if non_gravs is not None:
+            ng_arr = non_gravs if isinstance(non_gravs, np.ndarray) else nongrav_array(non_gravs, n)
+            ng = ng_arr.ctypes.data_as(C.POINTER(_lib.KhNonGrav))
+        _lib.check(self.L.kh_propagate_n_body(
+            self.h, n, _lib.dptr(batch.jd), _lib.ptr_array(list(batch.xyz)), batch.center_id,
+            batch.frame, _lib.dptr(jf), stride, ng, epsilon, mode,
+            _lib.ptr_array([out[k] for k in range(6)]), _lib.iptr(st, C.c_int32),
+            _lib.iptr(iid, C.c_int64), _lib.dptr(ijd),
+            cnt.ctypes.data_as(C.POINTER(_lib.KhCounters)) if cnt is not None else None))
+        return dict(out=out, status=st, impact_id=iid, impact_jd=ijd, counters=cnt)
+
+    def last_kernel_ms(self) -> float:
+        return self.L.kh_last_kernel_ms(self.h)
+
+    def last_launch(self):
+        b, g, m = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(self.L.kh_last_launch(self.h, C.byref(b), C.byref(g), C.byref(m)))
+        return b.value, g.value, m.value
+
+
+_CTX: Context | None = None
+
+
+def get_context() -> Context:
+    """The default context, with the synthetic ephemeris loaded on first use."""
+    global _CTX
+    if _CTX is None:
+        ctx = Context(0)
+        from .synthetic import ephemeris_path
+        ctx.load_spk(ephemeris_path())
+        _CTX = ctx
+    return _CTX
+
+
+def load_spk(path: str) -> Type2Table:
+    """Replace the default context's ephemeris (kete ``spice.spk.load``)."""
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX.load_spk(path)
+
+
+def spk_state(obj_id, jd, center=10, frame=FRAME_ECLIPTIC):
+    out, st = get_context().spk_state(obj_id, jd, center, frame)
+    return out, st
+
+
+def _as_batch(states):
+    if isinstance(states, StateArray):
+        return states, True
+    if isinstance(states, State):
+        states = [states]
+    return StateArray.from_states(states), False
+
+
+def propagate_two_body(states, jd, observer_pos=None):
+    """kete ``propagate_two_body`` (``src/kete/rust/kepler.rs:68-108``): two-body
+    motion about the Sun to ``jd``, with an optional one-step light delay as
+    seen from ``observer_pos``.  Failed objects come back as NaN states."""
+    batch, soa = _as_batch(states)
+    out, _st = get_context().two_body(batch, [jd], observer_pos)
+    res = StateArray(np.full(len(batch), float(jd)), out[:, 0, :], batch.frame, batch.center_id,
+                     batch.desigs)
+    return res if soa else res.to_states()
+
+
+def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppress_errors=True,
+                     suppress_impact_errors=False):
+    """kete ``propagate_n_body`` (``src/kete/rust/propagation.rs:70-151``):
+    Radau-15 integration with the planets, Moon and solar/Jovian GR+J2 read from
+    the loaded SPK.  Failed objects come back as NaN states; with
+    ``suppress_errors=False`` the first failure raises ``ValueError``."""
+    if include_asteroids:
+        raise NotImplementedError("include_asteroids needs SPK Type 21 perturbers, "
+                                  "which this build does not evaluate")
+    batch, soa = _as_batch(states)
+    if non_gravs is not None and len(non_gravs) != len(batch):
+        raise ValueError("non_gravs must be the same length as states.")
+    r = get_context().n_body(batch, [jd], non_gravs=non_gravs)
+    st = r["status"]
+    if not suppress_errors and np.any(st != 0):
+        i = int(np.flatnonzero(st != 0)[0])
+        raise ValueError(f"object {i}: {STATUS_NAMES.get(int(st[i]), st[i])}")
+    if not suppress_impact_errors:
+        for i in np.flatnonzero(st == 6):
+            d = batch.desigs[i] if batch.desigs else i
+            print(f"Impact detected between ({d}) <-> ({r['impact_id'][i]}) at time "
+                  f"{r['impact_jd'][i]}", file=sys.stderr)
+    res = StateArray(np.full(len(batch), float(jd)), r["out"], batch.frame, batch.center_id,
+                     batch.desigs)
+    return res if soa else res.to_states()

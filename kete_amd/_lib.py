@@ -1,0 +1,139 @@
+"""ctypes binding of libkete_hip.so (the C ABI in include/kete_hip.h).
+
+The library is built in-tree by ``kete_amd/csrc/Makefile``.  There is no CPU
+fallback: if the library is missing or no GPU is visible, every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkete_hip.so")
+CSRC = os.path.join(HERE, "csrc")
+
+KH_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP runtime failure", -3: "no ephemeris",
+          -4: "no GPU visible"}
+
+# Symbols declared in include/kete_hip.h
+EXPORTS = [
+    "kh_ctx_create", "kh_ctx_destroy", "kh_last_error", "kh_ctx_stream",
+    "kh_spk_upload_type2", "kh_spk_states", "kh_propagate_two_body",
+    "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
+    "kh_last_launch", "kh_last_kernel_ms",
+]
+
+
+class KhSpk2Seg(C.Structure):
+    _fields_ = [("obj_id", C.c_int64), ("center_id", C.c_int64), ("frame", C.c_int32),
+                ("rsize", C.c_int32), ("n_rec", C.c_int64), ("jd_start", C.c_double),
+                ("jd_end", C.c_double), ("intlen_s", C.c_double),
+                ("records", C.POINTER(C.c_double))]
+
+
+class KhNonGrav(C.Structure):
+    _fields_ = [("kind", C.c_int32)] + [(k, C.c_double) for k in
+                                        ("a1", "a2", "a3", "alpha", "r_0", "m", "n", "k", "dt",
+                                         "beta")]
+
+
+class KhCounters(C.Structure):
+    _fields_ = [("n_force", C.c_int32), ("n_sweep", C.c_int32), ("n_acc", C.c_int32),
+                ("n_fail", C.c_int32)]
+
+
+NONGRAV_DTYPE = np.dtype([("kind", np.int32), ("pad", np.int32)] +
+                         [(k, np.float64) for k in ("a1", "a2", "a3", "alpha", "r_0", "m", "n",
+                                                    "k", "dt", "beta")])
+assert NONGRAV_DTYPE.itemsize == C.sizeof(KhNonGrav)
+
+_lib = None
+
+
+class KeteHipError(RuntimeError):
+    pass
+
+
+def build(force: bool = False) -> str:
+    """Compile libkete_hip.so for gfx950 with hipcc (kete_amd/csrc/Makefile)."""
+    cmd = ["make", "-s", "-C", CSRC, "-j8"]
+    if force:
+        subprocess.run(["make", "-s", "-C", CSRC, "clean"], check=True)
+    subprocess.run(cmd, check=True)
+    return LIB_PATH
+
+
+def load():
+    """Load the library (raising if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KeteHipError(f"{LIB_PATH} is not built; run kete_amd._lib.build() "
+                           "(hipcc --offload-arch=gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P, PP, I32, I64, D, V = (C.POINTER(C.c_double), C.POINTER(C.POINTER(C.c_double)), C.c_int32,
+                             C.c_int64, C.c_double, C.c_void_p)
+    for name in EXPORTS:
+        getattr(L, name)
+    L.kh_ctx_create.argtypes = [I32, C.POINTER(V)]
+    L.kh_ctx_create.restype = I32
+    L.kh_ctx_destroy.argtypes = [V]
+    L.kh_ctx_destroy.restype = I32
+    L.kh_last_error.restype = C.c_char_p
+    L.kh_ctx_stream.argtypes = [V]
+    L.kh_ctx_stream.restype = V
+    L.kh_spk_upload_type2.argtypes = [V, C.POINTER(KhSpk2Seg), I64]
+    L.kh_spk_upload_type2.restype = I32
+    L.kh_spk_states.argtypes = [V, I64, I64, I32, I64, P, PP, C.POINTER(I32)]
+    L.kh_spk_states.restype = I32
+    tb = [V, I64, P, PP, I64, I32, I64, P, P, PP, C.POINTER(I32)]
+    L.kh_propagate_two_body.argtypes = tb
+    L.kh_propagate_two_body.restype = I32
+    L.kh_propagate_two_body_dev.argtypes = tb + [I32]
+    L.kh_propagate_two_body_dev.restype = I32
+    nb = [V, I64, P, PP, I64, I32, P, I32, C.POINTER(KhNonGrav), D, I32, PP, C.POINTER(I32),
+          C.POINTER(I64), P, C.POINTER(KhCounters)]
+    L.kh_propagate_n_body.argtypes = nb
+    L.kh_propagate_n_body.restype = I32
+    L.kh_propagate_n_body_dev.argtypes = nb + [I32]
+    L.kh_propagate_n_body_dev.restype = I32
+    L.kh_last_launch.argtypes = [V, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
+    L.kh_last_launch.restype = I32
+    L.kh_last_kernel_ms.argtypes = [V]
+    L.kh_last_kernel_ms.restype = D
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != KH_OK:
+        msg = load().kh_last_error().decode(errors="replace")
+        raise KeteHipError(f"kete_hip error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray, ct):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def ptr_array(ptrs):
+    """Array of double* from numpy arrays or raw device addresses (ints)."""
+    arr = (C.POINTER(C.c_double) * len(ptrs))()
+    for k, p in enumerate(ptrs):
+        arr[k] = C.cast(C.c_void_p(p), C.POINTER(C.c_double)) if isinstance(p, int) else dptr(p)
+    return arr
+
+
+def raw(p: int, ct=C.c_double):
+    return C.cast(C.c_void_p(p), C.POINTER(ct)) if p else None

@@ -1,0 +1,527 @@
+// kh_api.cpp -- host side of libkete_hip.so: the C ABI declared in
+// include/kete_hip.h.  Owns the device, the stream, the uploaded ephemeris and
+// the staging buffers; validates arguments; launches the kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kete_hip.h"
+#include "kh_internal.h"
+
+using namespace kh;
+
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return fail(KH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct kh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  SpkDev spk{};           // host copy
+  SpkDev* d_spk = nullptr;  // device copy read by the kernels
+  bool has_spk = false;
+  double* d_coef = nullptr;
+  std::vector<int64_t> seg_obj, seg_center;
+  void* dbuf = nullptr;
+  size_t dbuf_size = 0;
+  unsigned long long* d_queue = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  int32_t last_block = 0, last_grid = 0, last_mode = 0;
+  int n_cu = 0;
+};
+
+namespace kh {
+void fill_radau_tables(RadauTables& t) {
+  // GAUSS_RADAU_SPACINGS (radau.rs:24-33)
+  static const double H[8] = {0.0,
+                              0.05626256053692215,
+                              0.18024069173689236,
+                              0.3526247171131696,
+                              0.5471536263305554,
+                              0.7342101772154105,
+                              0.8853209468390958,
+                              0.9775206135612875};
+  std::memset(&t, 0, sizeof(t));
+  for (int i = 0; i < 8; ++i) t.h[i] = H[i];
+  // W_VEC, U_VEC, C_MAT (radau.rs:35-68)
+  for (int i = 0; i < 7; ++i) {
+    t.w[i] = 1.0 / (double)((i + 2) * (i + 3));
+    t.u[i] = 1.0 / (double)(i + 2);
+  }
+  for (int i = 0; i < 7; ++i) t.c[i][i] = 1.0;
+  for (int i = 0; i < 7; ++i) {
+    if (i > 0) t.c[i][0] = -H[i] * t.c[i - 1][0];
+    for (int j = 1; j < i; ++j) t.c[i][j] = t.c[i - 1][j - 1] - H[i] * t.c[i - 1][j];
+  }
+  for (int j = 1; j < 8; ++j) {
+    for (int k = 0; k < 7; ++k) {
+      // powi(1 + k) by repeated squaring
+      int e = k + 1;
+      double a = H[j], r = 1.0;
+      while (true) {
+        if (e & 1) r *= a;
+        e /= 2;
+        if (!e) break;
+        a *= a;
+      }
+      t.wpow[j][k] = r * t.w[k];
+      t.upow[j][k] = r * t.u[k];
+    }
+    t.inv_h[j] = 1.0 / H[j];
+    for (int z = 1; z < j; ++z) t.rdiff[j][z] = 1.0 / (H[j] - H[z]);
+  }
+}
+}  // namespace kh
+
+extern "C" {
+
+const char* kh_last_error(void) { return g_err.c_str(); }
+
+int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
+  if (!out) return fail(KH_E_ARG, "out is null");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(KH_E_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(KH_E_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  kh_ctx* c = new kh_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return fail(KH_E_HIP, "hipGetDeviceProperties failed");
+  }
+  c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return fail(KH_E_HIP, "stream/event/queue allocation failed");
+  }
+  RadauTables t;
+  fill_radau_tables(t);
+  if (upload_radau_tables(t) != 0) {
+    delete c;
+    return fail(KH_E_HIP, "upload of Radau tables failed");
+  }
+  *out = c;
+  return KH_OK;
+}
+
+int32_t kh_ctx_destroy(kh_ctx* c) {
+  if (!c) return KH_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_coef) (void)hipFree(c->d_coef);
+  if (c->d_spk) (void)hipFree(c->d_spk);
+  if (c->dbuf) (void)hipFree(c->dbuf);
+  if (c->d_queue) (void)hipFree(c->d_queue);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return KH_OK;
+}
+
+void* kh_ctx_stream(kh_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+static int find_seg(const kh_ctx* c, int64_t obj) {
+  for (size_t k = 0; k < c->seg_obj.size(); ++k)
+    if (c->seg_obj[k] == obj) return (int)k;
+  return -1;
+}
+
+// Segments whose states sum to the SSB state of `id` (center chaining of
+// spice/spk.rs:99-126 for the Type 2 planet tree).
+static bool build_chain(const kh_ctx* c, int64_t id, Chain& ch) {
+  ch.n = 0;
+  int64_t cur = id;
+  while (cur != 0) {
+    int s = find_seg(c, cur);
+    if (s < 0 || ch.n >= MAX_CHAIN) return false;
+    ch.seg[ch.n++] = s;
+    cur = c->seg_center[s];
+  }
+  return true;
+}
+
+int32_t kh_spk_upload_type2(kh_ctx* c, const kh_spk2_seg* segs, int64_t nseg) {
+  if (!c || !segs || nseg <= 0) return fail(KH_E_ARG, "null context or no segments");
+  if (nseg > MAX_SEGS) return fail(KH_E_ARG, "too many segments (max 24)");
+  HIPCHK(hipSetDevice(c->device));
+  SpkDev spk{};
+  int64_t total = 0;
+  bool all14 = true;
+  c->seg_obj.clear();
+  c->seg_center.clear();
+  for (int64_t k = 0; k < nseg; ++k) {
+    const kh_spk2_seg& s = segs[k];
+    if (!s.records || s.n_rec <= 0 || s.rsize < 8 || (s.rsize - 2) % 3 != 0 ||
+        (s.rsize - 2) / 3 > MAX_COEF)
+      return fail(KH_E_ARG, "segment " + std::to_string(k) + ": bad record layout");
+    if (s.frame != KH_FRAME_EQUATORIAL)
+      return fail(KH_E_ARG, "segment " + std::to_string(k) + ": only frame 1 (equatorial) is supported");
+    if (!(s.intlen_s > 0.0) || !(s.jd_end >= s.jd_start))
+      return fail(KH_E_ARG, "segment " + std::to_string(k) + ": bad time range");
+    SegDev& g = spk.seg[k];
+    g.jd_start = s.jd_start;
+    g.jd_end = s.jd_end;
+    g.j0s = (s.jd_start - 2451545.0) * 86400.0;  // jd_to_spice_jd (spice/mod.rs:30-34)
+    g.intlen = s.intlen_s;
+    g.off = total;
+    g.n_rec = (int32_t)s.n_rec;
+    g.rsize = s.rsize;
+    g.n_coef = (s.rsize - 2) / 3;
+    if (g.n_coef != 14) all14 = false;
+    total += s.n_rec * s.rsize;
+    c->seg_obj.push_back(s.obj_id);
+    c->seg_center.push_back(s.center_id);
+  }
+  spk.n_seg = (int32_t)nseg;
+  spk.all14 = all14 ? 1 : 0;
+  // PLANETS (gravity.rs:43-101,145-148): id, mass, radius, J2 and the
+  // special terms of GravParams::add_acceleration (gravity.rs:171-216).
+  struct PB { int64_t id; double mass, radius, j2; int kind; };
+  const double GMS_ = GMS;
+  const PB pl[N_PLANETS] = {
+      {10, GMS_, 0.004654758765894654, SUN_J2, 1},
+      {1, 1.66012082548908e-07 * GMS_, 1.63139354095098e-05, 0.0, 0},
+      {2, 2.44783828779694e-06 * GMS_, 4.04537843465442e-05, 0.0, 0},
+      {399, 3.00348961546514e-06 * GMS_, 4.33036684926559e-05, EARTH_J2, 2},
+      {301, 3.69430335010988e-08 * GMS_, 1.16138016662292e-05, 0.0, 0},
+      {4, 3.22715608291416e-07 * GMS_, 2.27021279387769e-05, 0.0, 0},
+      {5, 0.0009547919099414246 * GMS_, 0.00047789450254521576, JUPITER_J2, 1},
+      {6, 0.00028588567002459455 * GMS_, 0.0004028666966848747, 0.0, 0},
+      {7, 4.36624961322212e-05 * GMS_, 0.0001708513622580592, 0.0, 0},
+      {8, 5.15138377265457e-05 * GMS_, 0.0001655371154958558, 0.0, 0},
+  };
+  spk.emb_seg = -1;
+  for (int b = 0; b < N_PLANETS; ++b) {
+    int s = find_seg(c, pl[b].id);
+    if (s < 0) return fail(KH_E_NO_SPK, "no segment for body " + std::to_string(pl[b].id));
+    BodyDev& bd = spk.body[b];
+    bd.seg = s;
+    bd.emb = 0;
+    bd.id = (int32_t)pl[b].id;
+    bd.kind = pl[b].kind;
+    bd.mass = pl[b].mass;
+    bd.radius = pl[b].radius;
+    bd.j2 = pl[b].j2;
+    int64_t cen = c->seg_center[s];
+    if (cen == 3) {
+      int e = find_seg(c, 3);
+      if (e < 0 || c->seg_center[e] != 0)
+        return fail(KH_E_NO_SPK, "no segment for the Earth-Moon barycentre (3) w.r.t. 0");
+      bd.emb = 1;
+      spk.emb_seg = e;
+    } else if (cen != 0) {
+      return fail(KH_E_NO_SPK, "body " + std::to_string(pl[b].id) + " must be w.r.t. 0 or 3");
+    }
+  }
+  if (spk.emb_seg < 0) spk.emb_seg = spk.body[0].seg;
+  spk.sun_seg = spk.body[0].seg;
+  if (c->d_coef) {
+    HIPCHK(hipFree(c->d_coef));
+    c->d_coef = nullptr;
+  }
+  HIPCHK(hipMalloc(&c->d_coef, total * sizeof(double)));
+  for (int64_t k = 0; k < nseg; ++k)
+    HIPCHK(hipMemcpy(c->d_coef + spk.seg[k].off, segs[k].records,
+                     segs[k].n_rec * segs[k].rsize * sizeof(double), hipMemcpyHostToDevice));
+  spk.coef = c->d_coef;
+  if (!c->d_spk) HIPCHK(hipMalloc(&c->d_spk, sizeof(SpkDev)));
+  HIPCHK(hipMemcpy(c->d_spk, &spk, sizeof(SpkDev), hipMemcpyHostToDevice));
+  c->spk = spk;
+  c->has_spk = true;
+  return KH_OK;
+}
+
+// Grow-only device staging buffer for the host-pointer entry points.
+static int32_t staging(kh_ctx* c, size_t bytes, char** out) {
+  if (bytes > c->dbuf_size) {
+    if (c->dbuf) HIPCHK(hipFree(c->dbuf));
+    c->dbuf = nullptr;
+    c->dbuf_size = 0;
+    HIPCHK(hipMalloc(&c->dbuf, bytes));
+    c->dbuf_size = bytes;
+  }
+  *out = (char*)c->dbuf;
+  return KH_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int32_t kh_spk_states(kh_ctx* c, int64_t obj_id, int64_t center, int32_t frame, int64_t m,
+                      const double* jd, double* const out[6], int32_t* status) {
+  if (!c || !jd || !out || !status || m < 0) return fail(KH_E_ARG, "null argument");
+  if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  Chain body, cen;
+  if (!build_chain(c, obj_id, body)) return fail(KH_E_NO_SPK, "no path from object to 0");
+  if (!build_chain(c, center, cen)) return fail(KH_E_NO_SPK, "no path from center to 0");
+  if (m == 0) return KH_OK;
+  HIPCHK(hipSetDevice(c->device));
+  size_t bj = align256(m * 8), bo = align256(m * 8), bs = align256(m * 4);
+  char* base;
+  int32_t r = staging(c, bj + 6 * bo + bs, &base);
+  if (r) return r;
+  double* d_jd = (double*)base;
+  double* d_out[6];
+  for (int k = 0; k < 6; ++k) d_out[k] = (double*)(base + bj + k * bo);
+  int32_t* d_st = (int32_t*)(base + bj + 6 * bo);
+  HIPCHK(hipMemcpyAsync(d_jd, jd, m * 8, hipMemcpyHostToDevice, c->stream));
+  if (launch_spk_states(c->d_spk, body, cen, frame, m, d_jd, d_out, d_st, c->stream))
+    return fail(KH_E_HIP, "spk_states launch failed");
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], d_out[k], m * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, d_st, m * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+static int32_t check_two_body_args(kh_ctx* c, int64_t n, const double* jd0,
+                                   const double* const xyz[6], int64_t center, int32_t frame,
+                                   int64_t m, const double* t, double* const out[6],
+                                   int32_t* status) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (n < 0 || m < 0) return fail(KH_E_ARG, "negative size");
+  if ((n > 0 && (!jd0 || !xyz)) || (m > 0 && !t) || (n * m > 0 && (!out || !status)))
+    return fail(KH_E_ARG, "null array");
+  for (int k = 0; n > 0 && k < 6; ++k)
+    if (!xyz[k] || (m > 0 && !out[k])) return fail(KH_E_ARG, "null state column");
+  if (center != 10 && center != 0) return fail(KH_E_ARG, "center must be 10 or 0");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  if (center == 0 && !c->has_spk) return fail(KH_E_NO_SPK, "center 0 needs an ephemeris");
+  return KH_OK;
+}
+
+int32_t kh_propagate_two_body_dev(kh_ctx* c, int64_t n, const double* jd0,
+                                  const double* const xyz[6], int64_t center, int32_t frame,
+                                  int64_t m, const double* jd_targets,
+                                  const double* observer_pos, double* const out[6],
+                                  int32_t* status, int32_t sync) {
+  int32_t r = check_two_body_args(c, n, jd0, xyz, center, frame, m, jd_targets, out, status);
+  if (r) return r;
+  if (n == 0 || m == 0) return KH_OK;
+  HIPCHK(hipSetDevice(c->device));
+  TwoBodyLaunch p{};
+  p.n = n;
+  p.m = m;
+  p.jd0 = jd0;
+  for (int k = 0; k < 6; ++k) {
+    p.xyz[k] = xyz[k];
+    p.out[k] = out[k];
+  }
+  p.targets = jd_targets;
+  p.has_observer = observer_pos != nullptr;
+  if (observer_pos)
+    for (int k = 0; k < 3; ++k) p.observer[k] = observer_pos[k];
+  p.center_sun = center == 10;
+  p.sun_seg = c->has_spk ? c->spk.sun_seg : 0;
+  p.frame = frame;
+  p.status = status;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (launch_two_body(c->d_spk, p, c->stream)) return fail(KH_E_HIP, "two-body launch failed");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  if (sync) HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+int32_t kh_propagate_two_body(kh_ctx* c, int64_t n, const double* jd0,
+                              const double* const xyz[6], int64_t center, int32_t frame,
+                              int64_t m, const double* jd_targets, const double* observer_pos,
+                              double* const out[6], int32_t* status) {
+  int32_t r = check_two_body_args(c, n, jd0, xyz, center, frame, m, jd_targets, out, status);
+  if (r) return r;
+  if (n == 0 || m == 0) return KH_OK;
+  HIPCHK(hipSetDevice(c->device));
+  size_t bn = align256(n * 8), bm = align256(m * 8), bo = align256(n * m * 8),
+         bs = align256(n * m * 4);
+  char* base;
+  r = staging(c, 7 * bn + bm + 6 * bo + bs, &base);
+  if (r) return r;
+  double* d_jd0 = (double*)base;
+  const double* d_xyz[6];
+  double* d_out[6];
+  for (int k = 0; k < 6; ++k) {
+    d_xyz[k] = (double*)(base + (1 + k) * bn);
+    d_out[k] = (double*)(base + 7 * bn + bm + k * bo);
+  }
+  double* d_t = (double*)(base + 7 * bn);
+  int32_t* d_st = (int32_t*)(base + 7 * bn + bm + 6 * bo);
+  HIPCHK(hipMemcpyAsync(d_jd0, jd0, n * 8, hipMemcpyHostToDevice, c->stream));
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync((void*)d_xyz[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_t, jd_targets, m * 8, hipMemcpyHostToDevice, c->stream));
+  r = kh_propagate_two_body_dev(c, n, d_jd0, d_xyz, center, frame, m, d_t, observer_pos, d_out,
+                                d_st, 0);
+  if (r) return r;
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], d_out[k], n * m * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, d_st, n * m * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+static int32_t check_nbody_args(kh_ctx* c, int64_t n, const double* jd0,
+                                const double* const xyz[6], int64_t center, int32_t frame,
+                                const double* jd_final, double epsilon, int32_t mode,
+                                double* const out[6], int32_t* status, Chain& ch) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  if (n < 0) return fail(KH_E_ARG, "negative size");
+  if (n > 0 && (!jd0 || !xyz || !jd_final || !out || !status)) return fail(KH_E_ARG, "null array");
+  for (int k = 0; n > 0 && k < 6; ++k)
+    if (!xyz[k] || !out[k]) return fail(KH_E_ARG, "null state column");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  if (!(epsilon > 0.0)) return fail(KH_E_ARG, "epsilon must be positive");
+  if (mode < -1 || mode > 1) return fail(KH_E_ARG, "mode must be -1, 0 or 1");
+  if (!build_chain(c, center, ch))
+    return fail(KH_E_NO_SPK, "center " + std::to_string(center) + " is not in the ephemeris");
+  return KH_OK;
+}
+
+int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
+                                const double* const xyz[6], int64_t center, int32_t frame,
+                                const double* jd_final, int32_t jd_final_stride,
+                                const kh_nongrav* ng, double epsilon, int32_t mode,
+                                double* const out[6], int32_t* status, int64_t* impact_id,
+                                double* impact_jd, kh_counters* counters, int32_t sync) {
+  Chain ch;
+  int32_t r = check_nbody_args(c, n, jd0, xyz, center, frame, jd_final, epsilon, mode, out, status,
+                               ch);
+  if (r) return r;
+  if (n == 0) return KH_OK;
+  HIPCHK(hipSetDevice(c->device));
+  NBodyLaunch p{};
+  p.n = n;
+  p.jd0 = jd0;
+  for (int k = 0; k < 6; ++k) {
+    p.xyz[k] = xyz[k];
+    p.out[k] = out[k];
+  }
+  p.jd_final = jd_final;
+  p.jd_final_stride = jd_final_stride ? 1 : 0;
+  p.frame = frame;
+  p.center = ch;
+  p.ng = ng;
+  p.epsilon = epsilon;
+  p.status = status;
+  p.impact_id = impact_id;
+  p.impact_jd = impact_jd;
+  p.counters = reinterpret_cast<int32_t*>(counters);
+  p.queue = c->d_queue;
+  p.mode = mode == 0 ? 0 : 1;
+  p.block = 256;
+  p.grid = 0;
+  if (p.mode == 1) {
+    // persistent: enough blocks to fill every CU at the kernel's occupancy
+    int per_cu = 2;
+    p.grid = c->n_cu * per_cu * 4;
+  }
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (launch_nbody(c->d_spk, p, c->stream, &c->last_block, &c->last_grid, &c->last_mode))
+    return fail(KH_E_HIP, "n-body launch failed");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  if (sync) HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
+                            const double* const xyz[6], int64_t center, int32_t frame,
+                            const double* jd_final, int32_t jd_final_stride,
+                            const kh_nongrav* ng, double epsilon, int32_t mode,
+                            double* const out[6], int32_t* status, int64_t* impact_id,
+                            double* impact_jd, kh_counters* counters) {
+  Chain ch;
+  int32_t r = check_nbody_args(c, n, jd0, xyz, center, frame, jd_final, epsilon, mode, out, status,
+                               ch);
+  if (r) return r;
+  if (n == 0) return KH_OK;
+  if (ng)
+    for (int64_t i = 0; i < n; ++i)
+      if (ng[i].kind == 1 && ng[i].dt != 0.0)
+        return fail(KH_E_ARG, "JplComet non-grav with a time delay (dt != 0) is not supported");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t nf = jd_final_stride ? n : 1;
+  size_t bn = align256(n * 8), bf = align256(nf * 8), bng = align256(n * sizeof(kh_nongrav)),
+         bs = align256(n * 4), bc = align256(n * sizeof(kh_counters));
+  char* base;
+  r = staging(c, 7 * bn + bf + (ng ? bng : 0) + 6 * bn + bs + 2 * bn + bc, &base);
+  if (r) return r;
+  char* q = base;
+  double* d_jd0 = (double*)q;
+  q += bn;
+  const double* d_xyz[6];
+  for (int k = 0; k < 6; ++k, q += bn) d_xyz[k] = (double*)q;
+  double* d_jf = (double*)q;
+  q += bf;
+  kh_nongrav* d_ng = nullptr;
+  if (ng) {
+    d_ng = (kh_nongrav*)q;
+    q += bng;
+  }
+  double* d_out[6];
+  for (int k = 0; k < 6; ++k, q += bn) d_out[k] = (double*)q;
+  int32_t* d_st = (int32_t*)q;
+  q += bs;
+  int64_t* d_iid = (int64_t*)q;
+  q += bn;
+  double* d_ijd = (double*)q;
+  q += bn;
+  kh_counters* d_cnt = (kh_counters*)q;
+  HIPCHK(hipMemcpyAsync(d_jd0, jd0, n * 8, hipMemcpyHostToDevice, c->stream));
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync((void*)d_xyz[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_jf, jd_final, nf * 8, hipMemcpyHostToDevice, c->stream));
+  if (ng) HIPCHK(hipMemcpyAsync(d_ng, ng, n * sizeof(kh_nongrav), hipMemcpyHostToDevice, c->stream));
+  r = kh_propagate_n_body_dev(c, n, d_jd0, d_xyz, center, frame, d_jf, jd_final_stride, d_ng,
+                              epsilon, mode, d_out, d_st, d_iid, d_ijd, counters ? d_cnt : nullptr,
+                              0);
+  if (r) return r;
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], d_out[k], n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, d_st, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (impact_id) HIPCHK(hipMemcpyAsync(impact_id, d_iid, n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (impact_jd) HIPCHK(hipMemcpyAsync(impact_jd, d_ijd, n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (counters)
+    HIPCHK(hipMemcpyAsync(counters, d_cnt, n * sizeof(kh_counters), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+int32_t kh_last_launch(kh_ctx* c, int32_t* block, int32_t* grid, int32_t* mode) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (block) *block = c->last_block;
+  if (grid) *grid = c->last_grid;
+  if (mode) *mode = c->last_mode;
+  return KH_OK;
+}
+
+double kh_last_kernel_ms(kh_ctx* c) {
+  if (!c || !c->timed) return -1.0;
+  if (hipEventSynchronize(c->ev1) != hipSuccess) return -1.0;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) return -1.0;
+  return (double)ms;
+}
+
+}  // extern "C"

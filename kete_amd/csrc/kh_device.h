@@ -1,0 +1,502 @@
+// kh_device.h -- device-side building blocks shared by the kete_hip kernels:
+// constants, the SPK Type 2 table, Chebyshev evaluation, the universal Kepler
+// solver and the spk_accel force model.  FP64 throughout.
+//
+// Each routine restates kete 1.1.3 (Caltech-IPAC/kete; paths relative to
+// src/kete_core/src) and cites it.  The arithmetic keeps kete's operation
+// order; the compiler may contract a*b+c into FMA, which changes results at
+// the last-ulp level only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kh {
+
+// constants/gravity.rs:9-40, constants/universal.rs:11-23
+constexpr double GMS = 0.00029591220828411956;
+constexpr double GMS_SQRT = 0.01720209894996;
+constexpr double SUN_J2 = 2.2e-7;
+constexpr double EARTH_J2 = 0.00108262998905;
+constexpr double JUPITER_J2 = 0.014696572;
+constexpr double C_AU_PER_DAY = 173.14463267424034;
+constexpr double C_AU_PER_DAY_INV = 0.005775518331436995;
+constexpr double C_AU_PER_DAY_INV_SQUARED = 3.33566119967647e-05;
+constexpr double AU_KM = 149597870.7;
+constexpr double TAU = 6.283185307179586;
+constexpr double PARABOLIC_BETA = 1e-10;  // kepler.rs:120
+
+// Per-object status (errors.rs:14-32)
+enum : int32_t {
+  ST_OK = 0,
+  ST_CONVERGENCE = 1,
+  ST_VALUE_ERROR = 2,
+  ST_DAF_LIMITS = 3,
+  ST_UNKNOWN_FRAME = 4,
+  ST_IO_ERROR = 5,
+  ST_IMPACT = 6,
+};
+
+constexpr int MAX_SEGS = 24;
+constexpr int MAX_CHAIN = 4;
+constexpr int MAX_COEF = 32;
+
+struct SegDev {
+  double jd_start, jd_end;  // JD TDB, as kete converts the summary floats
+  double j0s;               // jd_to_spice_jd(jd_start)
+  double intlen;            // seconds per record
+  int64_t off;              // first double of record 0 in the coefficient array
+  int32_t n_rec, rsize, n_coef, pad;
+};
+
+// A chain of segments whose states sum to a body's (or center's) SSB state.
+struct Chain {
+  int32_t n;
+  int32_t seg[MAX_CHAIN];
+};
+
+// Bodies of kete's PLANETS list (gravity.rs:145-148) in its order.
+constexpr int N_PLANETS = 10;
+
+// One body of kete's PLANETS list (gravity.rs:43-101,145-148).
+// kind: 0 Newtonian only, 1 Newtonian + GR + ecliptic-frame J2 (Sun,
+// Jupiter), 2 Newtonian + equatorial J2 (Earth).
+struct BodyDev {
+  int32_t seg;  // segment of the body w.r.t. its center
+  int32_t emb;  // 1: center is the Earth-Moon barycentre segment
+  int32_t id;   // NAIF id
+  int32_t kind;
+  double mass, radius, j2, pad;
+};
+
+// Device-resident ephemeris description; kernels take a pointer to it, so the
+// per-body and per-segment fields are read with scalar loads.
+struct SpkDev {
+  const double* __restrict__ coef;
+  int32_t n_seg;
+  int32_t all14;  // every segment has 14 coefficients
+  int32_t emb_seg;
+  int32_t sun_seg;
+  SegDev seg[MAX_SEGS];
+  BodyDev body[N_PLANETS];
+};
+
+struct Vec3 {
+  double x, y, z;
+};
+__device__ __forceinline__ Vec3 v3(double a, double b, double c) { return Vec3{a, b, c}; }
+__device__ __forceinline__ double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double norm(Vec3 a) { return sqrt(dot(a, a)); }
+
+// Ecliptic <-> equatorial rotation about x by the obliquity
+// (frames/definitions.rs:135,155-158,224-237).
+constexpr double ROT_C = 0.9174820620691818;  // cos(0.40909280422232897)
+constexpr double ROT_S = 0.3977771559319137;  // sin(0.40909280422232897)
+__device__ __forceinline__ Vec3 ecl_to_equ(Vec3 v) {
+  return Vec3{v.x, ROT_C * v.y + (-ROT_S) * v.z, ROT_S * v.y + ROT_C * v.z};
+}
+__device__ __forceinline__ Vec3 equ_to_ecl(Vec3 v) {
+  return Vec3{v.x, ROT_C * v.y + ROT_S * v.z, (-ROT_S) * v.y + ROT_C * v.z};
+}
+
+// ---------------------------------------------------------------------------
+// SPK Type 2 (spk_segments.rs:150-177,355-376; interpolation.rs:25-77)
+// ---------------------------------------------------------------------------
+// Record lookup: returns the record pointer and the normalised time x.
+__device__ __forceinline__ int32_t seg_record(const SpkDev* __restrict__ spk, int s, double jd,
+                                              const double*& rec, double& x,
+                                              double& js) {
+  const SegDev& g = spk->seg[s];
+  if (jd < g.jd_start || jd > g.jd_end) return ST_DAF_LIMITS;
+  js = (jd - 2451545.0) * 86400.0;
+  double fi = floor((js - g.j0s) / g.intlen);
+  int32_t ri = fi > 0.0 ? (int32_t)fmin(fi, (double)(g.n_rec - 1)) : 0;
+  rec = spk->coef + g.off + (int64_t)ri * g.rsize;
+  x = (js - rec[0]) / rec[1];
+  return ST_OK;
+}
+
+// Position-only Chebyshev sum, forward T recurrence in ascending order.
+template <int N>
+__device__ __forceinline__ Vec3 cheb_pos_n(const double* __restrict__ c, double x) {
+  double x2 = 2.0 * x;
+  double vx = c[0] + c[1] * x, vy = c[N] + c[N + 1] * x, vz = c[2 * N] + c[2 * N + 1] * x;
+  double t2 = 1.0, t1 = x;
+#pragma unroll
+  for (int k = 2; k < N; ++k) {
+    double t = x2 * t1 - t2;
+    vx += c[k] * t;
+    vy += c[N + k] * t;
+    vz += c[2 * N + k] * t;
+    t2 = t1;
+    t1 = t;
+  }
+  return Vec3{vx, vy, vz};
+}
+
+__device__ __forceinline__ Vec3 cheb_pos_dyn(const double* __restrict__ c, int n, double x) {
+  double x2 = 2.0 * x;
+  double vx = c[0] + c[1] * x, vy = c[n] + c[n + 1] * x, vz = c[2 * n] + c[2 * n + 1] * x;
+  double t2 = 1.0, t1 = x;
+  for (int k = 2; k < n; ++k) {
+    double t = x2 * t1 - t2;
+    vx += c[k] * t;
+    vy += c[n + k] * t;
+    vz += c[2 * n + k] * t;
+    t2 = t1;
+    t1 = t;
+  }
+  return Vec3{vx, vy, vz};
+}
+
+// Position and derivative (interpolation.rs:25-77): der += c_k * U_{k-1} * k.
+__device__ __forceinline__ void cheb_both_dyn(const double* __restrict__ c, int n, double x,
+                                              Vec3& p, Vec3& d) {
+  double x2 = 2.0 * x;
+  double vx = c[0] + c[1] * x, vy = c[n] + c[n + 1] * x, vz = c[2 * n] + c[2 * n + 1] * x;
+  double dx = c[1], dy = c[n + 1], dz = c[2 * n + 1];
+  double t2 = 1.0, t1 = x, u2 = 1.0, u1 = x2;
+  for (int k = 2; k < n; ++k) {
+    double t = x2 * t1 - t2;
+    vx += c[k] * t;
+    vy += c[n + k] * t;
+    vz += c[2 * n + k] * t;
+    t2 = t1;
+    t1 = t;
+    double u = x2 * u1 - u2;
+    double kk = (double)k;
+    dx += c[k] * u1 * kk;
+    dy += c[n + k] * u1 * kk;
+    dz += c[2 * n + k] * u1 * kk;
+    u2 = u1;
+    u1 = u;
+  }
+  p = Vec3{vx, vy, vz};
+  d = Vec3{dx, dy, dz};
+}
+
+// Segment state in AU and AU/day (spk_segments.rs:355-376).
+__device__ __forceinline__ int32_t seg_pos(const SpkDev* __restrict__ spk, int s, double jd, Vec3& pos) {
+  const double* rec;
+  double x, js;
+  int32_t st = seg_record(spk, s, jd, rec, x, js);
+  if (st) return st;
+  Vec3 p = spk->all14 ? cheb_pos_n<14>(rec + 2, x) : cheb_pos_dyn(rec + 2, spk->seg[s].n_coef, x);
+  pos = Vec3{p.x / AU_KM, p.y / AU_KM, p.z / AU_KM};
+  return ST_OK;
+}
+
+__device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk, int s, double jd, Vec3& pos,
+                                               Vec3& vel) {
+  const double* rec;
+  double x, js;
+  int32_t st = seg_record(spk, s, jd, rec, x, js);
+  if (st) return st;
+  Vec3 p, d;
+  cheb_both_dyn(rec + 2, spk->seg[s].n_coef, x, p, d);
+  double sc = 86400.0 / rec[1] / AU_KM;
+  pos = Vec3{p.x / AU_KM, p.y / AU_KM, p.z / AU_KM};
+  vel = Vec3{d.x * sc, d.y * sc, d.z * sc};
+  return ST_OK;
+}
+
+// Sum of a chain of segment states (the center-chaining of spk.rs:99-126).
+__device__ __forceinline__ int32_t chain_state(const SpkDev* __restrict__ spk, const Chain& ch, double jd,
+                                               Vec3& pos, Vec3& vel) {
+  pos = Vec3{0, 0, 0};
+  vel = Vec3{0, 0, 0};
+  for (int k = 0; k < ch.n; ++k) {
+    Vec3 p, v;
+    int32_t st = seg_pos_vel(spk, ch.seg[k], jd, p, v);
+    if (st) return st;
+    if (k == 0) {
+      pos = p;
+      vel = v;
+    } else {
+      pos = Vec3{pos.x + p.x, pos.y + p.y, pos.z + p.z};
+      vel = Vec3{vel.x + v.x, vel.y + v.y, vel.z + v.z};
+    }
+  }
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Universal Kepler solver (kepler.rs:120-297, fitting/newton.rs:18-65)
+// ---------------------------------------------------------------------------
+struct KepEll {
+  double r0, rv0, beta, b, dt;
+  __device__ __forceinline__ double f(double x) const {
+    double g1 = sin(b * x) / b;
+    double sh = sin(b * x / 2.0);
+    double g2 = 2.0 * (sh * sh) / beta;
+    double g3 = (x - g1) / beta;
+    return r0 * g1 + rv0 * g2 + GMS * g3 - dt;
+  }
+  __device__ __forceinline__ double d(double x) const {
+    double s, c;
+    sincos(b * x, &s, &c);
+    double g3d = (1.0 - c) / beta;
+    return r0 * c + rv0 * s + GMS * g3d;
+  }
+};
+
+struct KepHyp {
+  double r0, rv0, beta, b, dt;
+  __device__ __forceinline__ double f(double x) const {
+    double g1 = sinh(b * x) / b;
+    double g2 = (1.0 - cosh(b * x)) / beta;
+    double g3 = (x - g1) / beta;
+    return r0 * g1 + rv0 * g2 + GMS * g3 - dt;
+  }
+  __device__ __forceinline__ double d(double x) const {
+    double sh = sinh(b * x), ch = cosh(b * x);
+    double g3d = (1.0 - ch) / beta;
+    return r0 * ch + rv0 * sh + GMS * g3d;
+  }
+};
+
+// newton_raphson (fitting/newton.rs:18-65); returns iterations used or -1.
+template <class F>
+__device__ __forceinline__ int newton(const F& fn, double start, double atol, double& out) {
+  double x = start;
+  if (fabs(fn.d(x)) < 1e-12) x += 0.1;
+  for (int it = 0; it < 100; ++it) {
+    double f = fn.f(x);
+    if (fabs(f) < atol) {
+      out = x;
+      return it + 1;
+    }
+    double d = fn.d(x);
+    if (fabs(d) < 1e-12) return -1;
+    if (!isfinite(d) || !isfinite(f)) return -1;
+    x -= 0.5 * f / d;
+    d = fn.d(x);
+    if (fabs(d) < 1e-3) f = fn.f(x);
+    x -= 0.5 * f / d;
+  }
+  return -1;
+}
+
+// solve_kepler_universal (kepler.rs:179-233).  Returns newton iterations
+// (0 for the closed-form parabolic branch) or -1 on failure.
+__device__ __forceinline__ int solve_kepler_universal(double dt, double r0, double v0, double rv0,
+                                                      double& s, double& beta) {
+  beta = 2.0 * GMS / r0 - v0 * v0;
+  double b = sqrt(fabs(beta));
+  if (beta >= PARABOLIC_BETA) {
+    double period = GMS * (1.0 / (b * b * b));
+    dt = fmod(dt, period * TAU);
+    KepEll k{r0, rv0, beta, b, dt};
+    double guess = (1.0 / b) * dt / period;
+    return newton(k, guess, 1e-11, s);
+  } else if (fabs(beta) < PARABOLIC_BETA) {
+    double a0 = -6.0 * dt / GMS;
+    double a1 = 6.0 * r0 / GMS;
+    double a2 = 3.0 * rv0 / GMS;
+    double p = (3.0 * a1 - a2 * a2) / 3.0;
+    double q = (9.0 * a1 * a2 - 27.0 * a0 - 2.0 * (a2 * a2 * a2)) / 27.0;
+    double w = pow(q / 2.0 + sqrt(q * q / 4.0 + (p * p * p) / 27.0), 1.0 / 3.0);
+    s = w - p / (3.0 * w) - a2 / 3.0;
+    return 0;
+  }
+  KepHyp k{r0, rv0, beta, b, dt};
+  return newton(k, GMS_SQRT * fabs(beta) * dt, 1e-11, s);
+}
+
+__device__ __forceinline__ double g_1(double s, double beta) {
+  if (fabs(beta) < PARABOLIC_BETA) return s;
+  double bs = sqrt(fabs(beta));
+  if (beta >= 0.0) return sin(bs * s) / bs;
+  return sinh(bs * s) / bs;
+}
+__device__ __forceinline__ double g_2(double s, double beta) {
+  if (fabs(beta) < PARABOLIC_BETA) return s * s / 2.0;
+  double bs = sqrt(fabs(beta)) * s;
+  if (beta >= 0.0) {
+    double sh = sin(bs / 2.0);
+    return 2.0 * (sh * sh) / beta;
+  }
+  return (1.0 - cosh(bs)) / beta;
+}
+
+// One universal-variable solve and f/g update (kepler.rs:265-290).
+__device__ __forceinline__ int kepler_once(double time, Vec3 p, Vec3 v, Vec3& po, Vec3& vo,
+                                           int& iters) {
+  double r0 = norm(p), v0 = norm(v), rv0 = dot(p, v);
+  double s, beta;
+  int it = solve_kepler_universal(time, r0, v0, rv0, s, beta);
+  if (it < 0) return -1;
+  iters += it;
+  double g1 = g_1(s, beta), g2 = g_2(s, beta);
+  double f = 1.0 - GMS / r0 * g2;
+  double g = r0 * g1 + rv0 * g2;
+  Vec3 np{p.x * f + v.x * g, p.y * f + v.y * g, p.z * f + v.z * g};
+  double nr0 = norm(np);
+  double fd = -GMS / (nr0 * r0) * g1;
+  double gd = 1.0 - (GMS / nr0) * g2;
+  vo = Vec3{p.x * fd + v.x * gd, p.y * fd + v.y * gd, p.z * fd + v.z * gd};
+  po = np;
+  return 0;
+}
+
+// analytic_2_body (kepler.rs:254-297).  kete recurses on failure into two
+// half steps (depth <= 10); this walks the same binary tree iteratively:
+// pending[d] counts half steps still to take at depth d.
+__device__ __forceinline__ int32_t analytic_2_body(double time, Vec3 pos, Vec3 vel, Vec3& po,
+                                                   Vec3& vo, int* iters_out = nullptr) {
+  int iters = 0;
+  int pending[11];
+  double dts[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) pending[k] = 0;
+  int depth = 1;  // depth after kete's increment of the top-level call
+  pending[1] = 1;
+  dts[1] = time;
+  Vec3 p = pos, v = vel;
+  while (depth > 0) {
+    if (pending[depth] == 0) {
+      --depth;
+      continue;
+    }
+    pending[depth]--;
+    double t = dts[depth];
+    if (fabs(t) < 1e-10) continue;
+    double rv0 = dot(p, v);
+    if (isnan(rv0) || isinf(rv0)) return ST_CONVERGENCE;
+    Vec3 np, nv;
+    if (kepler_once(t, p, v, np, nv, iters) == 0) {
+      p = np;
+      v = nv;
+      continue;
+    }
+    // failed: two half steps one level deeper (kete's depth check is
+    // `depth >= 10` on entry to the recursive call)
+    if (depth >= 10) return ST_CONVERGENCE;
+    ++depth;
+    pending[depth] = 2;
+    dts[depth] = 0.5 * t;
+  }
+  po = p;
+  vo = v;
+  if (iters_out) *iters_out = iters;
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Force model: spk_accel with PLANETS (acceleration.rs:121-168),
+// GravParams::add_acceleration (gravity.rs:171-253), JplComet non-grav
+// (nongrav.rs:114-157).
+// ---------------------------------------------------------------------------
+// Same layout as kh_nongrav in kete_hip.h.
+struct NonGravDev {
+  int32_t kind;  // 0 none, 1 JplComet, 2 Dust
+  double a1, a2, a3, alpha, r_0, m, n, k, dt, beta;
+};
+
+__device__ __forceinline__ double inv_cube(double r) { return 1.0 / (r * r * r); }
+
+// j2_correction (gravity.rs:223-235)
+__device__ __forceinline__ Vec3 j2_correction(Vec3 p, double radius, double j2, double mass) {
+  double r = norm(p);
+  double zr = p.z / r;
+  double z_squared = 5.0 * (zr * zr);
+  double rr = radius / r;
+  double coef = 1.5 * j2 * mass * (rr * rr) * inv_cube(r);
+  return Vec3{p.x * coef * (z_squared - 1.0), p.y * coef * (z_squared - 1.0),
+              p.z * coef * (z_squared - 3.0)};
+}
+
+// apply_gr_correction (gravity.rs:239-253)
+__device__ __forceinline__ void gr_correction(Vec3& a, Vec3 p, Vec3 v, double mass) {
+  double r_v = 4.0 * dot(p, v);
+  double v2 = dot(v, v);
+  double r = norm(p);
+  double gc = mass * C_AU_PER_DAY_INV_SQUARED * inv_cube(r);
+  double c = 4. * mass / r - v2;
+  a.x += gc * (c * p.x + r_v * v.x);
+  a.y += gc * (c * p.y + r_v * v.y);
+  a.z += gc * (c * p.z + r_v * v.z);
+}
+
+__device__ __forceinline__ void nongrav_add(const NonGravDev& ng, Vec3& a, Vec3 p, Vec3 v) {
+  if (ng.kind == 2) {
+    double n = norm(p);
+    Vec3 pn{p.x / n, p.y / n, p.z / n};
+    double r_dot = dot(pn, v);
+    double norm2_inv = 1.0 / dot(p, p);
+    double sc = GMS * ng.beta * norm2_inv;
+    double f1 = 1.0 - r_dot * C_AU_PER_DAY_INV_SQUARED;
+    a.x += sc * (f1 * pn.x - v.x * C_AU_PER_DAY_INV_SQUARED);
+    a.y += sc * (f1 * pn.y - v.y * C_AU_PER_DAY_INV_SQUARED);
+    a.z += sc * (f1 * pn.z - v.z * C_AU_PER_DAY_INV_SQUARED);
+  } else if (ng.kind == 1) {
+    double n = norm(p);
+    Vec3 pn{p.x / n, p.y / n, p.z / n};
+    double vd = dot(v, pn);
+    Vec3 t{v.x - pn.x * vd, v.y - pn.y * vd, v.z - pn.z * vd};
+    double tn = norm(t);
+    t = Vec3{t.x / tn, t.y / tn, t.z / tn};
+    Vec3 nv{t.y * pn.z - t.z * pn.y, t.z * pn.x - t.x * pn.z, t.x * pn.y - t.y * pn.x};
+    double nn = norm(nv);
+    nv = Vec3{nv.x / nn, nv.y / nn, nv.z / nn};
+    double rr0 = n / ng.r_0;
+    double scale = ng.alpha * pow(rr0, -ng.m) * pow(1.0 + pow(rr0, ng.n), -ng.k);
+    double s1 = scale * ng.a1, s2 = scale * ng.a2, s3 = scale * ng.a3;
+    a.x += pn.x * s1; a.y += pn.y * s1; a.z += pn.z * s1;
+    a.x += t.x * s2;  a.y += t.y * s2;  a.z += t.z * s2;
+    a.x += nv.x * s3; a.y += nv.y * s3; a.z += nv.z * s3;
+  }
+}
+
+// spk_accel: acceleration of a massless body at barycentric equatorial
+// (x, v) and TDB JD t.  On exact evaluations, r <= radius is an Impact.
+// The Earth-Moon barycentre is evaluated once and shared by 399 and 301
+// (kete evaluates it for each, with identical results).
+__device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk, double t, Vec3 x,
+                                             Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
+                                             int32_t& impact_id) {
+  Vec3 a{0.0, 0.0, 0.0};
+  Vec3 emb;
+  int32_t st = seg_pos(spk, spk->emb_seg, t, emb);
+  if (st) return st;
+#pragma unroll 1
+  for (int b = 0; b < N_PLANETS; ++b) {
+    const BodyDev& bd = spk->body[b];
+    const double mass = bd.mass;
+    Vec3 bp, bv{0.0, 0.0, 0.0};
+    if (bd.kind == 1) {
+      st = seg_pos_vel(spk, bd.seg, t, bp, bv);
+    } else {
+      st = seg_pos(spk, bd.seg, t, bp);
+      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+    }
+    if (st) return st;
+    Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
+    double r = norm(rp);
+    if (exact && r <= bd.radius) {
+      impact_id = bd.id;
+      return ST_IMPACT;
+    }
+    double k = mass * inv_cube(r);
+    a.x -= rp.x * k;
+    a.y -= rp.y * k;
+    a.z -= rp.z * k;
+    if (bd.kind == 1) {
+      Vec3 rv{v.x - bv.x, v.y - bv.y, v.z - bv.z};
+      gr_correction(a, rp, rv, mass);
+      Vec3 j = ecl_to_equ(j2_correction(equ_to_ecl(rp), bd.radius, bd.j2, mass));
+      a.x += j.x;
+      a.y += j.y;
+      a.z += j.z;
+      if (bd.id == 10 && ng != nullptr) nongrav_add(*ng, a, rp, rv);
+    } else if (bd.kind == 2) {
+      Vec3 j = j2_correction(rp, bd.radius, bd.j2, mass);
+      a.x += j.x;
+      a.y += j.y;
+      a.z += j.z;
+    }
+  }
+  acc = a;
+  return ST_OK;
+}
+
+}  // namespace kh

@@ -1,0 +1,435 @@
+// kh_nbody.hip -- Gauss-Radau 15 N-body propagation on gfx950.
+//
+// Restates RadauIntegrator::integrate/step (src/kete_core/src/propagation/
+// radau.rs:119-346) driving spk_accel (acceleration.rs:121-168) inside
+// propagate_n_body_spk (propagation/mod.rs:70-118).
+//
+// Layout and launch
+//  - One lane integrates one object.  Object state is structure-of-arrays in
+//    HBM (coalesced loads/stores at start and end); the integrator state
+//    (t, h, x, v, a, B[7][3], G[7][3]) lives in VGPRs for the whole arc.
+//  - Each lane runs kete's integrator as a state machine whose loop body makes
+//    exactly one force evaluation: the initial exact evaluation, one of the 7
+//    substeps of a predictor-corrector sweep, or the exact evaluation after an
+//    accepted step.  The force model (11 Chebyshev lookups plus the Newton,
+//    GR and J2 terms) is >90% of the arithmetic and runs with every live lane
+//    of the wavefront active, whatever step, sweep or substep each lane is on.
+//  - The bookkeeping around it is branch-free: every lane computes the
+//    candidate updates (G column, B = G*C, accepted x and v) and keeps them
+//    with selects.  Long-lived state modified under divergent branches would
+//    otherwise be held twice by the register allocator.
+//  - The Radau tables live in LDS and are indexed per lane by its substep.
+//  - Persistent mode: the grid is sized to the machine and a lane that
+//    finishes an object takes the next one from a device-side counter, so
+//    objects with very different step counts do not leave lanes idle.
+#include "kh_device.h"
+#include "kh_internal.h"
+
+namespace kh {
+
+// Radau tables (radau.rs:24-68), filled by the host from the same formulas.
+__constant__ RadauTables RT;
+
+// The same tables as compile-time constants, for the fixed-index uses (C, W,
+// U), which then become instruction literals instead of registers.
+constexpr RadauTables make_tables() {
+  RadauTables t{};
+  const double H[8] = {0.0, 0.05626256053692215, 0.18024069173689236, 0.3526247171131696,
+                       0.5471536263305554, 0.7342101772154105, 0.8853209468390958,
+                       0.9775206135612875};
+  for (int i = 0; i < 8; ++i) t.h[i] = H[i];
+  for (int i = 0; i < 7; ++i) {
+    t.w[i] = 1.0 / (double)((i + 2) * (i + 3));
+    t.u[i] = 1.0 / (double)(i + 2);
+  }
+  for (int i = 0; i < 7; ++i) t.c[i][i] = 1.0;
+  for (int i = 0; i < 7; ++i) {
+    if (i > 0) t.c[i][0] = -H[i] * t.c[i - 1][0];
+    for (int j = 1; j < i; ++j) t.c[i][j] = t.c[i - 1][j - 1] - H[i] * t.c[i - 1][j];
+  }
+  return t;
+}
+constexpr RadauTables RTC = make_tables();
+
+struct NBodyArgs {
+  int64_t n;
+  const double* jd0;
+  const double* xyz[6];
+  const double* jd_final;
+  int32_t jd_final_stride;
+  int32_t frame;  // 1 or 17
+  Chain center;   // segments whose sum is the center's SSB state
+  const NonGravDev* ng;
+  double epsilon;
+  double* out[6];
+  int32_t* status;
+  int64_t* impact_id;
+  double* impact_jd;
+  int32_t* counters;  // [n][4] or null
+  unsigned long long* queue;
+  int64_t first_dynamic;  // objects below this are assigned by lane id
+};
+
+enum : int32_t { PH_INIT = 0, PH_SUB = 1, PH_EXACT = 2 };
+
+// Convert the input state to SSB-centred equatorial (mod.rs:78-79 via
+// spk.rs:99-126 and state.rs:146-281): add the center's chain states in the
+// input frame, then rotate to the equatorial frame.
+__device__ __forceinline__ int32_t to_ssb_equatorial(const SpkDev* __restrict__ spk,
+                                                     const NBodyArgs& a, double jd, Vec3& x,
+                                                     Vec3& v) {
+  const bool ecl = a.frame == 17;
+  for (int k = 0; k < a.center.n; ++k) {
+    Vec3 p, w;
+    int32_t st = seg_pos_vel(spk, a.center.seg[k], jd, p, w);
+    if (st) return st;
+    if (ecl) {
+      p = equ_to_ecl(p);
+      w = equ_to_ecl(w);
+    }
+    x = Vec3{x.x + p.x, x.y + p.y, x.z + p.z};
+    v = Vec3{v.x + w.x, v.y + w.y, v.z + w.z};
+  }
+  if (ecl) {
+    x = ecl_to_equ(x);
+    v = ecl_to_equ(v);
+  }
+  return ST_OK;
+}
+
+// Back to the input center and frame (mod.rs:115-116): subtract the chain
+// from the root down (the find_path order from 0), then rotate.
+__device__ __forceinline__ int32_t from_ssb_equatorial(const SpkDev* __restrict__ spk,
+                                                       const NBodyArgs& a, double jd, Vec3& x,
+                                                       Vec3& v) {
+  for (int k = a.center.n - 1; k >= 0; --k) {
+    Vec3 p, w;
+    int32_t st = seg_pos_vel(spk, a.center.seg[k], jd, p, w);
+    if (st) return st;
+    x = Vec3{x.x + (-p.x), x.y + (-p.y), x.z + (-p.z)};
+    v = Vec3{v.x + (-w.x), v.y + (-w.y), v.z + (-w.z)};
+  }
+  if (a.frame == 17) {
+    x = equ_to_ecl(x);
+    v = equ_to_ecl(v);
+  }
+  return ST_OK;
+}
+
+__device__ __forceinline__ void store_result(const NBodyArgs& a, int64_t i, int32_t st, Vec3 x,
+                                             Vec3 v, int32_t impact, double impact_jd,
+                                             int4 cnt) {
+  const double nan = __builtin_nan("");
+  if (st != ST_OK) x = v = Vec3{nan, nan, nan};
+  a.out[0][i] = x.x;
+  a.out[1][i] = x.y;
+  a.out[2][i] = x.z;
+  a.out[3][i] = v.x;
+  a.out[4][i] = v.y;
+  a.out[5][i] = v.z;
+  a.status[i] = st;
+  if (a.impact_id) a.impact_id[i] = st == ST_IMPACT ? impact : 0;
+  if (a.impact_jd) a.impact_jd[i] = st == ST_IMPACT ? impact_jd : nan;
+  if (a.counters) reinterpret_cast<int4*>(a.counters)[i] = cnt;
+}
+
+// Fetch objects until one starts cleanly (non-finite input and DAFLimits at
+// the center conversion finish immediately, as the pyo3 binding returns a NaN
+// state for them, src/kete/rust/propagation.rs:113-121).  Returns the object
+// index (>= n when none is left) and its starting state.
+template <bool PERSISTENT>
+__device__ __forceinline__ int64_t fetch(const SpkDev* __restrict__ spk, const NBodyArgs& a,
+                                         int64_t i, double& t, double& tf, Vec3& x, Vec3& v) {
+  while (i < a.n) {
+    t = a.jd0[i];
+    tf = a.jd_final[a.jd_final_stride ? i : 0];
+    x = Vec3{a.xyz[0][i], a.xyz[1][i], a.xyz[2][i]};
+    v = Vec3{a.xyz[3][i], a.xyz[4][i], a.xyz[5][i]};
+    bool finite = isfinite(t) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) &&
+                  isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
+    int32_t st = finite ? to_ssb_equatorial(spk, a, t, x, v) : ST_VALUE_ERROR;
+    if (st == ST_OK) return i;
+    store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+    if (!PERSISTENT) return a.n;
+    i = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
+  }
+  return i;
+}
+
+// LDS copy of the per-substep factors, indexed by each lane's substep j.
+struct LdsTables {
+  double h[8];
+  double inv_h[8];
+  double rdiff[8][8];
+};
+
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restrict__ spk,
+                                                          NBodyArgs a) {
+  __shared__ LdsTables lds;
+  for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
+    const int r = k / 8, c = k % 8;
+    lds.rdiff[r][c] = RT.rdiff[r][c];
+    if (r == 0) {
+      lds.h[c] = RT.h[c];
+      lds.inv_h[c] = RT.inv_h[c];
+    }
+  }
+  __syncthreads();
+
+  const double eps = a.epsilon;
+  int64_t obj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double t = 0.0, tf = 0.0, h = 0.0, h_next = 0.0;
+  Vec3 x{0, 0, 0}, v{0, 0, 0}, a0{0, 0, 0};
+  double B[7][3], G[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
+  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
+  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
+  obj = fetch<PERSISTENT>(spk, a, obj, t, tf, x, v);
+
+  while (obj < a.n) {
+    const bool sub = phase == PH_SUB;
+    // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
+    const double s = lds.h[j];
+    double xe[3], ve[3];
+    {
+      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+      // bw = sum_k B_k s^(k+1) w_k, bu = sum_k B_k s^(k+1) u_k, k ascending
+      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double sp = 1.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        sp *= s;
+        const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bw[d] += B[k][d] * wk;
+          bu[d] += B[k][d] * uk;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
+        double pv = vs[d] + s * h * (as[d] + bu[d]);
+        xe[d] = sub ? px : xs[d];
+        ve[d] = sub ? pv : vs[d];
+      }
+    }
+    const double te = sub ? t + s * h : t;
+
+    // ---- the one force evaluation of this iteration ----
+    Vec3 acc;
+    int32_t impact = 0;
+    n_force++;
+    const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
+    int32_t st = spk_accel(spk, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
+                           ngp, acc, impact);
+    const double ev[3] = {acc.x, acc.y, acc.z};
+
+    // ---- G column j (radau.rs:292-308), kept only on substeps ----
+    {
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double gk[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = lds.rdiff[j][z];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const bool put = sub && q == j - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
+      }
+    }
+    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
+    const bool es = sub && j == 7 && st == ST_OK;
+    double fmx[3], rmax = 0.0;
+    {
+      double Bn[7][3];
+#pragma unroll
+      for (int c = 0; c < 7; ++c) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double acc2 = G[c][d];
+#pragma unroll
+          for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
+          Bn[c][d] = acc2;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        fmx[d] = fabs(ev[d]) + 1e-6;
+        double r = fabs(Bn[6][d] - B[6][d]) / fmx[d];
+        rmax = d == 0 ? r : fmax(rmax, r);
+      }
+#pragma unroll
+      for (int c = 0; c < 7; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) B[c][d] = es ? Bn[c][d] : B[c][d];
+    }
+    const bool conv = es && rmax < 1e-14;
+    // ---- accepted step (radau.rs:320-338) ----
+    {
+      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double bw = 0.0, bu = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          bw += B[k][d] * RTC.w[k];
+          bu += B[k][d] * RTC.u[k];
+        }
+        double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
+        double nv = vs[d] + h * (as[d] + bu);
+        xs[d] = conv ? nx : xs[d];
+        vs[d] = conv ? nv : vs[d];
+      }
+      x = Vec3{xs[0], xs[1], xs[2]};
+      v = Vec3{vs[0], vs[1], vs[2]};
+    }
+    if (conv) {
+      // next step size (radau.rs:339-342)
+      double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
+      double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
+      double ratio = pow(eps * fm / bm, 1.0 / 7.0);
+      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
+      h_next = h * ratio;
+    }
+    t = conv ? t + h : t;
+    const bool take_a = st == ST_OK && !sub;
+    a0 = Vec3{take_a ? acc.x : a0.x, take_a ? acc.y : a0.y, take_a ? acc.z : a0.z};
+
+    // ---- scalar control (radau.rs:160-218) ----
+    int32_t done = -1;
+    bool begin = false;
+    if (st != ST_OK) {
+      done = st;  // Impact and DAFLimits abort the integration (radau.rs:207-208)
+    } else if (phase == PH_INIT) {
+      if (fabs(tf - t) < 1e-10) {
+        done = ST_OK;
+      } else {
+        h = copysign(0.1, tf - t);
+        begin = true;
+      }
+    } else if (phase == PH_EXACT) {
+      h = h_next;
+      if (fabs(t - tf) < 1e-12) {
+        done = ST_OK;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    } else if (j < 7) {
+      j = j + 1;
+    } else if (conv) {
+      n_acc++;
+      phase = PH_EXACT;
+    } else if (sweep + 1 < 10) {
+      sweep++;
+      j = 1;
+      n_sweep++;
+    } else {
+      // the step failed to converge (radau.rs:345): retry at 0.7 h
+      n_fail++;
+      fails++;
+      h *= 0.7;
+      if (fails > 10) {
+        done = ST_CONVERGENCE;
+      } else {
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    }
+    if (begin) {
+      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      sweep = 0;
+      j = 1;
+      n_sweep++;
+      phase = PH_SUB;
+    }
+
+    // ---- finished objects: store, then take the next one ----
+    if (done >= 0) {
+      Vec3 xo = x, vo = v;
+      int32_t fst = done;
+      if (fst == ST_OK) fst = from_ssb_equatorial(spk, a, tf, xo, vo);
+      store_result(a, obj, fst, xo, vo, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      int64_t nxt = a.n;
+      if (PERSISTENT) nxt = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
+      double nt, ntf;
+      Vec3 nx, nv;
+      obj = fetch<PERSISTENT>(spk, a, nxt, nt, ntf, nx, nv);
+      t = nt;
+      tf = ntf;
+      x = nx;
+      v = nv;
+      phase = PH_INIT;
+      j = 1;
+      sweep = 0;
+      fails = 0;
+      n_force = n_sweep = n_acc = n_fail = 0;
+    }
+    const bool fresh = done >= 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) B[k][d] = fresh ? 0.0 : B[k][d];
+  }
+}
+
+__global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+
+int32_t launch_nbody(const SpkDev* spk, const NBodyLaunch& p, hipStream_t stream,
+                     int32_t* used_block, int32_t* used_grid, int32_t* used_mode) {
+  NBodyArgs a;
+  a.n = p.n;
+  a.jd0 = p.jd0;
+  for (int k = 0; k < 6; ++k) {
+    a.xyz[k] = p.xyz[k];
+    a.out[k] = p.out[k];
+  }
+  a.jd_final = p.jd_final;
+  a.jd_final_stride = p.jd_final_stride;
+  a.frame = p.frame;
+  a.center = p.center;
+  a.ng = reinterpret_cast<const NonGravDev*>(p.ng);
+  a.epsilon = p.epsilon;
+  a.status = p.status;
+  a.impact_id = p.impact_id;
+  a.impact_jd = p.impact_jd;
+  a.counters = p.counters;
+  a.queue = p.queue;
+  const int block = p.block > 0 ? p.block : 256;
+  int64_t grid;
+  if (p.mode == 1) {
+    grid = p.grid > 0 ? p.grid : 1024;
+    if (grid * block > p.n) grid = (p.n + block - 1) / block;
+    a.first_dynamic = grid * block;
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
+    hipLaunchKernelGGL(radau_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), 0, stream,
+                       spk, a);
+  } else {
+    grid = (p.n + block - 1) / block;
+    a.first_dynamic = p.n;
+    hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), 0, stream,
+                       spk, a);
+  }
+  if (used_block) *used_block = block;
+  if (used_grid) *used_grid = (int32_t)grid;
+  if (used_mode) *used_mode = p.mode == 1 ? 1 : 0;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int32_t upload_radau_tables(const RadauTables& t) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(RT), &t, sizeof(RadauTables)) == hipSuccess ? 0 : -1;
+}
+
+}  // namespace kh

@@ -1,0 +1,284 @@
+"""ctypes binding of the oracle: a CPU restatement of kete's orbit-propagation path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the parity checker and the CPU
+baseline.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import it; the product (``kete_amd``)
+never does.
+
+The C source (``kete_oracle.c``) cites the kete Rust it restates line by line.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkete_oracle.so")
+
+OK, CONVERGENCE, VALUE_ERROR, DAF_LIMITS, UNKNOWN_FRAME, IO_ERROR, IMPACT = range(7)
+FRAME_EQUATORIAL, FRAME_ECLIPTIC = 1, 17
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -O3 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+class Counters(C.Structure):
+    _fields_ = [("n_force", C.c_int64), ("n_sweep", C.c_int64),
+                ("n_acc", C.c_int64), ("n_fail", C.c_int64)]
+
+
+class NonGrav(C.Structure):
+    _fields_ = [("kind", C.c_int32)] + [(k, C.c_double) for k in
+                                        ("a1", "a2", "a3", "alpha", "r_0", "m", "n", "k", "dt", "beta")]
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ptrs(cols):
+    arr = (C.POINTER(C.c_double) * len(cols))()
+    for i, c in enumerate(cols):
+        arr[i] = _d(c)
+    return arr
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        D, P, I64, I32 = C.c_double, C.POINTER(C.c_double), C.c_int64, C.c_int32
+        PP = C.POINTER(P)
+        L.ko_const.restype = D
+        L.ko_const.argtypes = [C.c_char_p]
+        L.ko_newton_square_root_test.argtypes = [D, D, P]
+        L.ko_analytic_2_body.argtypes = [D, P, P, P, P]
+        L.ko_compute_eccentric_anomaly.argtypes = [D, D, D, P]
+        L.ko_elements_to_state.argtypes = [D, D, D, D, D, D, D, P, P]
+        L.ko_spk_load_file.argtypes = [C.c_char_p]
+        L.ko_spk_get_state.argtypes = [I64, D, I64, I32, P, P]
+        L.ko_chebyshev_evaluate_both.argtypes = [D, P, P, P, C.c_int, P, P]
+        L.ko_spk_accel.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, P, C.POINTER(I64)]
+        L.ko_radau_central.argtypes = [D, D, P, P, P, P, C.POINTER(Counters)]
+        L.ko_batch_two_body.argtypes = [I64, P, PP, I64, I32, I64, P, P, PP,
+                                        C.POINTER(I32), C.c_int]
+        L.ko_batch_n_body.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.POINTER(NonGrav),
+                                      PP, C.POINTER(I32), C.POINTER(I64), P,
+                                      C.POINTER(Counters), C.c_int]
+        L.ko_rect_normals.argtypes = [P, D, D, D, P]
+        L.ko_rect_contains.argtypes = [P, P, P]
+        L.ko_fov_check_visible.restype = I64
+        L.ko_fov_check_visible.argtypes = [P, D, P, P, I32, I64, P, PP, D,
+                                           C.POINTER(I64), PP, C.c_int]
+        L.ko_fov_state_check.restype = I64
+        L.ko_fov_state_check.argtypes = [I64, D, PP, I32, I64, P, P, P, D, I64,
+                                         C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+        L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
+        L.ko_hg_apparent_mag.restype = D
+        L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
+        L.ko_hg_phase_curve_correction.restype = D
+        L.ko_hg_phase_curve_correction.argtypes = [D, D]
+        _lib = L
+    return _lib
+
+
+def const(name: str) -> float:
+    return lib().ko_const(name.encode())
+
+
+def _v3(x):
+    return np.ascontiguousarray(x, dtype=np.float64).reshape(3)
+
+
+def newton_square_root(start=0.0, atol=1e-10):
+    out = np.zeros(1)
+    st = lib().ko_newton_square_root_test(start, atol, _d(out))
+    return st, out[0]
+
+
+def analytic_2_body(dt, pos, vel):
+    p, v = _v3(pos), _v3(vel)
+    po, vo = np.zeros(3), np.zeros(3)
+    st = lib().ko_analytic_2_body(dt, _d(p), _d(v), _d(po), _d(vo))
+    return st, po, vo
+
+
+def eccentric_anomaly(ecc, mean_anom, peri_dist):
+    out = np.zeros(1)
+    st = lib().ko_compute_eccentric_anomaly(ecc, mean_anom, peri_dist, _d(out))
+    return st, out[0]
+
+
+def elements_to_state(epoch, ecc, incl, lon_asc, peri_arg, peri_dist, peri_time):
+    p, v = np.zeros(3), np.zeros(3)
+    st = lib().ko_elements_to_state(epoch, ecc, incl, lon_asc, peri_arg, peri_dist,
+                                    peri_time, _d(p), _d(v))
+    return st, p, v
+
+
+def spk_load(path: str) -> int:
+    lib().ko_spk_reset()
+    return lib().ko_spk_load_file(path.encode())
+
+
+def spk_state(naif_id, jd, center=0, frame=FRAME_EQUATORIAL):
+    p, v = np.zeros(3), np.zeros(3)
+    st = lib().ko_spk_get_state(naif_id, jd, center, frame, _d(p), _d(v))
+    return st, p, v
+
+
+def chebyshev_both(x, cx, cy, cz):
+    cx, cy, cz = (np.ascontiguousarray(c, dtype=np.float64) for c in (cx, cy, cz))
+    val, der = np.zeros(3), np.zeros(3)
+    st = lib().ko_chebyshev_evaluate_both(x, _d(cx), _d(cy), _d(cz), len(cx), _d(val), _d(der))
+    return st, val, der
+
+
+def spk_accel(t, pos, vel, exact=False, nongrav=None):
+    p, v = _v3(pos), _v3(vel)
+    a = np.zeros(3)
+    iid = C.c_int64(0)
+    ng = C.byref(nongrav) if nongrav is not None else None
+    st = lib().ko_spk_accel(t, _d(p), _d(v), ng, int(exact), _d(a), C.byref(iid))
+    return st, a, iid.value
+
+
+def radau_central(t0, t1, pos, vel):
+    p, v = _v3(pos), _v3(vel)
+    po, vo = np.zeros(3), np.zeros(3)
+    cnt = Counters()
+    st = lib().ko_radau_central(t0, t1, _d(p), _d(v), _d(po), _d(vo), C.byref(cnt))
+    return st, po, vo, cnt
+
+
+def _soa(xyz):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    assert xyz.ndim == 2 and xyz.shape[0] == 6
+    return xyz
+
+
+def batch_two_body(jd0, xyz, targets, center=10, frame=FRAME_ECLIPTIC,
+                   observer_pos=None, n_threads=None):
+    """propagate_two_body over n objects x m target epochs -> (out[6,m,n], status[m,n])."""
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    jd0 = np.ascontiguousarray(np.broadcast_to(jd0, (n,)), dtype=np.float64)
+    targets = np.ascontiguousarray(np.atleast_1d(targets), dtype=np.float64)
+    m = len(targets)
+    out = np.empty((6, m, n))
+    status = np.empty((m, n), dtype=np.int32)
+    obs = None if observer_pos is None else _d(_v3(observer_pos).copy())
+    obs_keep = None if observer_pos is None else _v3(observer_pos).copy()
+    if obs_keep is not None:
+        obs = _d(obs_keep)
+    lib().ko_batch_two_body(n, _d(jd0), _ptrs(list(xyz)), center, frame, m, _d(targets), obs,
+                            _ptrs([out[k] for k in range(6)]),
+                            status.ctypes.data_as(C.POINTER(C.c_int32)),
+                            n_threads or os.cpu_count())
+    return out, status
+
+
+def batch_n_body(jd0, xyz, jd_final, center=10, frame=FRAME_ECLIPTIC, nongravs=None,
+                 n_threads=None):
+    """propagate_n_body over n objects -> dict(out[6,n], status, impact_id, impact_jd, counters)."""
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    jd0 = np.ascontiguousarray(np.broadcast_to(jd0, (n,)), dtype=np.float64)
+    jf = np.ascontiguousarray(np.atleast_1d(jd_final), dtype=np.float64)
+    stride = 0 if len(jf) == 1 else 1
+    if stride:
+        assert len(jf) == n
+    out = np.empty((6, n))
+    status = np.empty(n, dtype=np.int32)
+    iid = np.empty(n, dtype=np.int64)
+    ijd = np.empty(n)
+    cnt = (Counters * max(n, 1))()
+    ng = None
+    if nongravs is not None:
+        ng = (NonGrav * n)(*nongravs)
+    lib().ko_batch_n_body(n, _d(jd0), _ptrs(list(xyz)), center, frame, _d(jf), stride, ng,
+                          _ptrs([out[k] for k in range(6)]),
+                          status.ctypes.data_as(C.POINTER(C.c_int32)),
+                          iid.ctypes.data_as(C.POINTER(C.c_int64)), _d(ijd), cnt,
+                          n_threads or os.cpu_count())
+    counters = np.array([(c.n_force, c.n_sweep, c.n_acc, c.n_fail) for c in cnt[:n]],
+                        dtype=np.int64).reshape(n, 4)
+    return dict(out=out, status=status, impact_id=iid, impact_jd=ijd, counters=counters)
+
+
+def rect_normals(pointing, rotation, lon_width, lat_width):
+    p = _v3(pointing)
+    out = np.zeros(12)
+    lib().ko_rect_normals(_d(p), rotation, lon_width, lat_width, _d(out))
+    return out.reshape(4, 3)
+
+
+def rect_contains(normals, v):
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(12)
+    vv = _v3(v)
+    dist = np.zeros(1)
+    inside = lib().ko_rect_contains(_d(nrm), _d(vv), _d(dist))
+    return bool(inside), dist[0]
+
+
+def fov_check_visible(normals, obs_jd, obs_pos, obs_vel, jd, xyz, dt_limit=3.0,
+                      frame=FRAME_ECLIPTIC, n_threads=None):
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    jd = np.ascontiguousarray(np.broadcast_to(jd, (n,)), dtype=np.float64)
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(12)
+    idx = np.empty(max(n, 1), dtype=np.int64)
+    vs = np.empty((7, max(n, 1)))
+    nv = lib().ko_fov_check_visible(_d(nrm), obs_jd, _d(_v3(obs_pos)), _d(_v3(obs_vel)),
+                                    frame, n, _d(jd), _ptrs(list(xyz)), dt_limit,
+                                    idx.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    _ptrs([vs[k] for k in range(7)]),
+                                    n_threads or os.cpu_count())
+    return idx[:nv].copy(), vs[:, :nv].copy()
+
+
+def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
+                    frame=FRAME_ECLIPTIC, cap=None, n_threads=None):
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    field_jd = np.ascontiguousarray(field_jd, dtype=np.float64)
+    nf = len(field_jd)
+    obs_xyz = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(nf, 6)
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(nf, 12)
+    cap = cap or max(1024, n * 4)
+    hf = np.empty(cap, dtype=np.int64)
+    ho = np.empty(cap, dtype=np.int64)
+    hs = np.empty((cap, 7))
+    nh = lib().ko_fov_state_check(n, jd0, _ptrs(list(xyz)), frame, nf, _d(field_jd),
+                                  _d(obs_xyz), _d(nrm), dt_limit, cap,
+                                  hf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  ho.ctypes.data_as(C.POINTER(C.c_int64)), _d(hs),
+                                  n_threads or os.cpu_count())
+    if nh < 0:
+        raise RuntimeError("hit capacity exceeded")
+    return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()
+
+
+def ra_dec_with_rates(pos, vel, obs_pos, obs_vel):
+    out = np.zeros(4)
+    lib().ko_ra_dec_with_rates(_d(_v3(pos)), _d(_v3(vel)), _d(_v3(obs_pos)),
+                               _d(_v3(obs_vel)), _d(out))
+    return out
+
+
+def hg_apparent_mag(h, g, sun2obj, sun2obs):
+    return lib().ko_hg_apparent_mag(h, g, _d(_v3(sun2obj)), _d(_v3(sun2obs)))
+
+
+def hg_phase_curve_correction(g, phase):
+    return lib().ko_hg_phase_curve_correction(g, phase)

@@ -1,0 +1,165 @@
+"""GPU parity of the Radau-15 N-body path (propagate_n_body) against the
+oracle, on the synthetic ephemeris.
+
+Tolerance: 1e-9 AU in position after a 1-year arc,
+1e-11 AU/day in velocity.  The integrator's adaptive decisions depend on
+last-ulp differences (FMA contraction, device libm), so step counts may
+differ by a step on a few objects; the tolerance covers that.
+"""
+import numpy as np
+import pytest
+
+import kete_amd as ka
+from kete_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-9
+VEL_TOL = 1e-11
+JD0 = syn.EPOCH_JD
+
+
+@pytest.fixture(scope="module")
+def mix():
+    xyz = syn.n_body_workload(n=1200, neo_fraction=0.25, seed_index=2)
+    return xyz
+
+
+def run_both(ctx, oracle_spk, xyz, jd_final, mode=-1, frame=ka.Frames.Ecliptic, center=10,
+             non_gravs=None, ng_oracle=None):
+    n = xyz.shape[1]
+    batch = ka.StateArray(np.full(n, JD0), xyz, frame, center)
+    g = ctx.n_body(batch, jd_final, non_gravs=non_gravs, mode=mode, counters=True)
+    r = oracle_spk.batch_n_body(JD0, xyz, jd_final, center=center, frame=frame,
+                                nongravs=ng_oracle)
+    return g, r
+
+
+def test_parity_one_year(ctx, oracle_spk, mix):
+    g, r = run_both(ctx, oracle_spk, mix, JD0 + 365.25)
+    assert np.array_equal(g["status"], r["status"])
+    assert np.all(g["status"] == 0)
+    dp = np.abs(g["out"][:3] - r["out"][:3]).max(axis=0)
+    dv = np.abs(g["out"][3:] - r["out"][3:]).max(axis=0)
+    assert dp.max() < POS_TOL, (dp.max(), int(dp.argmax()))
+    assert dv.max() < VEL_TOL, dv.max()
+    # the same algorithm: work counters agree object by object on nearly all
+    gc = g["counters"].astype(np.int64)
+    rc = r["counters"]
+    same = np.all(gc[:, :3] == rc[:, :3], axis=1)
+    assert same.mean() > 0.97, same.mean()
+    assert abs(gc[:, 0].sum() / rc[:, 0].sum() - 1) < 0.01
+
+
+def test_plain_and_persistent_identical(ctx, mix):
+    n = mix.shape[1]
+    batch = ka.StateArray(np.full(n, JD0), mix)
+    a = ctx.n_body(batch, JD0 - 200.0, mode=0)
+    b = ctx.n_body(batch, JD0 - 200.0, mode=1)
+    assert np.array_equal(a["status"], b["status"])
+    assert np.array_equal(a["out"], b["out"])
+
+
+def test_uneven_spans_parity(ctx, oracle_spk):
+    xyz = syn.n_body_workload(n=400, neo_fraction=0.1, seed_index=3)
+    span = syn.uneven_spans(400) / 4.0  # keep the oracle fast: up to 912 d
+    jf = JD0 + span
+    g, r = run_both(ctx, oracle_spk, xyz, jf, mode=1)
+    assert np.array_equal(g["status"], r["status"])
+    ok = g["status"] == 0
+    dp = np.abs(g["out"][:3, ok] - r["out"][:3, ok]).max()
+    # tolerance scales with arc length beyond one year
+    assert dp < POS_TOL * 3, dp
+
+
+def test_frames_and_centers(ctx, oracle_spk, mix):
+    xyz = mix[:, :200].copy()
+    # equatorial input
+    c, s = np.cos(0.40909280422232897), np.sin(0.40909280422232897)
+    eq = xyz.copy()
+    for k in (0, 3):
+        eq[k + 1] = c * xyz[k + 1] - s * xyz[k + 2]
+        eq[k + 2] = s * xyz[k + 1] + c * xyz[k + 2]
+    g, r = run_both(ctx, oracle_spk, eq, JD0 + 100.0, frame=ka.Frames.Equatorial)
+    assert np.all(g["status"] == 0) and np.array_equal(g["status"], r["status"])
+    assert np.abs(g["out"][:3] - r["out"][:3]).max() < POS_TOL
+    # SSB-centred input (center 0)
+    sun, _ = ctx.spk_state(10, [JD0], 0, ka.Frames.Ecliptic)
+    x0 = xyz + sun[:, :1]
+    g, r = run_both(ctx, oracle_spk, x0, JD0 + 100.0, center=0)
+    assert np.all(g["status"] == 0) and np.array_equal(g["status"], r["status"])
+    assert np.abs(g["out"][:3] - r["out"][:3]).max() < POS_TOL
+    # geocentric input (center 399, two-segment chain through the EMB)
+    earth, _ = ctx.spk_state(399, [JD0], 10, ka.Frames.Ecliptic)
+    xg = xyz - earth[:, :1]
+    g, r = run_both(ctx, oracle_spk, xg, JD0 + 50.0, center=399)
+    assert np.all(g["status"] == 0) and np.array_equal(g["status"], r["status"])
+    assert np.abs(g["out"][:3] - r["out"][:3]).max() < POS_TOL
+
+
+def test_error_semantics(ctx, oracle_spk, mix):
+    xyz = mix[:, :6].copy()
+    xyz[0, 1] = np.nan                    # non-finite input -> ValueError, NaN out
+    xyz[:3, 2] = [1e-3, 0.0, 0.0]         # inside the Sun -> Impact(10) at the first eval
+    xyz[3:, 2] = [0.0, 0.0, 0.0]
+    g, r = run_both(ctx, oracle_spk, xyz, JD0 + 30.0)
+    assert np.array_equal(g["status"], r["status"])
+    assert g["status"][1] == 2 and np.all(np.isnan(g["out"][:, 1]))
+    assert g["status"][2] == 6 and g["impact_id"][2] == 10
+    assert r["impact_id"][2] == 10
+    # beyond the ephemeris -> DAFLimits
+    g, r = run_both(ctx, oracle_spk, mix[:, :4].copy(), JD0 + 5000.0)
+    assert np.all(g["status"] == 3) and np.array_equal(g["status"], r["status"])
+    assert np.all(np.isnan(g["out"]))
+    # zero-length arc returns the input (radau.rs:183-189)
+    g, r = run_both(ctx, oracle_spk, mix[:, :4].copy(), JD0)
+    assert np.all(g["status"] == 0)
+    assert np.abs(g["out"] - mix[:, :4]).max() < 1e-15
+
+
+def test_impact_during_arc(ctx, oracle_spk):
+    """An object aimed at the Earth impacts on an exact evaluation."""
+    earth, _ = ctx.spk_state(399, [JD0 + 20.0], 10, ka.Frames.Ecliptic)
+    e0, _ = ctx.spk_state(399, [JD0], 10, ka.Frames.Ecliptic)
+    # start 0.02 AU away on a line towards where the Earth will be in 20 d
+    d = earth[:3, 0] - e0[:3, 0]
+    start = e0[:3, 0] - 0.01 * d / np.linalg.norm(d)
+    vel = (earth[:3, 0] - start) / 20.0
+    xyz = np.concatenate([start, vel]).reshape(6, 1)
+    g, r = run_both(ctx, oracle_spk, xyz, JD0 + 40.0)
+    assert r["status"][0] == g["status"][0]
+    if r["status"][0] == 6:
+        assert g["impact_id"][0] == r["impact_id"][0]
+        assert abs(g["impact_jd"][0] - r["impact_jd"][0]) < 1e-6
+
+
+def test_nongrav_models(ctx, oracle_spk, mix):
+    import oracle
+    xyz = mix[:, -100:].copy()
+    rng = np.random.default_rng(11)
+    models, ng_o = [], []
+    for i in range(100):
+        if i % 2:
+            a = rng.normal(0, 1e-8, 3)
+            models.append(ka.NonGravModel.new_asteroid(*a))
+            ng_o.append(oracle.NonGrav(1, a[0], a[1], a[2], 1.0, 1.0, 2.0, 1.0, 0.0, 0.0, 0.0))
+        else:
+            models.append(ka.NonGravModel.new_dust(0.01))
+            ng_o.append(oracle.NonGrav(2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0.01))
+    g, r = run_both(ctx, oracle_spk, xyz, JD0 + 200.0, non_gravs=models, ng_oracle=ng_o)
+    assert np.all(g["status"] == 0) and np.array_equal(g["status"], r["status"])
+    assert np.abs(g["out"][:3] - r["out"][:3]).max() < POS_TOL
+
+
+def test_python_surface(ctx):
+    states = [ka.State(f"obj{i}", JD0, [2.0 + 0.1 * i, 0.1, 0.0], [0.0, 0.011, 0.0005])
+              for i in range(5)]
+    out = ka.propagate_n_body(states, JD0 + 10.0)
+    assert len(out) == 5 and all(s.jd == JD0 + 10.0 for s in out)
+    assert out[2].desig == "obj2" and out[2].is_finite
+    out2 = ka.propagate_two_body(states, JD0 + 10.0)
+    assert np.linalg.norm(out2[0].pos - out[0].pos) < 1e-4
+    bad = [ka.State("x", JD0, [np.nan, 0, 0], [0, 0, 0])]
+    with pytest.raises(ValueError):
+        ka.propagate_n_body(bad, JD0 + 1.0, suppress_errors=False)
+    assert not ka.propagate_n_body(bad, JD0 + 1.0)[0].is_finite
