@@ -138,6 +138,41 @@ int32_t kh_propagate_n_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                                 double *const out[6], int32_t *status, int64_t *impact_id,
                                 double *impact_jd, kh_counters *counters, int32_t sync);
 
+/* Edge normals of a rectangular field (OnSkyRectangle::new,
+ * src/kete_core/src/fov/patches.rs:122-155; RectangleFOV's constructor
+ * src/kete/rust/fovs/definitions.rs:401-418 converts degrees to radians
+ * first).  Angles in radians; normals: [4][3].  Host-only, no device work. */
+void kh_fov_rect_normals(const double pointing[3], double rotation, double lon_width,
+                         double lat_width, double normals[12]);
+
+/* Visibility of n objects in rectangular fields with light time.  Replaces
+ * fov_checks_py (src/kete/rust/fovs/checks.rs:24-110) and the per-field
+ * FovLike::check_visible (src/kete_core/src/fov/fov_like.rs:134-192):
+ * fields are sorted by epoch (stably), grouped within 2*dt_limit, the
+ * objects re-propagated with the N-body integrator on kete's big/small step
+ * schedule, and every (object, field) pair checked.  States: n objects at
+ * epoch jd0, Sun-centred; fields: epoch, observer state [6][n_fields]
+ * (Sun-centred) and normals [n_fields][12]; everything in the ecliptic frame
+ * (17), the frame kete converts fields to (collection.rs:23-36).
+ * Outputs, ordered by (field epoch order, object index) as kete returns them:
+ * the original field index, the object index, and the light-time-corrected
+ * state [7][cap] (jd, pos, vel).  *n_hits receives the count; a count above
+ * cap returns KH_E_ARG with *n_hits set to the size needed. */
+int32_t kh_fov_state_check(kh_ctx *ctx, int64_t n, double jd0, const double *const xyz[6],
+                           int32_t frame, int64_t n_fields, const double *field_jd,
+                           const double *const obs[6], const double *normals, double dt_limit,
+                           int64_t cap, int64_t *n_hits, int64_t *hit_field, int64_t *hit_obj,
+                           double *const hit_state[7]);
+
+/* RA, Dec, dRA/dt*cos(Dec), dDec/dt (radians, radians/day) and V magnitude
+ * of n detections.  Replaces SimultaneousStates::ra_dec_with_rates
+ * (src/kete_core/src/simult_states.rs:86-120) and HGParams::apparent_mag
+ * (src/kete_core/src/flux/reflected.rs:255-268).  state and obs: [6][n],
+ * Sun-centred, in `frame`; h_mag: [n]; out: [5][n]. */
+int32_t kh_radec_vmag(kh_ctx *ctx, int64_t n, const double *const state[6],
+                      const double *const obs[6], int32_t frame, const double *h_mag,
+                      double g_param, double *const out[5]);
+
 /* Launch geometry last used by kh_propagate_n_body*: threads per block,
  * blocks, and mode actually run. */
 int32_t kh_last_launch(kh_ctx *ctx, int32_t *block, int32_t *grid, int32_t *mode);

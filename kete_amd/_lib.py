@@ -25,7 +25,8 @@ EXPORTS = [
     "kh_ctx_create", "kh_ctx_destroy", "kh_last_error", "kh_ctx_stream",
     "kh_spk_upload_type2", "kh_spk_states", "kh_propagate_two_body",
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
-    "kh_last_launch", "kh_last_kernel_ms",
+    "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
+    "kh_radec_vmag",
 ]
 
 
@@ -105,6 +106,13 @@ def load():
     L.kh_propagate_n_body_dev.restype = I32
     L.kh_last_launch.argtypes = [V, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
     L.kh_last_launch.restype = I32
+    L.kh_fov_rect_normals.argtypes = [P, D, D, D, P]
+    L.kh_fov_rect_normals.restype = None
+    L.kh_fov_state_check.argtypes = [V, I64, D, PP, I32, I64, P, PP, P, D, I64, C.POINTER(I64),
+                                     C.POINTER(I64), C.POINTER(I64), PP]
+    L.kh_fov_state_check.restype = I32
+    L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
+    L.kh_radec_vmag.restype = I32
     L.kh_last_kernel_ms.argtypes = [V]
     L.kh_last_kernel_ms.restype = D
     _lib = L

@@ -9,41 +9,17 @@
 #include <string>
 #include <vector>
 
-#include "../../include/kete_hip.h"
-#include "kh_internal.h"
+#include "kh_ctx.h"
 
 using namespace kh;
 
 static thread_local std::string g_err;
 
-static int32_t fail(int32_t code, const std::string& msg) {
+int32_t kh_fail(int32_t code, const std::string& msg) {
   g_err = msg;
   return code;
 }
-
-#define HIPCHK(expr)                                                            \
-  do {                                                                          \
-    hipError_t e_ = (expr);                                                     \
-    if (e_ != hipSuccess)                                                       \
-      return fail(KH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct kh_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  SpkDev spk{};           // host copy
-  SpkDev* d_spk = nullptr;  // device copy read by the kernels
-  bool has_spk = false;
-  double* d_coef = nullptr;
-  std::vector<int64_t> seg_obj, seg_center;
-  void* dbuf = nullptr;
-  size_t dbuf_size = 0;
-  unsigned long long* d_queue = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
-  int32_t last_block = 0, last_grid = 0, last_mode = 0;
-  int n_cu = 0;
-};
+#define fail kh_fail
 
 namespace kh {
 void fill_radau_tables(RadauTables& t) {
@@ -109,7 +85,8 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
   c->n_cu = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_counts, 4 * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
     return fail(KH_E_HIP, "stream/event/queue allocation failed");
   }
@@ -130,6 +107,8 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->d_coef) (void)hipFree(c->d_coef);
   if (c->d_spk) (void)hipFree(c->d_spk);
   if (c->dbuf) (void)hipFree(c->dbuf);
+  if (c->fbuf) (void)hipFree(c->fbuf);
+  if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -148,7 +127,7 @@ static int find_seg(const kh_ctx* c, int64_t obj) {
 
 // Segments whose states sum to the SSB state of `id` (center chaining of
 // spice/spk.rs:99-126 for the Type 2 planet tree).
-static bool build_chain(const kh_ctx* c, int64_t id, Chain& ch) {
+bool kh_build_chain(const kh_ctx* c, int64_t id, Chain& ch) {
   ch.n = 0;
   int64_t cur = id;
   while (cur != 0) {
@@ -251,17 +230,23 @@ int32_t kh_spk_upload_type2(kh_ctx* c, const kh_spk2_seg* segs, int64_t nseg) {
   return KH_OK;
 }
 
-// Grow-only device staging buffer for the host-pointer entry points.
-static int32_t staging(kh_ctx* c, size_t bytes, char** out) {
-  if (bytes > c->dbuf_size) {
-    if (c->dbuf) HIPCHK(hipFree(c->dbuf));
-    c->dbuf = nullptr;
-    c->dbuf_size = 0;
-    HIPCHK(hipMalloc(&c->dbuf, bytes));
-    c->dbuf_size = bytes;
+// Grow-only device buffer.
+int32_t kh_grow(void** buf, size_t* size, size_t bytes) {
+  if (bytes > *size) {
+    if (*buf) HIPCHK(hipFree(*buf));
+    *buf = nullptr;
+    *size = 0;
+    HIPCHK(hipMalloc(buf, bytes));
+    *size = bytes;
   }
-  *out = (char*)c->dbuf;
   return KH_OK;
+}
+
+// Staging buffer for the host-pointer entry points.
+static int32_t staging(kh_ctx* c, size_t bytes, char** out) {
+  int32_t r = kh_grow(&c->dbuf, &c->dbuf_size, bytes);
+  *out = (char*)c->dbuf;
+  return r;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -272,8 +257,8 @@ int32_t kh_spk_states(kh_ctx* c, int64_t obj_id, int64_t center, int32_t frame, 
   if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
   if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
   Chain body, cen;
-  if (!build_chain(c, obj_id, body)) return fail(KH_E_NO_SPK, "no path from object to 0");
-  if (!build_chain(c, center, cen)) return fail(KH_E_NO_SPK, "no path from center to 0");
+  if (!kh_build_chain(c, obj_id, body)) return fail(KH_E_NO_SPK, "no path from object to 0");
+  if (!kh_build_chain(c, center, cen)) return fail(KH_E_NO_SPK, "no path from center to 0");
   if (m == 0) return KH_OK;
   HIPCHK(hipSetDevice(c->device));
   size_t bj = align256(m * 8), bo = align256(m * 8), bs = align256(m * 4);
@@ -285,7 +270,7 @@ int32_t kh_spk_states(kh_ctx* c, int64_t obj_id, int64_t center, int32_t frame, 
   for (int k = 0; k < 6; ++k) d_out[k] = (double*)(base + bj + k * bo);
   int32_t* d_st = (int32_t*)(base + bj + 6 * bo);
   HIPCHK(hipMemcpyAsync(d_jd, jd, m * 8, hipMemcpyHostToDevice, c->stream));
-  if (launch_spk_states(c->d_spk, body, cen, frame, m, d_jd, d_out, d_st, c->stream))
+  if (launch_spk_states(c->d_spk, c->d_coef, body, cen, frame, m, d_jd, d_out, d_st, c->stream))
     return fail(KH_E_HIP, "spk_states launch failed");
   for (int k = 0; k < 6; ++k)
     HIPCHK(hipMemcpyAsync(out[k], d_out[k], m * 8, hipMemcpyDeviceToHost, c->stream));
@@ -336,7 +321,7 @@ int32_t kh_propagate_two_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.frame = frame;
   p.status = status;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (launch_two_body(c->d_spk, p, c->stream)) return fail(KH_E_HIP, "two-body launch failed");
+  if (launch_two_body(c->d_spk, c->d_coef, p, c->stream)) return fail(KH_E_HIP, "two-body launch failed");
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   if (sync) HIPCHK(hipStreamSynchronize(c->stream));
@@ -392,7 +377,7 @@ static int32_t check_nbody_args(kh_ctx* c, int64_t n, const double* jd0,
   if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
   if (!(epsilon > 0.0)) return fail(KH_E_ARG, "epsilon must be positive");
   if (mode < -1 || mode > 1) return fail(KH_E_ARG, "mode must be -1, 0 or 1");
-  if (!build_chain(c, center, ch))
+  if (!kh_build_chain(c, center, ch))
     return fail(KH_E_NO_SPK, "center " + std::to_string(center) + " is not in the ephemeris");
   return KH_OK;
 }
@@ -435,10 +420,9 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
     int per_cu = 2;
     p.grid = c->n_cu * per_cu * 4;
   }
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (launch_nbody(c->d_spk, p, c->stream, &c->last_block, &c->last_grid, &c->last_mode))
+  if (launch_nbody(c->d_spk, c->d_coef, p, c->stream, &c->last_block, &c->last_grid,
+                   &c->last_mode, c->ev0, c->ev1))
     return fail(KH_E_HIP, "n-body launch failed");
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   if (sync) HIPCHK(hipStreamSynchronize(c->stream));
   return KH_OK;

@@ -1,0 +1,44 @@
+// kh_ctx.h -- the library context and host helpers shared by the C ABI
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/kete_hip.h"
+#include "kh_internal.h"
+
+int32_t kh_fail(int32_t code, const std::string& msg);
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return kh_fail(KH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct kh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  kh::SpkDev spk{};            // host copy
+  kh::SpkDev* d_spk = nullptr;  // device copy read by the kernels
+  bool has_spk = false;
+  double* d_coef = nullptr;
+  std::vector<int64_t> seg_obj, seg_center;
+  void* dbuf = nullptr;
+  size_t dbuf_size = 0;
+  void* fbuf = nullptr;  // FOV working memory
+  size_t fbuf_size = 0;
+  unsigned long long* d_queue = nullptr;
+  unsigned long long* d_counts = nullptr;  // FOV list counters [4]
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  int32_t last_block = 0, last_grid = 0, last_mode = 0;
+  int n_cu = 0;
+};
+
+extern "C" {
+bool kh_build_chain(const kh_ctx* c, int64_t id, kh::Chain& ch);
+int32_t kh_grow(void** buf, size_t* size, size_t bytes);
+}

@@ -101,20 +101,6 @@ __device__ __forceinline__ Vec3 equ_to_ecl(Vec3 v) {
 // ---------------------------------------------------------------------------
 // SPK Type 2 (spk_segments.rs:150-177,355-376; interpolation.rs:25-77)
 // ---------------------------------------------------------------------------
-// Record lookup: returns the record pointer and the normalised time x.
-__device__ __forceinline__ int32_t seg_record(const SpkDev* __restrict__ spk, int s, double jd,
-                                              const double*& rec, double& x,
-                                              double& js) {
-  const SegDev& g = spk->seg[s];
-  if (jd < g.jd_start || jd > g.jd_end) return ST_DAF_LIMITS;
-  js = (jd - 2451545.0) * 86400.0;
-  double fi = floor((js - g.j0s) / g.intlen);
-  int32_t ri = fi > 0.0 ? (int32_t)fmin(fi, (double)(g.n_rec - 1)) : 0;
-  rec = spk->coef + g.off + (int64_t)ri * g.rsize;
-  x = (js - rec[0]) / rec[1];
-  return ST_OK;
-}
-
 // Position-only Chebyshev sum, forward T recurrence in ascending order.
 template <int N>
 __device__ __forceinline__ Vec3 cheb_pos_n(const double* __restrict__ c, double x) {
@@ -174,39 +160,103 @@ __device__ __forceinline__ void cheb_both_dyn(const double* __restrict__ c, int 
   d = Vec3{dx, dy, dz};
 }
 
-// Segment state in AU and AU/day (spk_segments.rs:355-376).
-__device__ __forceinline__ int32_t seg_pos(const SpkDev* __restrict__ spk, int s, double jd, Vec3& pos) {
-  const double* rec;
-  double x, js;
-  int32_t st = seg_record(spk, s, jd, rec, x, js);
-  if (st) return st;
-  Vec3 p = spk->all14 ? cheb_pos_n<14>(rec + 2, x) : cheb_pos_dyn(rec + 2, spk->seg[s].n_coef, x);
+template <int N>
+__device__ __forceinline__ void cheb_both_n(const double* __restrict__ c, double x, Vec3& p,
+                                            Vec3& d) {
+  double x2 = 2.0 * x;
+  double vx = c[0] + c[1] * x, vy = c[N] + c[N + 1] * x, vz = c[2 * N] + c[2 * N + 1] * x;
+  double dx = c[1], dy = c[N + 1], dz = c[2 * N + 1];
+  double t2 = 1.0, t1 = x, u2 = 1.0, u1 = x2;
+#pragma unroll
+  for (int k = 2; k < N; ++k) {
+    double t = x2 * t1 - t2;
+    vx += c[k] * t;
+    vy += c[N + k] * t;
+    vz += c[2 * N + k] * t;
+    t2 = t1;
+    t1 = t;
+    double u = x2 * u1 - u2;
+    const double kk = (double)k;
+    dx += c[k] * u1 * kk;
+    dy += c[N + k] * u1 * kk;
+    dz += c[2 * N + k] * u1 * kk;
+    u2 = u1;
+    u1 = u;
+  }
+  p = Vec3{vx, vy, vz};
+  d = Vec3{dx, dy, dz};
+}
+
+// Record index of jd in segment g (spk_segments.rs:355-358), clamped to the
+// records present: kete reads one record past the end at jd == jd_end.
+__device__ __forceinline__ int32_t record_index(const SegDev& g, double js) {
+  double fi = floor((js - g.j0s) / g.intlen);
+  return fi > 0.0 ? (int32_t)fmin(fi, (double)(g.n_rec - 1)) : 0;
+}
+
+// Chebyshev lookups read a 44-double record.  When every active lane of the
+// wavefront needs the same record -- the common case, since a wavefront's
+// objects advance through time together -- the record address is uniform and
+// the coefficients are fetched once per wavefront with scalar loads into
+// SGPRs; otherwise each lane loads its own record.
+template <bool VEL>
+__device__ __forceinline__ int32_t seg_eval(const SpkDev* __restrict__ spk,
+                                            const double* __restrict__ coef, int s, double jd,
+                                            Vec3& pos, Vec3& vel) {
+  const SegDev& g = spk->seg[s];
+  const bool in = !(jd < g.jd_start || jd > g.jd_end);
+  const double js = (jd - 2451545.0) * 86400.0;
+  const int32_t ri = record_index(g, js);
+  const int32_t r0 = __builtin_amdgcn_readfirstlane(ri);
+  const bool all14 = spk->all14 != 0;
+  Vec3 p, d{0.0, 0.0, 0.0};
+  double radius;
+  if (__all(ri == r0) && all14) {
+    const double* __restrict__ rec = coef + g.off + (int64_t)r0 * g.rsize;
+    radius = rec[1];
+    const double x = (js - rec[0]) / radius;
+    if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
+    else p = cheb_pos_n<14>(rec + 2, x);
+  } else {
+    const double* __restrict__ rec = coef + g.off + (int64_t)ri * g.rsize;
+    radius = rec[1];
+    const double x = (js - rec[0]) / radius;
+    if (VEL) cheb_both_dyn(rec + 2, g.n_coef, x, p, d);
+    else p = cheb_pos_dyn(rec + 2, g.n_coef, x);
+  }
+  if (!in) return ST_DAF_LIMITS;
   pos = Vec3{p.x / AU_KM, p.y / AU_KM, p.z / AU_KM};
+  if (VEL) {
+    const double sc = 86400.0 / radius / AU_KM;
+    vel = Vec3{d.x * sc, d.y * sc, d.z * sc};
+  }
   return ST_OK;
 }
 
-__device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk, int s, double jd, Vec3& pos,
-                                               Vec3& vel) {
-  const double* rec;
-  double x, js;
-  int32_t st = seg_record(spk, s, jd, rec, x, js);
-  if (st) return st;
-  Vec3 p, d;
-  cheb_both_dyn(rec + 2, spk->seg[s].n_coef, x, p, d);
-  double sc = 86400.0 / rec[1] / AU_KM;
-  pos = Vec3{p.x / AU_KM, p.y / AU_KM, p.z / AU_KM};
-  vel = Vec3{d.x * sc, d.y * sc, d.z * sc};
-  return ST_OK;
+// Segment position in AU (spk_segments.rs:355-376).
+__device__ __forceinline__ int32_t seg_pos(const SpkDev* __restrict__ spk,
+                                           const double* __restrict__ coef, int s, double jd,
+                                           Vec3& pos) {
+  Vec3 unused;
+  return seg_eval<false>(spk, coef, s, jd, pos, unused);
+}
+
+// Segment position (AU) and velocity (AU/day).
+__device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk,
+                                               const double* __restrict__ coef, int s, double jd,
+                                               Vec3& pos, Vec3& vel) {
+  return seg_eval<true>(spk, coef, s, jd, pos, vel);
 }
 
 // Sum of a chain of segment states (the center-chaining of spk.rs:99-126).
-__device__ __forceinline__ int32_t chain_state(const SpkDev* __restrict__ spk, const Chain& ch, double jd,
-                                               Vec3& pos, Vec3& vel) {
+__device__ __forceinline__ int32_t chain_state(const SpkDev* __restrict__ spk,
+                                               const double* __restrict__ coef, const Chain& ch,
+                                               double jd, Vec3& pos, Vec3& vel) {
   pos = Vec3{0, 0, 0};
   vel = Vec3{0, 0, 0};
   for (int k = 0; k < ch.n; ++k) {
     Vec3 p, v;
-    int32_t st = seg_pos_vel(spk, ch.seg[k], jd, p, v);
+    int32_t st = seg_pos_vel(spk, coef, ch.seg[k], jd, p, v);
     if (st) return st;
     if (k == 0) {
       pos = p;
@@ -451,12 +501,13 @@ __device__ __forceinline__ void nongrav_add(const NonGravDev& ng, Vec3& a, Vec3 
 // (x, v) and TDB JD t.  On exact evaluations, r <= radius is an Impact.
 // The Earth-Moon barycentre is evaluated once and shared by 399 and 301
 // (kete evaluates it for each, with identical results).
-__device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk, double t, Vec3 x,
+__device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
+                                             const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
                                              int32_t& impact_id) {
   Vec3 a{0.0, 0.0, 0.0};
   Vec3 emb;
-  int32_t st = seg_pos(spk, spk->emb_seg, t, emb);
+  int32_t st = seg_pos(spk, coef, spk->emb_seg, t, emb);
   if (st) return st;
 #pragma unroll 1
   for (int b = 0; b < N_PLANETS; ++b) {
@@ -464,9 +515,9 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk, dou
     const double mass = bd.mass;
     Vec3 bp, bv{0.0, 0.0, 0.0};
     if (bd.kind == 1) {
-      st = seg_pos_vel(spk, bd.seg, t, bp, bv);
+      st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv);
     } else {
-      st = seg_pos(spk, bd.seg, t, bp);
+      st = seg_pos(spk, coef, bd.seg, t, bp);
       if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
     }
     if (st) return st;

@@ -1,0 +1,283 @@
+// kh_fov.hip -- field-of-view visibility with light time, and the RA/Dec and
+// V-magnitude epilogue, on gfx950.
+//
+// Restates FovLike::check_visible (src/kete_core/src/fov/fov_like.rs:134-192)
+// for rectangular fields (SphericalPolygon::contains, fov/patches.rs:214-234)
+// as three data-parallel phases over a group of fields that share one state
+// epoch (the grouping of fov_checks_py, src/kete/rust/fovs/checks.rs:24-110):
+//  1. prefilter: one lane per object, looping over the group's fields held in
+//     LDS; the linear-motion check (fov_like.rs:43-66) against the
+//     2|dv|dt_limit margin (:147-150).  Survivors are appended to a candidate
+//     list, so the expensive checks run with full wavefronts.
+//  2. two-body: one lane per candidate -- two universal-Kepler solves with the
+//     light-time step (fov_like.rs:71-85) and the containment test.  Hits are
+//     appended with their light-time-corrected state.
+//  3. n-body (only for fields more than dt_limit from the state epoch): the
+//     candidates are re-propagated with the Radau kernel, then a light-time
+//     step and containment (fov_like.rs:90-108) finish them.
+#include "kh_device.h"
+#include "kh_internal.h"
+
+namespace kh {
+
+// SphericalPolygon::contains: inside iff no edge has a negative (sign bit
+// set) dot product; else the largest such |d|, capped by |v|; NaN -> outside.
+__device__ __forceinline__ bool rect_contains(const double* __restrict__ nrm, Vec3 v,
+                                              double& dist) {
+  double closest = -__builtin_inf();
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double d = v.x * nrm[3 * e] + v.y * nrm[3 * e + 1] + v.z * nrm[3 * e + 2];
+    if (isnan(d)) {
+      dist = d;
+      return false;
+    }
+    if (signbit(d) && fabs(d) > closest) closest = fabs(d);
+  }
+  if (isfinite(closest)) {
+    dist = fmin(closest, norm(v));
+    return false;
+  }
+  dist = 0.0;
+  return true;
+}
+
+constexpr int MAX_GROUP_FIELDS = 256;
+
+__global__ void __launch_bounds__(256) fov_prefilter_kernel(FovPrefilter p) {
+  __shared__ FieldDev sf[MAX_GROUP_FIELDS];
+  for (int k = threadIdx.x; k < p.nf; k += blockDim.x) sf[k] = p.fields[k];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const double jd = p.jd[i];
+  const Vec3 x{p.x[0][i], p.x[1][i], p.x[2][i]};
+  const Vec3 v{p.x[3][i], p.x[4][i], p.x[5][i]};
+  // objects whose re-propagation failed were dropped by kete (filter_map);
+  // their NaN states can never be seen, so they are skipped here.
+  if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
+        isfinite(v.z) && isfinite(jd)))
+    return;
+  for (int f = 0; f < p.nf; ++f) {
+    const FieldDev& F = sf[f];
+    const Vec3 dv{v.x - F.obs[3], v.y - F.obs[4], v.z - F.obs[5]};
+    const double max_dist = norm(dv) * p.dt_limit * 2.0;
+    int32_t kind;
+    if (fabs(jd - F.jd) < p.dt_limit) {
+      // check_linear (fov_like.rs:43-66)
+      const Vec3 rel{x.x - F.obs[0], x.y - F.obs[1], x.z - F.obs[2]};
+      const double dt = F.jd - jd - norm(rel) * C_AU_PER_DAY_INV;
+      const Vec3 nr{(x.x + v.x * dt) - F.obs[0], (x.y + v.y * dt) - F.obs[1],
+                    (x.z + v.z * dt) - F.obs[2]};
+      double dist;
+      const bool inside = rect_contains(F.nrm, nr, dist);
+      if (!inside && dist > max_dist) continue;
+      kind = 0;
+    } else {
+      kind = 1;  // two-body prefilter, then n-body
+    }
+    const unsigned long long slot = atomicAdd(p.n_cand, 1ull);
+    if (slot < (unsigned long long)p.cap) {
+      p.cand_field[slot] = f;
+      p.cand_kind[slot] = kind;
+      p.cand_obj[slot] = i;
+    }
+  }
+}
+
+// check_two_body (fov_like.rs:71-85) for candidates; kind 0 -> hits, kind 1
+// -> n-body list when the two-body position is within the margin.
+__global__ void __launch_bounds__(256) fov_two_body_kernel(FovTwoBody p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.n_cand) return;
+  const int32_t f = p.cand_field[q];
+  const int64_t i = p.cand_obj[q];
+  const FieldDev& F = p.fields[f];
+  const double jd = p.jd[i];
+  const Vec3 x{p.x[0][i], p.x[1][i], p.x[2][i]};
+  const Vec3 v{p.x[3][i], p.x[4][i], p.x[5][i]};
+  Vec3 p1, v1, p2, v2;
+  if (analytic_2_body(F.jd - jd, x, v, p1, v1) != ST_OK) return;
+  const Vec3 d1{p1.x - F.obs[0], p1.y - F.obs[1], p1.z - F.obs[2]};
+  const double dt = -norm(d1) * C_AU_PER_DAY_INV;
+  const double jd_obs = F.jd + dt;
+  if (analytic_2_body(jd_obs - F.jd, p1, v1, p2, v2) != ST_OK) return;
+  const Vec3 rel{p2.x - F.obs[0], p2.y - F.obs[1], p2.z - F.obs[2]};
+  double dist;
+  const bool inside = rect_contains(F.nrm, rel, dist);
+  if (p.cand_kind[q] == 0) {
+    if (!inside) return;
+    const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
+    if (slot < (unsigned long long)p.hit_cap) {
+      p.hit_field[slot] = f;
+      p.hit_obj[slot] = i;
+      p.hit_state[0][slot] = jd_obs;
+      p.hit_state[1][slot] = p2.x;
+      p.hit_state[2][slot] = p2.y;
+      p.hit_state[3][slot] = p2.z;
+      p.hit_state[4][slot] = v2.x;
+      p.hit_state[5][slot] = v2.y;
+      p.hit_state[6][slot] = v2.z;
+    }
+  } else {
+    const Vec3 dvv{v.x - F.obs[3], v.y - F.obs[4], v.z - F.obs[5]};
+    const double max_dist = norm(dvv) * p.dt_limit * 2.0;
+    if (!inside && dist > max_dist) return;
+    const unsigned long long slot = atomicAdd(p.n_nb, 1ull);
+    if (slot < (unsigned long long)p.nb_cap) {
+      p.nb_field[slot] = f;
+      p.nb_obj[slot] = i;
+    }
+  }
+}
+
+// Gather the states of the n-body candidates and their target epochs.
+__global__ void fov_nb_gather_kernel(FovNBody p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.m) return;
+  const int64_t i = p.nb_obj[q];
+  p.g_jd0[q] = p.jd[i];
+  for (int k = 0; k < 6; ++k) p.g_x[k][q] = p.x[k][i];
+  p.g_jdf[q] = p.fields[p.nb_field[q]].jd;
+}
+
+// check_n_body tail (fov_like.rs:98-107): light-time step and containment.
+__global__ void fov_nb_finish_kernel(FovNBody p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.m) return;
+  if (p.g_status[q] != ST_OK) return;
+  const int32_t f = p.nb_field[q];
+  const FieldDev& F = p.fields[f];
+  const Vec3 x{p.g_out[0][q], p.g_out[1][q], p.g_out[2][q]};
+  const Vec3 v{p.g_out[3][q], p.g_out[4][q], p.g_out[5][q]};
+  const Vec3 d1{x.x - F.obs[0], x.y - F.obs[1], x.z - F.obs[2]};
+  const double dt = -norm(d1) * C_AU_PER_DAY_INV;
+  const double jd_obs = F.jd + dt;
+  Vec3 p2, v2;
+  if (analytic_2_body(jd_obs - F.jd, x, v, p2, v2) != ST_OK) return;
+  const Vec3 rel{p2.x - F.obs[0], p2.y - F.obs[1], p2.z - F.obs[2]};
+  double dist;
+  if (!rect_contains(F.nrm, rel, dist)) return;
+  const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
+  if (slot < (unsigned long long)p.hit_cap) {
+    p.hit_field[slot] = f;
+    p.hit_obj[slot] = p.nb_obj[q];
+    p.hit_state[0][slot] = jd_obs;
+    p.hit_state[1][slot] = p2.x;
+    p.hit_state[2][slot] = p2.y;
+    p.hit_state[3][slot] = p2.z;
+    p.hit_state[4][slot] = v2.x;
+    p.hit_state[5][slot] = v2.y;
+    p.hit_state[6][slot] = v2.z;
+  }
+}
+
+// ra_dec_with_rates (simult_states.rs:86-120, to_lat_lon frames/definitions.rs:
+// 337-345) and HGParams::apparent_mag (flux/reflected.rs:23-33,255-268).
+__device__ __forceinline__ double rem_euclid(double x, double m) {
+  const double r = fmod(x, m);
+  return r < 0.0 ? r + fabs(m) : r;
+}
+
+__device__ __forceinline__ double hg_helper(double a, double b, double c, double phase) {
+  const double ps = sin(phase);
+  const double th = tan(0.5 * phase);
+  const double theta_l = exp(-a * pow(th, b));
+  const double theta_s = 1.0 - c * ps / (0.119 + 1.341 * ps - 0.754 * (ps * ps));
+  const double w = exp(-90.56 * (th * th));
+  return w * theta_s + (1.0 - w) * theta_l;
+}
+
+__global__ void radec_vmag_kernel(RadecArgs p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.n) return;
+  Vec3 x{p.st[0][q], p.st[1][q], p.st[2][q]};
+  Vec3 v{p.st[3][q], p.st[4][q], p.st[5][q]};
+  Vec3 o{p.obs[0][q], p.obs[1][q], p.obs[2][q]};
+  Vec3 ov{p.obs[3][q], p.obs[4][q], p.obs[5][q]};
+  // V magnitude is frame invariant: use the input frame, Sun-centred.
+  double vmag;
+  {
+    const double obj_r = norm(x);
+    const Vec3 o2o{o.x - x.x, o.y - x.y, o.z - x.z};
+    const double o2o_r = norm(o2o);
+    const Vec3 neg{-o2o.x, -o2o.y, -o2o.z};
+    const double n2 = norm(neg);
+    double phase;
+    if (obj_r == 0.0 || n2 == 0.0) {
+      phase = 0.0;
+    } else {
+      const double cang = dot(x, neg) / (obj_r * n2);
+      phase = cang > 1.0 ? 0.0 : (cang < -1.0 ? 3.141592653589793 : acos(cang));
+    }
+    if (phase > 2.792526803190927) {
+      vmag = __builtin_nan("");
+    } else {
+      const double g = p.g;
+      const double corr = log10((1.0 - g) * hg_helper(3.332, 0.631, 0.986, phase) +
+                                g * hg_helper(1.862, 1.218, 0.238, phase));
+      vmag = p.h[q] + 5.0 * log10(obj_r * o2o_r) - 2.5 * corr;
+    }
+  }
+  if (p.frame == 17) {
+    x = ecl_to_equ(x);
+    v = ecl_to_equ(v);
+    o = ecl_to_equ(o);
+    ov = ecl_to_equ(ov);
+  }
+  const Vec3 d{x.x - o.x, x.y - o.y, x.z - o.z};
+  const Vec3 w{v.x - ov.x, v.y - ov.y, v.z - ov.z};
+  const double d_ra = (d.x * w.y - d.y * w.x) / (d.x * d.x + d.y * d.y);
+  const double r2 = dot(d, d);
+  const double d_dec = (w.z - d.z * dot(d, w) / r2) / sqrt(r2 - d.z * d.z);
+  const double r = norm(d);
+  double dec = 0.0, ra = 0.0;
+  if (!(r < 1e-10)) {
+    const double zr = fmin(fmax(d.z / r, -1.0), 1.0);
+    dec = rem_euclid(3.0 * (3.141592653589793 / 2.0) - acos(zr), 6.283185307179586) -
+          3.141592653589793;
+    ra = rem_euclid(atan2(d.y, d.x), 6.283185307179586);
+  }
+  p.out[0][q] = ra;
+  p.out[1][q] = dec;
+  p.out[2][q] = d_ra * cos(dec);
+  p.out[3][q] = d_dec;
+  p.out[4][q] = vmag;
+}
+
+int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s) {
+  if (p.n <= 0 || p.nf <= 0) return 0;
+  if (p.nf > MAX_GROUP_FIELDS) return -1;
+  hipLaunchKernelGGL(fov_prefilter_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s,
+                     p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int32_t launch_fov_two_body(const FovTwoBody& p, hipStream_t s) {
+  if (p.n_cand <= 0) return 0;
+  hipLaunchKernelGGL(fov_two_body_kernel, dim3((unsigned)((p.n_cand + 255) / 256)), dim3(256), 0,
+                     s, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s) {
+  if (p.m <= 0) return 0;
+  hipLaunchKernelGGL(fov_nb_gather_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s,
+                     p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s) {
+  if (p.m <= 0) return 0;
+  hipLaunchKernelGGL(fov_nb_finish_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s,
+                     p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s) {
+  if (p.n <= 0) return 0;
+  hipLaunchKernelGGL(radec_vmag_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace kh

@@ -1,0 +1,382 @@
+// kh_fov_host.cpp -- host driver of the field-of-view path: the grouping and
+// re-propagation schedule of fov_checks_py (src/kete/rust/fovs/checks.rs:
+// 24-110) around the device phases in kh_fov.hip, the rectangle constructor
+// (OnSkyRectangle::new, src/kete_core/src/fov/patches.rs:122-155) and the
+// RA/Dec + V-magnitude epilogue.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "kh_ctx.h"
+
+using namespace kh;
+
+namespace {
+
+// rotate_around (frames/definitions.rs:356-363): Rodrigues rotation about the
+// normalised axis; identity for a zero angle.
+void rotate_around(const double* v, const double* axis, double angle, double* o) {
+  if (!(angle != 0.0)) {
+    std::memcpy(o, v, 3 * sizeof(double));
+    return;
+  }
+  const double n = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+  const double ux = axis[0] / n, uy = axis[1] / n, uz = axis[2] / n;
+  const double sqx = ux * ux, sqy = uy * uy, sqz = uz * uz;
+  const double s = std::sin(angle), c = std::cos(angle), omc = 1.0 - c;
+  const double m[3][3] = {
+      {sqx + (1.0 - sqx) * c, ux * uy * omc - uz * s, ux * uz * omc + uy * s},
+      {ux * uy * omc + uz * s, sqy + (1.0 - sqy) * c, uy * uz * omc - ux * s},
+      {ux * uz * omc - uy * s, uy * uz * omc + ux * s, sqz + (1.0 - sqz) * c},
+  };
+  const double x = v[0], y = v[1], z = v[2];
+  for (int i = 0; i < 3; ++i) o[i] = m[i][0] * x + m[i][1] * y + m[i][2] * z;
+}
+
+void cross(const double* a, const double* b, double* o) {
+  const double x = a[1] * b[2] - a[2] * b[1];
+  const double y = a[2] * b[0] - a[0] * b[2];
+  const double z = a[0] * b[1] - a[1] * b[0];
+  o[0] = x;
+  o[1] = y;
+  o[2] = z;
+}
+
+__global__ void fill_f64(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void zero_counts(unsigned long long* p) {
+  if (threadIdx.x < 4) p[threadIdx.x] = 0ull;
+}
+
+// candidate and n-body list counters restart per chunk; the hit count does not
+__global__ void zero_list_counts(unsigned long long* p) {
+  if (threadIdx.x == 0) p[0] = 0ull;
+  if (threadIdx.x == 2) p[2] = 0ull;
+}
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+void kh_fov_rect_normals(const double pointing[3], double rotation, double lon_width,
+                         double lat_width, double normals[12]) {
+  const double zax[3] = {0.0, 0.0, 1.0};
+  double up[3], left[3], nleft[3], nup[3];
+  rotate_around(zax, pointing, -rotation, up);
+  cross(pointing, up, left);
+  cross(pointing, left, up);
+  rotate_around(left, up, -lon_width / 2.0, normals + 0);
+  rotate_around(up, left, lat_width / 2.0, normals + 3);
+  for (int i = 0; i < 3; ++i) {
+    nleft[i] = -left[i];
+    nup[i] = -up[i];
+  }
+  rotate_around(nleft, up, lon_width / 2.0, normals + 6);
+  rotate_around(nup, left, -lat_width / 2.0, normals + 9);
+}
+
+int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const xyz[6],
+                           int32_t frame, int64_t n_fields, const double* field_jd,
+                           const double* const obs[6], const double* normals, double dt_limit,
+                           int64_t cap, int64_t* n_hits, int64_t* hit_field, int64_t* hit_obj,
+                           double* const hit_state[7]) {
+  if (!c || !n_hits) return kh_fail(KH_E_ARG, "null argument");
+  *n_hits = 0;
+  if (n < 0 || n_fields < 0 || cap < 0) return kh_fail(KH_E_ARG, "negative size");
+  if (frame != KH_FRAME_ECLIPTIC)
+    return kh_fail(KH_E_ARG, "fields and states must be ecliptic (kete converts FOVs to it)");
+  if (!(dt_limit > 0.0)) return kh_fail(KH_E_ARG, "dt_limit must be positive");
+  if (!c->has_spk) return kh_fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  if (n == 0 || n_fields == 0) return KH_OK;
+  if (!xyz || !field_jd || !obs || !normals || (cap > 0 && (!hit_field || !hit_obj || !hit_state)))
+    return kh_fail(KH_E_ARG, "null array");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+
+  // fovs.sort_by(jd) (collection.rs:23-36): a stable sort
+  std::vector<int64_t> order(n_fields);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return field_jd[a] < field_jd[b]; });
+  std::vector<FieldDev> fields(n_fields);
+  for (int64_t k = 0; k < n_fields; ++k) {
+    const int64_t f = order[k];
+    FieldDev& F = fields[k];
+    F.jd = field_jd[f];
+    for (int q = 0; q < 6; ++q) F.obs[q] = obs[q][f];
+    for (int q = 0; q < 12; ++q) F.nrm[q] = normals[12 * f + q];
+    F.pad = 0.0;
+  }
+  // chunks within 2*dt_limit of their first field (checks.rs:34-57)
+  std::vector<std::pair<int64_t, int64_t>> chunks;
+  for (int64_t k = 0; k < n_fields;) {
+    int64_t e = k + 1;
+    while (e < n_fields && !(std::fabs(fields[e].jd - fields[k].jd) >= 2.0 * dt_limit)) ++e;
+    chunks.push_back({k, e});
+    k = e;
+  }
+  int64_t max_group = 0;
+  for (auto& ch : chunks) max_group = std::max(max_group, ch.second - ch.first);
+  if (max_group > 256)
+    return kh_fail(KH_E_ARG, "more than 256 fields within 2*dt_limit of each other");
+
+  // device memory: states (cur, big) with per-object epochs, fields, lists
+  int64_t cand_cap = std::max<int64_t>(1 << 20, 8 * n);
+  int64_t nb_cap = std::max<int64_t>(1 << 16, n / 4);
+  const int64_t hit_cap = std::max<int64_t>(cap, 1);
+  std::vector<std::pair<int64_t, int64_t>> chunk_hits;  // (first sorted field, hits so far)
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    const size_t bn = al(n * 8), bs = al(n * 4);
+    const size_t need = 2 * (7 * bn) + bs + al(n_fields * sizeof(FieldDev)) +
+                        al(cand_cap * 4) * 2 + al(cand_cap * 8) + al(nb_cap * 4) +
+                        al(nb_cap * 8) + al(nb_cap * 8) * 14 + al(nb_cap * 4) +
+                        al(hit_cap * 4) + al(hit_cap * 8) * 8 + al(8);
+    int32_t r = kh_grow(&c->fbuf, &c->fbuf_size, need);
+    if (r) return r;
+    char* q = (char*)c->fbuf;
+    auto take = [&](size_t b) {
+      char* p = q;
+      q += b;
+      return p;
+    };
+    double* cur_jd = (double*)take(bn);
+    double* cur[6];
+    for (int k = 0; k < 6; ++k) cur[k] = (double*)take(bn);
+    double* big_jd = (double*)take(bn);
+    double* big[6];
+    for (int k = 0; k < 6; ++k) big[k] = (double*)take(bn);
+    int32_t* st = (int32_t*)take(bs);
+    FieldDev* d_fields = (FieldDev*)take(al(n_fields * sizeof(FieldDev)));
+    int32_t* cand_field = (int32_t*)take(al(cand_cap * 4));
+    int32_t* cand_kind = (int32_t*)take(al(cand_cap * 4));
+    int64_t* cand_obj = (int64_t*)take(al(cand_cap * 8));
+    int32_t* nb_field = (int32_t*)take(al(nb_cap * 4));
+    int64_t* nb_obj = (int64_t*)take(al(nb_cap * 8));
+    double* g_jd0 = (double*)take(al(nb_cap * 8));
+    double* g_jdf = (double*)take(al(nb_cap * 8));
+    double* g_x[6];
+    double* g_out[6];
+    for (int k = 0; k < 6; ++k) g_x[k] = (double*)take(al(nb_cap * 8));
+    for (int k = 0; k < 6; ++k) g_out[k] = (double*)take(al(nb_cap * 8));
+    int32_t* g_st = (int32_t*)take(al(nb_cap * 4));
+    int32_t* h_field = (int32_t*)take(al(hit_cap * 4));
+    int64_t* h_obj = (int64_t*)take(al(hit_cap * 8));
+    double* h_state[7];
+    for (int k = 0; k < 7; ++k) h_state[k] = (double*)take(al(hit_cap * 8));
+    double* d_scalar = (double*)take(al(8));
+
+    HIPCHK(hipMemcpyAsync(d_fields, fields.data(), n_fields * sizeof(FieldDev),
+                          hipMemcpyHostToDevice, s));
+    for (int k = 0; k < 6; ++k) {
+      HIPCHK(hipMemcpyAsync(cur[k], xyz[k], n * 8, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(big[k], cur[k], n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, cur_jd, n, jd0);
+    hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, big_jd, n, jd0);
+    hipLaunchKernelGGL(zero_counts, dim3(1), dim3(64), 0, s, c->d_counts);
+    unsigned long long* d_nhit = c->d_counts + 1;
+
+    // propagate states (in place) to jd_target, then set their epochs
+    auto propagate = [&](double* const* x, double* xjd, double jd_target) -> int32_t {
+      HIPCHK(hipMemcpyAsync(d_scalar, &jd_target, 8, hipMemcpyHostToDevice, s));
+      const double* xin[6] = {x[0], x[1], x[2], x[3], x[4], x[5]};
+      int32_t rr = kh_propagate_n_body_dev(c, n, xjd, xin, 10, frame, d_scalar, 0, nullptr, 1e-6,
+                                           1, x, st, nullptr, nullptr, nullptr, 0);
+      if (rr) return rr;
+      hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, xjd, n, jd_target);
+      return KH_OK;
+    };
+
+    double jd = jd0, big_epoch_ref = jd0;
+    bool overflow = false;
+    for (auto& ch : chunks) {
+      const int64_t f0 = ch.first, f1 = ch.second;
+      const double jd_mean = (fields[f1 - 1].jd + fields[f0].jd) / 2.0;
+      // big steps (checks.rs:63-72): the big-step states move to the
+      // current small-step epoch
+      if (std::fabs(jd_mean - big_epoch_ref) >= dt_limit * 50.0) {
+        big_epoch_ref = jd_mean;
+        r = propagate(big, big_jd, jd);
+        if (r) return r;
+      }
+      // small steps (checks.rs:74-85)
+      if (std::fabs(jd_mean - jd) >= dt_limit) {
+        if (std::fabs(jd - big_epoch_ref) >= dt_limit * 25.0) {
+          for (int k = 0; k < 6; ++k)
+            HIPCHK(hipMemcpyAsync(cur[k], big[k], n * 8, hipMemcpyDeviceToDevice, s));
+          HIPCHK(hipMemcpyAsync(cur_jd, big_jd, n * 8, hipMemcpyDeviceToDevice, s));
+        }
+        jd = jd_mean;
+        r = propagate(cur, cur_jd, jd);
+        if (r) return r;
+      }
+      // phase 1: prefilter over (object, field) pairs of this chunk
+      hipLaunchKernelGGL(zero_list_counts, dim3(1), dim3(64), 0, s, c->d_counts);
+      FovPrefilter pf{};
+      pf.n = n;
+      pf.jd = cur_jd;
+      for (int k = 0; k < 6; ++k) pf.x[k] = cur[k];
+      pf.fields = d_fields + f0;
+      pf.nf = (int32_t)(f1 - f0);
+      pf.dt_limit = dt_limit;
+      pf.n_cand = c->d_counts;
+      pf.cap = cand_cap;
+      pf.cand_field = cand_field;
+      pf.cand_kind = cand_kind;
+      pf.cand_obj = cand_obj;
+      if (launch_fov_prefilter(pf, s)) return kh_fail(KH_E_HIP, "prefilter launch failed");
+      unsigned long long counts[4];
+      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if ((int64_t)counts[0] > cand_cap) {
+        cand_cap = (int64_t)counts[0] * 2;
+        overflow = true;
+        break;
+      }
+      // phase 2: two-body checks of the candidates
+      FovTwoBody tb{};
+      tb.n_cand = (int64_t)counts[0];
+      tb.jd = cur_jd;
+      for (int k = 0; k < 6; ++k) tb.x[k] = cur[k];
+      tb.fields = d_fields + f0;
+      tb.dt_limit = dt_limit;
+      tb.cand_field = cand_field;
+      tb.cand_kind = cand_kind;
+      tb.cand_obj = cand_obj;
+      tb.n_hit = d_nhit;
+      tb.hit_cap = hit_cap;
+      tb.hit_field = h_field;
+      tb.hit_obj = h_obj;
+      for (int k = 0; k < 7; ++k) tb.hit_state[k] = h_state[k];
+      tb.n_nb = c->d_counts + 2;
+      tb.nb_cap = nb_cap;
+      tb.nb_field = nb_field;
+      tb.nb_obj = nb_obj;
+      if (launch_fov_two_body(tb, s)) return kh_fail(KH_E_HIP, "two-body launch failed");
+      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if ((int64_t)counts[2] > nb_cap) {
+        nb_cap = (int64_t)counts[2] * 2;
+        overflow = true;
+        break;
+      }
+      // phase 3: n-body re-checks (fields more than dt_limit from the epoch)
+      if (counts[2] > 0) {
+        FovNBody nb{};
+        nb.m = (int64_t)counts[2];
+        nb.jd = cur_jd;
+        for (int k = 0; k < 6; ++k) nb.x[k] = cur[k];
+        nb.fields = d_fields + f0;
+        nb.nb_field = nb_field;
+        nb.nb_obj = nb_obj;
+        nb.g_jd0 = g_jd0;
+        nb.g_jdf = g_jdf;
+        for (int k = 0; k < 6; ++k) {
+          nb.g_x[k] = g_x[k];
+          nb.g_out[k] = g_out[k];
+        }
+        nb.g_status = g_st;
+        nb.n_hit = d_nhit;
+        nb.hit_cap = hit_cap;
+        nb.hit_field = h_field;
+        nb.hit_obj = h_obj;
+        for (int k = 0; k < 7; ++k) nb.hit_state[k] = h_state[k];
+        if (launch_fov_nb_gather(nb, s)) return kh_fail(KH_E_HIP, "gather launch failed");
+        const double* gx[6] = {g_x[0], g_x[1], g_x[2], g_x[3], g_x[4], g_x[5]};
+        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, nullptr, 1e-6, 1,
+                                    g_out, g_st, nullptr, nullptr, nullptr, 0);
+        if (r) return r;
+        if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");
+      }
+      // global field index of this chunk's hits is resolved on the host
+      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      chunk_hits.push_back({f0, (int64_t)counts[1]});
+    }
+    if (overflow) {
+      chunk_hits.clear();
+      continue;
+    }
+    unsigned long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, d_nhit, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *n_hits = (int64_t)total;
+    if ((int64_t)total > cap) return kh_fail(KH_E_ARG, "hit capacity too small: need " +
+                                                           std::to_string(total));
+    // hits in (sorted field, object index) order, the order kete returns
+    std::vector<int32_t> hf(total);
+    std::vector<int64_t> ho(total);
+    std::vector<double> hs(7 * total);
+    if (total) {
+      HIPCHK(hipMemcpy(hf.data(), h_field, total * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ho.data(), h_obj, total * 8, hipMemcpyDeviceToHost));
+      for (int k = 0; k < 7; ++k)
+        HIPCHK(hipMemcpy(hs.data() + k * total, h_state[k], total * 8, hipMemcpyDeviceToHost));
+    }
+    // chunk-local field index -> sorted index
+    std::vector<int64_t> gfield(total);
+    size_t ci = 0;
+    for (size_t h = 0; h < total; ++h) {
+      while (ci < chunk_hits.size() && (int64_t)h >= chunk_hits[ci].second) ++ci;
+      gfield[h] = chunk_hits[ci].first + hf[h];
+    }
+    std::vector<int64_t> idx(total);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+      return gfield[a] != gfield[b] ? gfield[a] < gfield[b] : ho[a] < ho[b];
+    });
+    for (size_t k = 0; k < total; ++k) {
+      const int64_t h = idx[k];
+      hit_field[k] = order[gfield[h]];
+      hit_obj[k] = ho[h];
+      for (int q = 0; q < 7; ++q) hit_state[q][k] = hs[q * total + h];
+    }
+    return KH_OK;
+  }
+  return kh_fail(KH_E_HIP, "candidate lists overflowed twice");
+}
+
+int32_t kh_radec_vmag(kh_ctx* c, int64_t n, const double* const state[6],
+                      const double* const obs[6], int32_t frame, const double* h_mag,
+                      double g_param, double* const out[5]) {
+  if (!c || n < 0) return kh_fail(KH_E_ARG, "bad argument");
+  if (n == 0) return KH_OK;
+  if (!state || !obs || !h_mag || !out) return kh_fail(KH_E_ARG, "null array");
+  if (frame != 1 && frame != 17) return kh_fail(KH_E_ARG, "frame must be 1 or 17");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bn = al(n * 8);
+  int32_t r = kh_grow(&c->fbuf, &c->fbuf_size, 18 * bn);
+  if (r) return r;
+  double* base = (double*)c->fbuf;
+  RadecArgs a{};
+  a.n = n;
+  for (int k = 0; k < 6; ++k) {
+    double* ds = (double*)((char*)base + k * bn);
+    double* dobs = (double*)((char*)base + (6 + k) * bn);
+    HIPCHK(hipMemcpyAsync(ds, state[k], n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dobs, obs[k], n * 8, hipMemcpyHostToDevice, c->stream));
+    a.st[k] = ds;
+    a.obs[k] = dobs;
+  }
+  double* dh = (double*)((char*)base + 12 * bn);
+  HIPCHK(hipMemcpyAsync(dh, h_mag, n * 8, hipMemcpyHostToDevice, c->stream));
+  a.h = dh;
+  a.g = g_param;
+  a.frame = frame;
+  for (int k = 0; k < 5; ++k) a.out[k] = (double*)((char*)base + (13 + k) * bn);
+  if (launch_radec_vmag(a, c->stream)) return kh_fail(KH_E_HIP, "radec launch failed");
+  for (int k = 0; k < 5; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], a.out[k], n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+}  // extern "C"

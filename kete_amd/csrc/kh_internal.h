@@ -46,8 +46,11 @@ struct NBodyLaunch {
   int32_t grid;   // persistent grid size in blocks (0: default)
 };
 
-int32_t launch_nbody(const SpkDev* spk, const NBodyLaunch& p, hipStream_t stream,
-                     int32_t* used_block, int32_t* used_grid, int32_t* used_mode);
+// Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket
+// the integrator kernel alone.
+int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
+                     hipStream_t stream, int32_t* used_block, int32_t* used_grid,
+                     int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1);
 
 struct TwoBodyLaunch {
   int64_t n, m;
@@ -63,10 +66,89 @@ struct TwoBodyLaunch {
   int32_t* status;
 };
 
-int32_t launch_two_body(const SpkDev* spk, const TwoBodyLaunch& p, hipStream_t stream);
+int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaunch& p,
+                        hipStream_t stream);
 
-int32_t launch_spk_states(const SpkDev* spk, Chain body, Chain center, int32_t frame, int64_t m,
+int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center, int32_t frame, int64_t m,
                           const double* jd, double* const out[6], int32_t* status,
                           hipStream_t stream);
+
+// One rectangular field: epoch, observer state (center 10, same frame as the
+// objects) and the four edge normals (fov/patches.rs:122-155).
+struct FieldDev {
+  double jd;
+  double obs[6];
+  double nrm[12];
+  double pad;
+};
+
+struct FovPrefilter {
+  int64_t n;
+  const double* jd;
+  const double* x[6];
+  const FieldDev* fields;
+  int32_t nf;
+  double dt_limit;
+  unsigned long long* n_cand;
+  int64_t cap;
+  int32_t* cand_field;
+  int32_t* cand_kind;
+  int64_t* cand_obj;
+};
+
+struct FovTwoBody {
+  int64_t n_cand;
+  const double* jd;
+  const double* x[6];
+  const FieldDev* fields;
+  double dt_limit;
+  const int32_t* cand_field;
+  const int32_t* cand_kind;
+  const int64_t* cand_obj;
+  unsigned long long* n_hit;
+  int64_t hit_cap;
+  int32_t* hit_field;
+  int64_t* hit_obj;
+  double* hit_state[7];
+  unsigned long long* n_nb;
+  int64_t nb_cap;
+  int32_t* nb_field;
+  int64_t* nb_obj;
+};
+
+struct FovNBody {
+  int64_t m;
+  const double* jd;
+  const double* x[6];
+  const FieldDev* fields;
+  const int32_t* nb_field;
+  const int64_t* nb_obj;
+  double* g_jd0;
+  double* g_jdf;
+  double* g_x[6];
+  double* g_out[6];
+  const int32_t* g_status;
+  unsigned long long* n_hit;
+  int64_t hit_cap;
+  int32_t* hit_field;
+  int64_t* hit_obj;
+  double* hit_state[7];
+};
+
+struct RadecArgs {
+  int64_t n;
+  const double* st[6];
+  const double* obs[6];
+  const double* h;
+  double g;
+  int32_t frame;
+  double* out[5];
+};
+
+int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s);
+int32_t launch_fov_two_body(const FovTwoBody& p, hipStream_t s);
+int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s);
+int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s);
+int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s);
 
 }  // namespace kh

@@ -13,7 +13,9 @@
 
 namespace kh {
 
-__global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict__ spk, TwoBodyLaunch p) {
+__global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict__ spk,
+                                                       const double* __restrict__ coef,
+                                                       TwoBodyLaunch p) {
   const int64_t total = p.n * p.m;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const double nan = __builtin_nan("");
@@ -29,7 +31,7 @@ __global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict_
     if (!p.center_sun) {
       // SSB -> Sun: add the flipped Sun state (spk.rs:104-107, state.rs:262-280)
       Vec3 sp, sv;
-      st = seg_pos_vel(spk, p.sun_seg, jd0, sp, sv);
+      st = seg_pos_vel(spk, coef, p.sun_seg, jd0, sp, sv);
       if (ecl) {
         sp = equ_to_ecl(sp);
         sv = equ_to_ecl(sv);
@@ -51,7 +53,7 @@ __global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict_
       // Sun -> SSB at the new epoch (spk.rs:102-103)
       double jd_new = jd;
       Vec3 sp, sv;
-      st = seg_pos_vel(spk, p.sun_seg, jd_new, sp, sv);
+      st = seg_pos_vel(spk, coef, p.sun_seg, jd_new, sp, sv);
       if (ecl) {
         sp = equ_to_ecl(sp);
         sv = equ_to_ecl(sv);
@@ -70,25 +72,28 @@ __global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict_
   }
 }
 
-int32_t launch_two_body(const SpkDev* spk, const TwoBodyLaunch& p, hipStream_t stream) {
+int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaunch& p,
+                        hipStream_t stream) {
   const int64_t total = p.n * p.m;
   if (total <= 0) return 0;
   const int block = 256;
   int64_t grid = (total + block - 1) / block;
   if (grid > 65536) grid = 65536;
-  hipLaunchKernelGGL(two_body_kernel, dim3((unsigned)grid), dim3(block), 0, stream, spk, p);
+  hipLaunchKernelGGL(two_body_kernel, dim3((unsigned)grid), dim3(block), 0, stream, spk, coef,
+                     p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // State of a body chain relative to a center chain, rotated to `frame`.
-__global__ void spk_states_kernel(const SpkDev* __restrict__ spk, Chain body, Chain center, int32_t frame, int64_t m,
+__global__ void spk_states_kernel(const SpkDev* __restrict__ spk,
+                                  const double* __restrict__ coef, Chain body, Chain center, int32_t frame, int64_t m,
                                   const double* jd, double* o0, double* o1, double* o2, double* o3,
                                   double* o4, double* o5, int32_t* status) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
   Vec3 bp, bv, cp, cv;
-  int32_t st = chain_state(spk, body, jd[q], bp, bv);
-  if (st == ST_OK) st = chain_state(spk, center, jd[q], cp, cv);
+  int32_t st = chain_state(spk, coef, body, jd[q], bp, bv);
+  if (st == ST_OK) st = chain_state(spk, coef, center, jd[q], cp, cv);
   Vec3 p{bp.x - cp.x, bp.y - cp.y, bp.z - cp.z};
   Vec3 v{bv.x - cv.x, bv.y - cv.y, bv.z - cv.z};
   if (frame == 17) {
@@ -106,13 +111,13 @@ __global__ void spk_states_kernel(const SpkDev* __restrict__ spk, Chain body, Ch
   status[q] = st;
 }
 
-int32_t launch_spk_states(const SpkDev* spk, Chain body, Chain center, int32_t frame, int64_t m,
+int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center, int32_t frame, int64_t m,
                           const double* jd, double* const out[6], int32_t* status,
                           hipStream_t stream) {
   if (m <= 0) return 0;
   const int block = 256;
   hipLaunchKernelGGL(spk_states_kernel, dim3((unsigned)((m + block - 1) / block)), dim3(block), 0,
-                     stream, spk, body, center, frame, m, jd, out[0], out[1], out[2], out[3],
+                     stream, spk, coef, body, center, frame, m, jd, out[0], out[1], out[2], out[3],
                      out[4], out[5], status);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -76,12 +76,13 @@ enum : int32_t { PH_INIT = 0, PH_SUB = 1, PH_EXACT = 2 };
 // spk.rs:99-126 and state.rs:146-281): add the center's chain states in the
 // input frame, then rotate to the equatorial frame.
 __device__ __forceinline__ int32_t to_ssb_equatorial(const SpkDev* __restrict__ spk,
+                                                     const double* __restrict__ coef,
                                                      const NBodyArgs& a, double jd, Vec3& x,
                                                      Vec3& v) {
   const bool ecl = a.frame == 17;
   for (int k = 0; k < a.center.n; ++k) {
     Vec3 p, w;
-    int32_t st = seg_pos_vel(spk, a.center.seg[k], jd, p, w);
+    int32_t st = seg_pos_vel(spk, coef, a.center.seg[k], jd, p, w);
     if (st) return st;
     if (ecl) {
       p = equ_to_ecl(p);
@@ -100,11 +101,12 @@ __device__ __forceinline__ int32_t to_ssb_equatorial(const SpkDev* __restrict__ 
 // Back to the input center and frame (mod.rs:115-116): subtract the chain
 // from the root down (the find_path order from 0), then rotate.
 __device__ __forceinline__ int32_t from_ssb_equatorial(const SpkDev* __restrict__ spk,
+                                                       const double* __restrict__ coef,
                                                        const NBodyArgs& a, double jd, Vec3& x,
                                                        Vec3& v) {
   for (int k = a.center.n - 1; k >= 0; --k) {
     Vec3 p, w;
-    int32_t st = seg_pos_vel(spk, a.center.seg[k], jd, p, w);
+    int32_t st = seg_pos_vel(spk, coef, a.center.seg[k], jd, p, w);
     if (st) return st;
     x = Vec3{x.x + (-p.x), x.y + (-p.y), x.z + (-p.z)};
     v = Vec3{v.x + (-w.x), v.y + (-w.y), v.z + (-w.z)};
@@ -133,27 +135,78 @@ __device__ __forceinline__ void store_result(const NBodyArgs& a, int64_t i, int3
   if (a.counters) reinterpret_cast<int4*>(a.counters)[i] = cnt;
 }
 
-// Fetch objects until one starts cleanly (non-finite input and DAFLimits at
-// the center conversion finish immediately, as the pyo3 binding returns a NaN
-// state for them, src/kete/rust/propagation.rs:113-121).  Returns the object
-// index (>= n when none is left) and its starting state.
+// Pre-pass, one lane per object: non-finite inputs and DAFLimits at the
+// center conversion finish here with a NaN state (the pyo3 binding's
+// behaviour, src/kete/rust/propagation.rs:113-121); every other object's
+// starting state is converted to SSB-centred equatorial and parked in the
+// output arrays, where the integrator picks it up.
+__global__ void nbody_prepare_kernel(const SpkDev* __restrict__ spk,
+                                     const double* __restrict__ coef, NBodyArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const double t = a.jd0[i];
+  Vec3 x{a.xyz[0][i], a.xyz[1][i], a.xyz[2][i]};
+  Vec3 v{a.xyz[3][i], a.xyz[4][i], a.xyz[5][i]};
+  const bool finite = isfinite(t) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) &&
+                      isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
+  const int32_t st = finite ? to_ssb_equatorial(spk, coef, a, t, x, v) : ST_VALUE_ERROR;
+  store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+}
+
+// Post-pass: back to the input center and frame at jd_final (mod.rs:115-116).
+__global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
+                                    const double* __restrict__ coef, NBodyArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n || a.status[i] != ST_OK) return;
+  const double tf = a.jd_final[a.jd_final_stride ? i : 0];
+  Vec3 x{a.out[0][i], a.out[1][i], a.out[2][i]};
+  Vec3 v{a.out[3][i], a.out[4][i], a.out[5][i]};
+  const int32_t st = from_ssb_equatorial(spk, coef, a, tf, x, v);
+  if (st != ST_OK) {
+    const double nan = __builtin_nan("");
+    x = v = Vec3{nan, nan, nan};
+    a.status[i] = st;
+  }
+  a.out[0][i] = x.x;
+  a.out[1][i] = x.y;
+  a.out[2][i] = x.z;
+  a.out[3][i] = v.x;
+  a.out[4][i] = v.y;
+  a.out[5][i] = v.z;
+}
+
+// Next object whose pre-pass succeeded, from index i on; >= n when none is
+// left.  Its SSB-equatorial starting state is read from the output arrays.
 template <bool PERSISTENT>
-__device__ __forceinline__ int64_t fetch(const SpkDev* __restrict__ spk, const NBodyArgs& a,
-                                         int64_t i, double& t, double& tf, Vec3& x, Vec3& v) {
+__device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t i, double& t, double& tf,
+                                         Vec3& x, Vec3& v) {
   while (i < a.n) {
-    t = a.jd0[i];
-    tf = a.jd_final[a.jd_final_stride ? i : 0];
-    x = Vec3{a.xyz[0][i], a.xyz[1][i], a.xyz[2][i]};
-    v = Vec3{a.xyz[3][i], a.xyz[4][i], a.xyz[5][i]};
-    bool finite = isfinite(t) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) &&
-                  isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
-    int32_t st = finite ? to_ssb_equatorial(spk, a, t, x, v) : ST_VALUE_ERROR;
-    if (st == ST_OK) return i;
-    store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+    if (a.status[i] == ST_OK) {
+      t = a.jd0[i];
+      tf = a.jd_final[a.jd_final_stride ? i : 0];
+      x = Vec3{a.out[0][i], a.out[1][i], a.out[2][i]};
+      v = Vec3{a.out[3][i], a.out[4][i], a.out[5][i]};
+      return i;
+    }
     if (!PERSISTENT) return a.n;
     i = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
   }
   return i;
+}
+
+// y = x^(1/7) for the step-size ratio (radau.rs:339-342), by two Newton steps
+// from a single-precision seed; inputs outside [3e-5, 3e4] only matter through
+// the clamp to [0.25, 4] that follows, so they are clamped first.
+__device__ __forceinline__ double root7(double x) {
+  if (isnan(x)) return x;
+  const double xc = fmin(fmax(x, 3e-5), 3e4);
+  double y = (double)exp2f(log2f((float)xc) * (1.0f / 7.0f));
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double y2 = y * y, y3 = y2 * y, y6 = y3 * y3, y7 = y6 * y;
+    y = y - (y7 - xc) / (7.0 * y6);
+  }
+  return y;
 }
 
 // LDS copy of the per-substep factors, indexed by each lane's substep j.
@@ -165,6 +218,7 @@ struct LdsTables {
 
 template <bool PERSISTENT>
 __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restrict__ spk,
+                                                          const double* __restrict__ coef,
                                                           NBodyArgs a) {
   __shared__ LdsTables lds;
   for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
@@ -186,7 +240,7 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
   for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
   int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
   int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
-  obj = fetch<PERSISTENT>(spk, a, obj, t, tf, x, v);
+  obj = fetch<PERSISTENT>(a, obj, t, tf, x, v);
 
   while (obj < a.n) {
     const bool sub = phase == PH_SUB;
@@ -224,8 +278,8 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
     int32_t impact = 0;
     n_force++;
     const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
-    int32_t st = spk_accel(spk, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
-                           ngp, acc, impact);
+    int32_t st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]},
+                           !sub, ngp, acc, impact);
     const double ev[3] = {acc.x, acc.y, acc.z};
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
@@ -298,7 +352,7 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
       // next step size (radau.rs:339-342)
       double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
       double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
-      double ratio = pow(eps * fm / bm, 1.0 / 7.0);
+      double ratio = root7(eps * fm / bm);
       ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
       h_next = h * ratio;
     }
@@ -358,15 +412,13 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
 
     // ---- finished objects: store, then take the next one ----
     if (done >= 0) {
-      Vec3 xo = x, vo = v;
-      int32_t fst = done;
-      if (fst == ST_OK) fst = from_ssb_equatorial(spk, a, tf, xo, vo);
-      store_result(a, obj, fst, xo, vo, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      // SSB-equatorial result; the post-pass converts it back
+      store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
       int64_t nxt = a.n;
       if (PERSISTENT) nxt = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
       double nt, ntf;
       Vec3 nx, nv;
-      obj = fetch<PERSISTENT>(spk, a, nxt, nt, ntf, nx, nv);
+      obj = fetch<PERSISTENT>(a, nxt, nt, ntf, nx, nv);
       t = nt;
       tf = ntf;
       x = nx;
@@ -387,8 +439,9 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 
-int32_t launch_nbody(const SpkDev* spk, const NBodyLaunch& p, hipStream_t stream,
-                     int32_t* used_block, int32_t* used_grid, int32_t* used_mode) {
+int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
+                     hipStream_t stream, int32_t* used_block, int32_t* used_grid,
+                     int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1) {
   NBodyArgs a;
   a.n = p.n;
   a.jd0 = p.jd0;
@@ -408,20 +461,27 @@ int32_t launch_nbody(const SpkDev* spk, const NBodyLaunch& p, hipStream_t stream
   a.counters = p.counters;
   a.queue = p.queue;
   const int block = p.block > 0 ? p.block : 256;
+  const unsigned g1 = (unsigned)((p.n + 255) / 256);
+  hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   int64_t grid;
   if (p.mode == 1) {
     grid = p.grid > 0 ? p.grid : 1024;
     if (grid * block > p.n) grid = (p.n + block - 1) / block;
     a.first_dynamic = grid * block;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
+    if (ev0) (void)hipEventRecord(ev0, stream);
     hipLaunchKernelGGL(radau_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), 0, stream,
-                       spk, a);
+                       spk, coef, a);
+    if (ev1) (void)hipEventRecord(ev1, stream);
   } else {
     grid = (p.n + block - 1) / block;
     a.first_dynamic = p.n;
+    if (ev0) (void)hipEventRecord(ev0, stream);
     hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), 0, stream,
-                       spk, a);
+                       spk, coef, a);
+    if (ev1) (void)hipEventRecord(ev1, stream);
   }
+  hipLaunchKernelGGL(nbody_finish_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   if (used_block) *used_block = block;
   if (used_grid) *used_grid = (int32_t)grid;
   if (used_mode) *used_mode = p.mode == 1 ? 1 : 0;

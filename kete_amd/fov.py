@@ -1,0 +1,142 @@
+"""Field-of-view checks: ``RectangleFOV`` and ``fov_state_check``.
+
+Mirrors kete's ``RectangleFOV`` (``src/kete/rust/fovs/definitions.rs:
+181-196,401-480``) and ``fov_state_check`` (``src/kete/rust/fovs/checks.rs:
+24-110``) over the C ABI; the work runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .spk import FRAME_ECLIPTIC, FRAME_EQUATORIAL
+
+_OBL = 0.40909280422232897
+
+
+def _equ_to_ecl(v):
+    c, s = math.cos(_OBL), math.sin(_OBL)
+    v = np.asarray(v, dtype=np.float64)
+    return np.array([v[0], c * v[1] + s * v[2], -s * v[1] + c * v[2]])
+
+
+@dataclass
+class RectangleFOV:
+    """A rectangular field: pointing vector, rotation, widths (degrees, as
+    kete's constructor takes them), and the observer's Sun-centred state."""
+    pointing: np.ndarray
+    rotation: float
+    observer_jd: float
+    observer_pos: np.ndarray
+    observer_vel: np.ndarray
+    lon_width: float
+    lat_width: float
+    frame: int = FRAME_ECLIPTIC
+
+    def normals(self) -> np.ndarray:
+        L = _lib.load()
+        p = np.ascontiguousarray(self.pointing, dtype=np.float64).reshape(3)
+        out = np.zeros(12)
+        L.kh_fov_rect_normals(_lib.dptr(p), math.radians(self.rotation),
+                              math.radians(self.lon_width), math.radians(self.lat_width),
+                              _lib.dptr(out))
+        return out.reshape(4, 3)
+
+    def ecliptic(self):
+        """Field and observer converted to the ecliptic frame
+        (try_frame_change_mut, collection.rs:23-36)."""
+        nrm = self.normals()
+        if self.frame == FRAME_EQUATORIAL:
+            nrm = np.array([_equ_to_ecl(n) for n in nrm])
+            return nrm, _equ_to_ecl(self.observer_pos), _equ_to_ecl(self.observer_vel)
+        return nrm, np.asarray(self.observer_pos, float), np.asarray(self.observer_vel, float)
+
+    @property
+    def jd(self):
+        return self.observer_jd
+
+
+@dataclass
+class FovHits:
+    """Visible objects: one row per (field, object), ordered as kete returns
+    them (fields by epoch, objects by input index)."""
+    field: np.ndarray
+    obj: np.ndarray
+    state: np.ndarray  # [7, n_hits]: jd, pos, vel (ecliptic, Sun-centred)
+
+    def __len__(self):
+        return len(self.field)
+
+
+def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=None,
+                    cap=None, fields=None) -> FovHits:
+    """kete ``fov_state_check``: which objects are visible in which fields.
+
+    ``obj_state`` is a :class:`~kete_amd.StateArray` (one epoch, Sun-centred,
+    ecliptic); ``fovs`` a list of :class:`RectangleFOV`, or ``fields`` a
+    prepared tuple ``(jd[nf], obs[6, nf], normals[nf, 12])`` in the ecliptic
+    frame.
+    """
+    if include_asteroids:
+        raise NotImplementedError("include_asteroids needs SPK Type 21 perturbers")
+    from . import get_context
+    ctx = ctx or get_context()
+    L = ctx.L
+    if obj_state.frame != FRAME_ECLIPTIC or obj_state.center_id != 10:
+        raise ValueError("states must be Sun-centred (10) in the ecliptic frame")
+    jd0 = float(obj_state.jd[0])
+    if np.any(obj_state.jd != jd0):
+        raise ValueError("all states must share one epoch (SimultaneousStates)")
+    if fields is None:
+        nf = len(fovs)
+        fjd = np.array([f.jd for f in fovs], dtype=np.float64)
+        obs = np.empty((6, nf))
+        nrm = np.empty((nf, 12))
+        for k, f in enumerate(fovs):
+            n_, p_, v_ = f.ecliptic()
+            nrm[k] = n_.reshape(12)
+            obs[:3, k] = p_
+            obs[3:, k] = v_
+    else:
+        fjd, obs, nrm = fields
+        fjd = np.ascontiguousarray(fjd, dtype=np.float64)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float64).reshape(-1, 12)
+        nf = len(fjd)
+    n = len(obj_state)
+    cap = cap or max(1 << 16, n // 100 * max(nf, 1) // 10 + 4096)
+    while True:
+        hf = np.empty(cap, dtype=np.int64)
+        ho = np.empty(cap, dtype=np.int64)
+        hs = np.empty((7, cap))
+        nh = C.c_int64(0)
+        rc = L.kh_fov_state_check(ctx.h, n, jd0, _lib.ptr_array(list(obj_state.xyz)),
+                                  FRAME_ECLIPTIC, nf, _lib.dptr(fjd),
+                                  _lib.ptr_array([obs[k] for k in range(6)]),
+                                  _lib.dptr(nrm.reshape(-1)), float(dt_limit), cap, C.byref(nh),
+                                  _lib.iptr(hf, C.c_int64), _lib.iptr(ho, C.c_int64),
+                                  _lib.ptr_array([hs[k] for k in range(7)]))
+        if rc == -1 and nh.value > cap:
+            cap = nh.value
+            continue
+        _lib.check(rc)
+        k = nh.value
+        return FovHits(hf[:k].copy(), ho[:k].copy(), hs[:, :k].copy())
+
+
+def ra_dec_vmag(ctx, state, obs, h_mag, g_param=0.15, frame=FRAME_ECLIPTIC):
+    """RA, Dec, RA rate x cos(Dec), Dec rate (rad, rad/day) and V magnitude per
+    detection -> array [5, n]."""
+    state = np.ascontiguousarray(state, dtype=np.float64)
+    obs = np.ascontiguousarray(obs, dtype=np.float64)
+    h = np.ascontiguousarray(h_mag, dtype=np.float64)
+    n = state.shape[1]
+    out = np.empty((5, n))
+    _lib.check(ctx.L.kh_radec_vmag(ctx.h, n, _lib.ptr_array(list(state)),
+                                   _lib.ptr_array(list(obs)), frame, _lib.dptr(h),
+                                   float(g_param), _lib.ptr_array([out[k] for k in range(5)])))
+    return out

@@ -332,3 +332,83 @@ def uneven_spans(n, seed_index=3):
     span = np.exp(rng.uniform(0.0, np.log(3650.0), n))
     sign = np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0)
     return span * sign
+
+
+# --------------------------------------------------------------------------
+# Field-of-view workload
+# --------------------------------------------------------------------------
+def body_state(table, obj_id, jd, center=10, ecliptic=True):
+    """State of a body w.r.t. ``center`` from a :class:`Type2Table`, by summing
+    segment chains to the barycentre (numpy, host side)."""
+    def chain(i):
+        out = []
+        while i != 0:
+            k = table.find(i)
+            out.append(k)
+            i = table.segments[k].center_id
+        return out
+
+    jd = np.atleast_1d(np.asarray(jd, dtype=np.float64))
+    tot = np.zeros((6, len(jd)))
+    for sign, ident in ((1.0, obj_id), (-1.0, center)):
+        for k in chain(ident):
+            p, v = table.state(table.segments[k].obj_id, jd)
+            tot[:3] += sign * p
+            tot[3:] += sign * v
+    if ecliptic:
+        c, s = np.cos(OBLIQUITY), np.sin(OBLIQUITY)
+        for o in (0, 3):
+            y, z = tot[o + 1].copy(), tot[o + 2].copy()
+            tot[o + 1] = c * y + s * z
+            tot[o + 2] = -s * y + c * z
+    return tot
+
+
+def fov_workload(n=1_000_000, n_fields=3000, span_days=365.25, seed_index=4, aimed=0.0,
+                 width_deg=6.9, normals_fn=None):
+    """States (config-1 main-belt distribution at EPOCH_JD) and rectangular
+    fields: centres uniform on the sphere (a fraction ``aimed`` instead points
+    at a random object's two-body position), epochs uniform over the span,
+    observer the synthetic Earth.  Returns xyz[6, n], field jd[nf],
+    obs[6, nf], pointing[nf, 3], H[n]."""
+    from .spk import Type2Table, read_spk
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + seed_index))
+    xyz = main_belt(rng, n)
+    fjd = np.sort(EPOCH_JD + rng.uniform(0.0, span_days, n_fields))
+    pts = rng.normal(size=(n_fields, 3))
+    pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+    table = Type2Table(read_spk(ephemeris_path()))
+    obs = body_state(table, 399, fjd)
+    n_aim = int(round(aimed * n_fields))
+    if n_aim:
+        which = rng.integers(0, n, n_aim)
+        for k, i in enumerate(which):
+            # two-body position at the field epoch (rough aim is enough)
+            p, _ = _kepler_np(xyz[:, i], fjd[k] - EPOCH_JD)
+            d = p - obs[:3, k]
+            d /= np.linalg.norm(d)
+            jitter = rng.normal(size=3) * np.radians(width_deg) * 0.2
+            pts[k] = (d + jitter) / np.linalg.norm(d + jitter)
+    h_mag = rng.uniform(12.0, 22.0, n)
+    return xyz, fjd, obs, pts, h_mag
+
+
+def _kepler_np(state, dt):
+    """Two-body propagation of one state by dt days (numpy, for aiming only)."""
+    r0 = state[:3].copy()
+    v0 = state[3:].copy()
+    mu = GMS
+    r = np.linalg.norm(r0)
+    a = 1.0 / (2.0 / r - v0 @ v0 / mu)
+    n = np.sqrt(mu / a ** 3)
+    h = np.cross(r0, v0)
+    evec = np.cross(v0, h) / mu - r0 / r
+    e = np.linalg.norm(evec)
+    cosE0 = (1 - r / a) / e
+    sinE0 = (r0 @ v0) / (e * np.sqrt(mu * a))
+    E0 = np.arctan2(sinE0, cosE0)
+    M = E0 - e * np.sin(E0) + n * dt
+    E = _kepler_E(np.array([np.mod(M, 2 * np.pi)]), np.array([e]))[0]
+    f = a / r * (np.cos(E - E0) - 1) + 1
+    g = dt + (np.sin(E - E0) - (E - E0)) / n
+    return f * r0 + g * v0, None

@@ -1,0 +1,92 @@
+"""GPU parity of fov_state_check (kete's field-of-view path with light time)
+and the RA/Dec + V-magnitude epilogue, against the oracle.
+
+Visible sets must be identical except for objects within 1e-9 rad of a field
+edge; light-time-corrected states agree to 1e-9 AU; RA/Dec to 1e-9 rad.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import kete_amd as ka
+from kete_amd import synthetic as syn
+from kete_amd.fov import fov_state_check, ra_dec_vmag
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def workload():
+    xyz, fjd, obs, pts, h = syn.fov_workload(n=3000, n_fields=24, span_days=40.0, aimed=0.75,
+                                             seed_index=4)
+    import oracle
+    w = math.radians(6.9)
+    nrm = np.array([oracle.rect_normals(p, 0.0, w, w).reshape(12) for p in pts])
+    return xyz, fjd, obs, nrm, h
+
+
+def edge_distance(nrm, obs, state):
+    """Angular distance (rad) of the object's direction from the nearest
+    edge plane of the field."""
+    rel = state[1:4] - obs[:3]
+    u = rel / np.linalg.norm(rel)
+    n = nrm.reshape(4, 3)
+    return np.min(np.abs(n @ u) / np.linalg.norm(n, axis=1))
+
+
+def test_fov_parity(ctx, oracle_spk, workload):
+    xyz, fjd, obs, nrm, h = workload
+    n = xyz.shape[1]
+    batch = ka.StateArray(np.full(n, syn.EPOCH_JD), xyz)
+    hits = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    of, oo, os_ = oracle_spk.fov_state_check(syn.EPOCH_JD, xyz, fjd, obs.T.copy(), nrm,
+                                             dt_limit=3.0)
+    assert len(of) > 50, "workload should produce detections"
+    g = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(hits.field, hits.obj))}
+    r = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(of, oo))}
+    for key in set(g) ^ set(r):
+        f, i = key
+        st = os_[r[key]] if key in r else hits.state[:, g[key]]
+        assert edge_distance(nrm[f], obs[:, f], st) < 1e-9, key
+    common = sorted(set(g) & set(r))
+    gs = np.array([hits.state[:, g[k]] for k in common])
+    rs = np.array([os_[r[k]] for k in common])
+    assert np.abs(gs[:, 0] - rs[:, 0]).max() < 1e-9        # light-time epoch (days)
+    assert np.abs(gs[:, 1:4] - rs[:, 1:4]).max() < 1e-9    # position (AU)
+    # order: fields by epoch, objects ascending (as kete returns them)
+    keys = list(zip(hits.field, hits.obj))
+    order = sorted(range(len(keys)), key=lambda k: (fjd[keys[k][0]], keys[k][0], keys[k][1]))
+    assert order == list(range(len(keys)))
+
+
+def test_radec_vmag_parity(ctx, oracle_spk, workload):
+    xyz, fjd, obs, nrm, h = workload
+    n = xyz.shape[1]
+    batch = ka.StateArray(np.full(n, syn.EPOCH_JD), xyz)
+    hits = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    st = hits.state[1:7]
+    ob = obs[:, hits.field]
+    out = ra_dec_vmag(ctx, st, ob, h[hits.obj], 0.15)
+    c, s = math.cos(0.40909280422232897), math.sin(0.40909280422232897)
+
+    def equ(v):
+        return np.array([v[0], c * v[1] - s * v[2], s * v[1] + c * v[2]])
+
+    for k in range(0, len(hits), max(1, len(hits) // 200)):
+        ref = oracle_spk.ra_dec_with_rates(equ(st[:3, k]), equ(st[3:, k]), equ(ob[:3, k]),
+                                           equ(ob[3:, k]))
+        assert np.abs(out[:2, k] - ref[:2]).max() < 1e-12
+        assert np.abs(out[2:4, k] - ref[2:4]).max() < 1e-12
+        v = oracle_spk.hg_apparent_mag(h[hits.obj[k]], 0.15, st[:3, k], ob[:3, k])
+        assert abs(out[4, k] - v) < 1e-9
+
+
+def test_fov_empty_and_far_fields(ctx, workload):
+    xyz, fjd, obs, nrm, h = workload
+    batch = ka.StateArray(np.full(10, syn.EPOCH_JD), xyz[:, :10])
+    none = fov_state_check(batch, None, fields=(fjd[:0], obs[:, :0], nrm[:0]))
+    assert len(none) == 0
+    # a field 200 days after the states' epoch exercises the re-propagation
+    far = fov_state_check(batch, None, fields=(fjd[:1] + 200.0, obs[:, :1], nrm[:1]))
+    assert far.field.shape == far.obj.shape
