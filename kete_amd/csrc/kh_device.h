@@ -522,28 +522,51 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     }
     if (st) return st;
     Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
-    double r = norm(rp);
+    const double r = norm(rp);
     if (exact && r <= bd.radius) {
       impact_id = bd.id;
       return ST_IMPACT;
     }
-    double k = mass * inv_cube(r);
+    // one division per body: 1/r, then 1/r^3 = (1/r)^3 (kete: r.powi(-3))
+    const double inv_r = 1.0 / r;
+    const double inv_r3 = inv_r * inv_r * inv_r;
+    const double k = mass * inv_r3;
     a.x -= rp.x * k;
     a.y -= rp.y * k;
     a.z -= rp.z * k;
     if (bd.kind == 1) {
-      Vec3 rv{v.x - bv.x, v.y - bv.y, v.z - bv.z};
-      gr_correction(a, rp, rv, mass);
-      Vec3 j = ecl_to_equ(j2_correction(equ_to_ecl(rp), bd.radius, bd.j2, mass));
+      // apply_gr_correction (gravity.rs:239-253)
+      const Vec3 rv{v.x - bv.x, v.y - bv.y, v.z - bv.z};
+      const double r_v = 4.0 * dot(rp, rv);
+      const double v2 = dot(rv, rv);
+      const double gc = mass * C_AU_PER_DAY_INV_SQUARED * inv_r3;
+      const double c = 4. * mass * inv_r - v2;
+      a.x += gc * (c * rp.x + r_v * rv.x);
+      a.y += gc * (c * rp.y + r_v * rv.y);
+      a.z += gc * (c * rp.z + r_v * rv.z);
+      // j2_correction in the ecliptic frame and back (gravity.rs:184-201,223-235);
+      // |rp| is the same in both frames
+      const Vec3 pe = equ_to_ecl(rp);
+      const double zr = pe.z * inv_r;
+      const double z_squared = 5.0 * (zr * zr);
+      const double rr = bd.radius * inv_r;
+      const double coef = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
+      const Vec3 j = ecl_to_equ(Vec3{pe.x * coef * (z_squared - 1.0),
+                                     pe.y * coef * (z_squared - 1.0),
+                                     pe.z * coef * (z_squared - 3.0)});
       a.x += j.x;
       a.y += j.y;
       a.z += j.z;
       if (bd.id == 10 && ng != nullptr) nongrav_add(*ng, a, rp, rv);
     } else if (bd.kind == 2) {
-      Vec3 j = j2_correction(rp, bd.radius, bd.j2, mass);
-      a.x += j.x;
-      a.y += j.y;
-      a.z += j.z;
+      // Earth J2 in the equatorial frame (gravity.rs:203)
+      const double zr = rp.z * inv_r;
+      const double z_squared = 5.0 * (zr * zr);
+      const double rr = bd.radius * inv_r;
+      const double coef = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
+      a.x += rp.x * coef * (z_squared - 1.0);
+      a.y += rp.y * coef * (z_squared - 1.0);
+      a.z += rp.z * coef * (z_squared - 3.0);
     }
   }
   acc = a;

@@ -217,7 +217,8 @@ struct LdsTables {
 };
 
 template <bool PERSISTENT>
-__global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restrict__ spk,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+radau_nbody_kernel(const SpkDev* __restrict__ spk,
                                                           const double* __restrict__ coef,
                                                           NBodyArgs a) {
   __shared__ LdsTables lds;
@@ -303,29 +304,25 @@ __global__ void __launch_bounds__(256) radau_nbody_kernel(const SpkDev* __restri
     }
     // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
     const bool es = sub && j == 7 && st == ST_OK;
-    double fmx[3], rmax = 0.0;
-    {
-      double Bn[7][3];
+    // B6 of the previous sweep (radau.rs:233); B is rewritten column by
+    // column so no second copy of it is ever live.
+    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
 #pragma unroll
-      for (int c = 0; c < 7; ++c) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          double acc2 = G[c][d];
-#pragma unroll
-          for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
-          Bn[c][d] = acc2;
-        }
-      }
+    for (int c = 0; c < 7; ++c) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        fmx[d] = fabs(ev[d]) + 1e-6;
-        double r = fabs(Bn[6][d] - B[6][d]) / fmx[d];
-        rmax = d == 0 ? r : fmax(rmax, r);
+        double acc2 = G[c][d];
+#pragma unroll
+        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
+        B[c][d] = es ? acc2 : B[c][d];
       }
+    }
+    double fmx[3], rmax = 0.0;
 #pragma unroll
-      for (int c = 0; c < 7; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) B[c][d] = es ? Bn[c][d] : B[c][d];
+    for (int d = 0; d < 3; ++d) {
+      fmx[d] = fabs(ev[d]) + 1e-6;
+      const double r = fabs(B[6][d] - bp6[d]) / fmx[d];
+      rmax = d == 0 ? r : fmax(rmax, r);
     }
     const bool conv = es && rmax < 1e-14;
     // ---- accepted step (radau.rs:320-338) ----
