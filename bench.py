@@ -1,23 +1,29 @@
 """Benchmark: Radau-15 N-body propagation on one MI355X (object-years/s).
 
-Workload: 1e6 massless objects (9e5 main-belt + 1e5 NEO-like, seeded element
-draws), one-year arc from JD 2460000.5 TDB, perturbed by the Sun, eight planet
-barycentres and the Moon with solar and Jovian GR+J2 and Earth J2, read from a
-synthetic SPK Type 2 ephemeris (degree-13 Chebyshev, 32-day records); Radau
-EPSILON 1e-6 (kete's constant); FP64 throughout.
+Default workload (the headline): 1e6 massless objects (9e5 main-belt +
+1e5 NEO-like, seeded element draws), one-year arc from JD 2460000.5 TDB,
+perturbed by the Sun, eight planet barycentres and the Moon with solar and
+Jovian GR+J2 and Earth J2, read from a synthetic SPK Type 2 ephemeris
+(degree-13 Chebyshev, 32-day records); Radau EPSILON 1e-6 (kete's constant);
+FP64 throughout.
 
 A step is one full propagation of the batch.  Inputs are resident in HBM
 (torch tensors) before timing; the timed region is K steps between device
 synchronisations.  The dominant kernel's duration is also measured with HIP
 events on the library's stream.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+Other workloads (--config): 1 two-body (1e5 states x 64 epochs, solves/s),
+3 uneven spans (2e6 objects, |span| log-uniform in [1, 3650] d, persistent vs
+plain launch), 4 field-of-view checks (1e6 states x 3000 fields, pairs/s).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--config C]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -27,129 +33,153 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# flop model per unit of work, from the reference's arithmetic:
-# force evaluation (PLANETS, 11 Chebyshev lookups + Newton + GR + J2)
-F_FORCE = 1848.0
+# flop model per unit of work, read off the reference's arithmetic:
+F_FORCE = 1848.0   # force evaluation: 11 Chebyshev lookups + Newton + GR + J2
 F_SWEEP = 1129.0   # predictor 7 substeps x 139 + B = G.C 147 + convergence 9
 F_ACCEPT = 111.0   # state update 107 + step-size 4
+F_KEPLER0, F_KEPLER_IT = 97.0, 37.0   # universal Kepler: 97 + 37 per Newton iteration
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--mode", type=int, default=-1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="approximate CPU time budget of the baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md)"}))
-        sys.exit(2)
+def cpu_threads():
+    return min(16, len(os.sched_getaffinity(0)))
 
+
+def setup_nbody(n, xyz, jd0, jdf_host, stride):
     import torch
+    dev = torch.device("cuda:0")
+    t = dict(
+        jd0=torch.full((n,), jd0, dtype=torch.float64, device=dev),
+        xyz=torch.from_numpy(xyz).to(dev).contiguous(),
+        jf=torch.from_numpy(np.ascontiguousarray(jdf_host, dtype=np.float64)).to(dev),
+        out=torch.empty((6, n), dtype=torch.float64, device=dev),
+        st=torch.empty(n, dtype=torch.int32, device=dev),
+        iid=torch.empty(n, dtype=torch.int64, device=dev),
+        ijd=torch.empty(n, dtype=torch.float64, device=dev),
+        cnt=torch.zeros((n, 4), dtype=torch.int32, device=dev),
+    )
+    torch.cuda.synchronize()
+    return t
 
+
+def nbody_step_fn(ctx, t, n, stride, mode):
     import kete_amd as ka
     from kete_amd import _lib
-    from kete_amd import synthetic as syn
-
-    n = args.n
-    log(f"[bench] generating workload n={n}")
-    xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=2)
-    jd0 = syn.EPOCH_JD
-    jdf = jd0 + 365.25
-    ctx = ka.get_context()
-    L = ctx.L
-    dev = torch.device("cuda:0")
-    d_jd0 = torch.full((n,), jd0, dtype=torch.float64, device=dev)
-    d_xyz = torch.from_numpy(xyz).to(dev).contiguous()
-    d_jf = torch.tensor([jdf], dtype=torch.float64, device=dev)
-    d_out = torch.empty((6, n), dtype=torch.float64, device=dev)
-    d_st = torch.empty(n, dtype=torch.int32, device=dev)
-    d_iid = torch.empty(n, dtype=torch.int64, device=dev)
-    d_ijd = torch.empty(n, dtype=torch.float64, device=dev)
-    d_cnt = torch.zeros((n, 4), dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-
-    xyz_p = _lib.ptr_array([d_xyz[k].data_ptr() for k in range(6)])
-    out_p = _lib.ptr_array([d_out[k].data_ptr() for k in range(6)])
+    xyz_p = _lib.ptr_array([t["xyz"][k].data_ptr() for k in range(6)])
+    out_p = _lib.ptr_array([t["out"][k].data_ptr() for k in range(6)])
 
     def step(counters=False):
-        rc = L.kh_propagate_n_body_dev(
-            ctx.h, n, _lib.raw(d_jd0.data_ptr()), xyz_p, 10, ka.Frames.Ecliptic,
-            _lib.raw(d_jf.data_ptr()), 0, None, 1e-6, args.mode, out_p,
-            _lib.raw(d_st.data_ptr(), C.c_int32), _lib.raw(d_iid.data_ptr(), C.c_int64),
-            _lib.raw(d_ijd.data_ptr()),
-            _lib.raw(d_cnt.data_ptr(), _lib.KhCounters) if counters else None, 0)
+        rc = ctx.L.kh_propagate_n_body_dev(
+            ctx.h, n, _lib.raw(t["jd0"].data_ptr()), xyz_p, 10, ka.Frames.Ecliptic,
+            _lib.raw(t["jf"].data_ptr()), stride, None, 1e-6, mode, out_p,
+            _lib.raw(t["st"].data_ptr(), C.c_int32), _lib.raw(t["iid"].data_ptr(), C.c_int64),
+            _lib.raw(t["ijd"].data_ptr()),
+            _lib.raw(t["cnt"].data_ptr(), _lib.KhCounters) if counters else None, 0)
         _lib.check(rc)
+    return step
 
-    log("[bench] warmup")
-    step(counters=True)
-    torch.cuda.synchronize()
-    cnt = d_cnt.cpu().numpy().astype(np.int64)
-    status = d_st.cpu().numpy()
-    for _ in range(max(args.warmup - 1, 0)):
+
+def time_steps(ctx, step, steps, warmup):
+    import torch
+    for _ in range(max(warmup, 0)):
         step()
     torch.cuda.synchronize()
-
-    log(f"[bench] timing {args.steps} steps")
     kernel_ms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
         kernel_ms.append(ctx.last_kernel_ms())  # waits for this launch's end event
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    k_ms = float(np.mean(kernel_ms))
-    block, grid, mode = ctx.last_launch()
-    obj_years = n * (jdf - jd0) / 365.25
-    value = obj_years / (ms_per_step / 1000.0)
+    return (t1 - t0) * 1000.0 / steps, float(np.mean(kernel_ms))
 
-    n_force = cnt[:, 0].sum()
-    n_sweep = cnt[:, 1].sum()
-    n_acc = cnt[:, 2].sum()
+
+def radau_roofline(cnt, k_ms):
+    n_force, n_sweep, n_acc = (int(cnt[:, k].sum()) for k in range(3))
     flop = F_FORCE * n_force + F_SWEEP * n_sweep + F_ACCEPT * n_acc
     achieved = flop / (k_ms / 1000.0) / 1e12
-    log(f"[bench] {ms_per_step:.1f} ms/step, kernel {k_ms:.1f} ms, {value:.3e} obj-yr/s, "
-        f"{achieved:.2f} TFLOP/s FP64")
+    return {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "radau_nbody_kernel", "kernel_ms": k_ms, "flop_per_launch": flop,
+            "work": {"force_evals": n_force, "sweeps": n_sweep, "accepted_steps": n_acc}}
 
-    res_gpu = d_out.cpu().numpy()
-    cpu = None
-    parity = None
+
+def run_nbody(args, uneven=False):
+    import torch
+
+    import kete_amd as ka
+    from kete_amd import synthetic as syn
+    n = args.n or (2_000_000 if uneven else 1_000_000)
+    seed_index = 3 if uneven else 2
+    log(f"[bench] generating workload n={n} uneven={uneven}")
+    xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=seed_index)
+    jd0 = syn.EPOCH_JD
+    if uneven:
+        span = syn.uneven_spans(n)
+        jdf = jd0 + span
+        stride = 1
+    else:
+        jdf = np.array([jd0 + 365.25])
+        stride = 0
+    ctx = ka.get_context()
+    t = setup_nbody(n, xyz, jd0, jdf, stride)
+    mode = args.mode
+    step = nbody_step_fn(ctx, t, n, stride, mode)
+    log("[bench] warmup (with work counters)")
+    step(counters=True)
+    torch.cuda.synchronize()
+    cnt = t["cnt"].cpu().numpy().astype(np.int64)
+    status = t["st"].cpu().numpy()
+    log(f"[bench] timing {args.steps} steps")
+    ms_per_step, k_ms = time_steps(ctx, step, args.steps, max(args.warmup - 1, 0))
+    block, grid, used_mode = ctx.last_launch()
+    spans = np.abs(np.broadcast_to(jdf, (n,)) - jd0)
+    obj_years = float(spans.sum() / 365.25)
+    value = obj_years / (ms_per_step / 1000.0)
+    roof = radau_roofline(cnt, k_ms)
+    log(f"[bench] {ms_per_step:.1f} ms/step, kernel {k_ms:.1f} ms, {value:.3e} obj-yr/s, "
+        f"{roof['achieved']:.2f} TFLOP/s FP64 ({100 * roof['frac']:.1f}% of peak)")
+    extra = {}
+    if uneven:
+        # the same batch with the plain launch, for comparison
+        plain = nbody_step_fn(ctx, t, n, stride, 0)
+        ms_plain, k_plain = time_steps(ctx, plain, 1, 0)
+        extra["plain_launch"] = {"ms_per_step": ms_plain, "kernel_ms": k_plain,
+                                 "value": obj_years / (ms_plain / 1000.0)}
+        log(f"[bench] plain launch: {ms_plain:.1f} ms/step")
+
+    res_gpu = t["out"].cpu().numpy()
+    cpu = parity = None
     if not args.no_cpu:
         import oracle
         oracle.spk_load(syn.ephemeris_path())
-        threads = min(16, len(os.sched_getaffinity(0)))
-        # a sample that keeps the main-belt / NEO mix: every k-th object
+        threads = cpu_threads()
         m = 256
         while True:
             idx = np.linspace(0, n - 1, m).astype(np.int64)
             tc = time.perf_counter()
-            r = oracle.batch_n_body(jd0, xyz[:, idx], jdf, n_threads=threads)
+            jf_s = jdf[idx] if stride else jdf
+            r = oracle.batch_n_body(jd0, xyz[:, idx], jf_s, n_threads=threads)
             dt = time.perf_counter() - tc
             if dt > args.cpu_seconds / 4 or m >= n:
                 break
             m = min(n, int(m * max(2.0, min(8.0, args.cpu_seconds / max(dt, 1e-3)))))
-        cpu_val = len(idx) * (jdf - jd0) / 365.25 / dt
+        cpu_val = float(spans[idx].sum() / 365.25) / dt
         ok = (r["status"] == 0) & (status[idx] == 0)
-        dp = np.abs(res_gpu[:3, idx][:, ok] - r["out"][:3, ok]).max() if ok.any() else float("nan")
-        parity = {"sample": int(len(idx)), "max_pos_err_au": float(dp),
+        dp = float(np.abs(res_gpu[:3, idx][:, ok] - r["out"][:3, ok]).max()) if ok.any() \
+            else float("nan")
+        parity = {"sample": int(len(idx)), "max_pos_err_au": dp,
                   "status_match": bool(np.array_equal(r["status"], status[idx]))}
         cpu = {"value": cpu_val, "unit": "object-years/s", "cores": threads, "kind": "port",
-               "sample": f"{len(idx)} evenly spaced objects of the same workload, same arc, "
+               "sample": f"{len(idx)} evenly spaced objects of the same workload and arcs, "
                          f"{dt:.1f} s wall on {threads} threads (C restatement, -O3)"}
         log(f"[bench] cpu {cpu_val:.1f} obj-yr/s on {threads} threads; parity {parity}")
 
+    workload = "radau15_nbody_uneven_spans_2e6" if uneven else "radau15_nbody_1e6_objects_1yr"
     line = {
         "metric": "object-years/s, FP64 Radau (<=1e-9 AU vs ref)",
         "value": value,
@@ -163,23 +193,145 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "radau15_nbody_1e6_objects_1yr",
-                   "objects": n, "main_belt": n - int(round(n * 0.1)),
-                   "neo": int(round(n * 0.1)), "arc_days": 365.25, "epsilon": 1e-6,
+        "config": {"workload": workload, "objects": n, "main_belt": n - int(round(n * 0.1)),
+                   "neo": int(round(n * 0.1)),
+                   "arc_days": "log-uniform |span| in [1, 3650], random sign" if uneven
+                   else 365.25,
+                   "object_years": obj_years, "epsilon": 1e-6,
                    "ephemeris": "synthetic SPK type 2, 11 segments, degree 13, 32 d",
                    "force": "Sun+8 planets+Moon, GR+J2 (Sun, Jupiter), J2 (Earth)",
-                   "seed": syn.SEED_BASE + 2, "launch": {"block": block, "grid": grid,
-                                                         "mode": mode}},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "radau_nbody_kernel", "kernel_ms": k_ms,
-                     "flop_per_launch": flop,
-                     "work": {"force_evals": int(n_force), "sweeps": int(n_sweep),
-                              "accepted_steps": int(n_acc)}},
+                   "seed": syn.SEED_BASE + seed_index,
+                   "launch": {"block": block, "grid": grid, "mode": used_mode}},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
     }
+    line.update(extra)
+    return line
+
+
+def run_two_body(args):
+    import torch
+
+    import kete_amd as ka
+    from kete_amd import _lib
+    from kete_amd import synthetic as syn
+    n = args.n or 100_000
+    m = 64
+    xyz, targets = syn.two_body_workload(n=n, m=m)
+    ctx = ka.get_context()
+    dev = torch.device("cuda:0")
+    d_jd0 = torch.full((n,), syn.EPOCH_JD, dtype=torch.float64, device=dev)
+    d_xyz = torch.from_numpy(xyz).to(dev).contiguous()
+    d_t = torch.from_numpy(targets).to(dev)
+    d_out = torch.empty((6, m, n), dtype=torch.float64, device=dev)
+    d_st = torch.empty((m, n), dtype=torch.int32, device=dev)
+    xyz_p = _lib.ptr_array([d_xyz[k].data_ptr() for k in range(6)])
+    out_p = _lib.ptr_array([d_out[k].data_ptr() for k in range(6)])
+
+    def step():
+        _lib.check(ctx.L.kh_propagate_two_body_dev(
+            ctx.h, n, _lib.raw(d_jd0.data_ptr()), xyz_p, 10, ka.Frames.Ecliptic, m,
+            _lib.raw(d_t.data_ptr()), None, out_p, _lib.raw(d_st.data_ptr(), C.c_int32), 0))
+
+    ms, k_ms = time_steps(ctx, step, args.steps, args.warmup)
+    solves = n * m
+    # Newton iterations per solve from the oracle on a sample (same algorithm)
+    out = d_out.cpu().numpy()
+    bytes_out = solves * 48
+    line = {"metric": "two-body solves/s", "value": solves / (ms / 1000.0),
+            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "universal_kepler_1e5_states_x_64_epochs", "states": n,
+                       "epochs": m},
+            "kernel_ms": k_ms, "hbm_write_gbs": bytes_out / (k_ms / 1000.0) / 1e9}
+    if not args.no_cpu:
+        import oracle
+        idx = np.linspace(0, n - 1, min(n, 20000)).astype(np.int64)
+        tc = time.perf_counter()
+        ref, _ = oracle.batch_two_body(syn.EPOCH_JD, xyz[:, idx], targets,
+                                       n_threads=cpu_threads())
+        dt = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": len(idx) * m / dt, "unit": "solves/s",
+                                "cores": cpu_threads(), "kind": "port",
+                                "sample": f"{len(idx)} states x {m} epochs"}
+        line["parity"] = {"max_pos_err_au": float(np.abs(out[:3][:, :, idx] - ref[:3]).max())}
+    return line
+
+
+def run_fov(args):
+    import kete_amd as ka
+    from kete_amd import synthetic as syn
+    from kete_amd.fov import fov_state_check
+    n = args.n or 1_000_000
+    nf = args.fields
+    xyz, fjd, obs, pts, h = syn.fov_workload(n=n, n_fields=nf)
+    L = ka._lib.load()
+    w = math.radians(6.9)
+    nrm = np.empty((nf, 12))
+    for k in range(nf):
+        out = np.zeros(12)
+        L.kh_fov_rect_normals(ka._lib.dptr(np.ascontiguousarray(pts[k])), 0.0, w, w,
+                              ka._lib.dptr(out))
+        nrm[k] = out
+    batch = ka.StateArray(np.full(n, syn.EPOCH_JD), xyz)
+    ctx = ka.get_context()
+    fov_state_check(batch, None, fields=(fjd[:8], obs[:, :8], nrm[:8]))  # warm-up
+    t0 = time.perf_counter()
+    hits = fov_state_check(batch, None, fields=(fjd, obs, nrm))
+    dt = time.perf_counter() - t0
+    return {"metric": "fov (state x field) pairs/s", "value": n * nf / dt, "unit": "pairs/s",
+            "n_gpus": 1, "steps": 1, "ms_per_step": dt * 1000, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "fov_rect_6.9deg_1e6_states_x_3000_fields", "states": n,
+                       "fields": nf, "dt_limit": 3.0},
+            "hits": len(hits)}
+
+
+def run_copy_calibration(args):
+    """A plain device copy of known size (2^27 doubles read, 2^27 written) for
+    scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 1 << 27
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"metric": "copy calibration", "bytes_read": n * 8, "bytes_written": n * 8,
+            "ms": dt * 1000.0, "gbs": 2 * n * 8 / dt / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--fields", type=int, default=3000)
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4])
+    ap.add_argument("--mode", type=int, default=-1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate CPU time budget of the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+    if args.config == 2:
+        line = run_nbody(args)
+    elif args.config == 3:
+        line = run_nbody(args, uneven=True)
+    elif args.config == 1:
+        line = run_two_body(args)
+    elif args.config == 0:
+        line = run_copy_calibration(args)
+    else:
+        line = run_fov(args)
     print(json.dumps(line))
 
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -86,7 +87,8 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&c->d_counts, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&c->d_counts, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_range, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete c;
     return fail(KH_E_HIP, "stream/event/queue allocation failed");
   }
@@ -109,6 +111,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->dbuf) (void)hipFree(c->dbuf);
   if (c->fbuf) (void)hipFree(c->fbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
+  if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -415,6 +418,16 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.mode = mode == 0 ? 0 : 1;
   p.block = 256;
   p.grid = 0;
+  // LDS window of 16 records (62 KB for 11 segments) when all segments have
+  // 14 coefficients and the window fits next to a second block on the CU
+  p.n_seg = c->spk.n_seg;
+  const size_t win_bytes = (size_t)c->spk.n_seg * 16 * 44 * sizeof(double);
+  p.range = (c->spk.all14 && win_bytes <= 64 * 1024) ? c->d_range : nullptr;
+  p.window = p.range ? 16 : 0;
+  if (const char* w = getenv("KH_LDS_WINDOW")) {
+    p.window = atoi(w);
+    if (p.window <= 0 || (size_t)c->spk.n_seg * p.window * 44 * 8 > 64 * 1024) p.range = nullptr;
+  }
   if (p.mode == 1) {
     // persistent: enough blocks to fill every CU at the kernel's occupancy
     int per_cu = 2;

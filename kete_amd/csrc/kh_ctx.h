@@ -31,6 +31,7 @@ struct kh_ctx {
   void* fbuf = nullptr;  // FOV working memory
   size_t fbuf_size = 0;
   unsigned long long* d_queue = nullptr;
+  unsigned long long* d_range = nullptr;  // epoch range of the last N-body batch [2]
   unsigned long long* d_counts = nullptr;  // FOV list counters [4]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;

@@ -22,6 +22,7 @@ constexpr double C_AU_PER_DAY = 173.14463267424034;
 constexpr double C_AU_PER_DAY_INV = 0.005775518331436995;
 constexpr double C_AU_PER_DAY_INV_SQUARED = 3.33566119967647e-05;
 constexpr double AU_KM = 149597870.7;
+constexpr double INV_AU_KM = 1.0 / AU_KM;
 constexpr double TAU = 6.283185307179586;
 constexpr double PARABOLIC_BETA = 1e-10;  // kepler.rs:120
 
@@ -194,6 +195,13 @@ __device__ __forceinline__ int32_t record_index(const SegDev& g, double js) {
   return fi > 0.0 ? (int32_t)fmin(fi, (double)(g.n_rec - 1)) : 0;
 }
 
+// A window of W consecutive records of every segment, staged in LDS by the
+// N-body kernel (all segments must then have 14 coefficients, i.e. 44-double
+// records): kh_win_rec is [n_seg][W][44], kh_win_w0 the first staged record
+// of each segment.  Declared at file scope so that reads compile to ds_read.
+extern __shared__ double kh_win_rec[];
+__shared__ int32_t kh_win_w0[MAX_SEGS];
+
 // Chebyshev lookups read a 44-double record.  When every active lane of the
 // wavefront needs the same record -- the common case, since a wavefront's
 // objects advance through time together -- the record address is uniform and
@@ -202,32 +210,43 @@ __device__ __forceinline__ int32_t record_index(const SegDev& g, double js) {
 template <bool VEL>
 __device__ __forceinline__ int32_t seg_eval(const SpkDev* __restrict__ spk,
                                             const double* __restrict__ coef, int s, double jd,
-                                            Vec3& pos, Vec3& vel) {
+                                            Vec3& pos, Vec3& vel, int32_t win = 0) {
   const SegDev& g = spk->seg[s];
   const bool in = !(jd < g.jd_start || jd > g.jd_end);
   const double js = (jd - 2451545.0) * 86400.0;
   const int32_t ri = record_index(g, js);
-  const int32_t r0 = __builtin_amdgcn_readfirstlane(ri);
   const bool all14 = spk->all14 != 0;
   Vec3 p, d{0.0, 0.0, 0.0};
-  double radius;
-  if (__all(ri == r0) && all14) {
+  double inv_radius;
+  int32_t loc = -1;
+  if (win > 0) loc = ri - kh_win_w0[s];
+  const int32_t r0 = __builtin_amdgcn_readfirstlane(ri);
+  if (win > 0 && __all(loc >= 0 && loc < win)) {
+    // every lane's record is in the LDS window
+    const double* rec = kh_win_rec + (s * win + loc) * 44;
+    inv_radius = 1.0 / rec[1];
+    const double x = (js - rec[0]) * inv_radius;
+    if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
+    else p = cheb_pos_n<14>(rec + 2, x);
+  } else if (__all(ri == r0) && all14) {
     const double* __restrict__ rec = coef + g.off + (int64_t)r0 * g.rsize;
-    radius = rec[1];
-    const double x = (js - rec[0]) / radius;
+    inv_radius = 1.0 / rec[1];
+    const double x = (js - rec[0]) * inv_radius;
     if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
     else p = cheb_pos_n<14>(rec + 2, x);
   } else {
     const double* __restrict__ rec = coef + g.off + (int64_t)ri * g.rsize;
-    radius = rec[1];
-    const double x = (js - rec[0]) / radius;
+    inv_radius = 1.0 / rec[1];
+    const double x = (js - rec[0]) * inv_radius;
     if (VEL) cheb_both_dyn(rec + 2, g.n_coef, x, p, d);
     else p = cheb_pos_dyn(rec + 2, g.n_coef, x);
   }
   if (!in) return ST_DAF_LIMITS;
-  pos = Vec3{p.x / AU_KM, p.y / AU_KM, p.z / AU_KM};
+  // kete divides by AU_KM and by the record radius (spk_segments.rs:366-375);
+  // multiplying by reciprocals changes the result by at most an ulp
+  pos = Vec3{p.x * INV_AU_KM, p.y * INV_AU_KM, p.z * INV_AU_KM};
   if (VEL) {
-    const double sc = 86400.0 / radius / AU_KM;
+    const double sc = 86400.0 * inv_radius * INV_AU_KM;
     vel = Vec3{d.x * sc, d.y * sc, d.z * sc};
   }
   return ST_OK;
@@ -236,16 +255,16 @@ __device__ __forceinline__ int32_t seg_eval(const SpkDev* __restrict__ spk,
 // Segment position in AU (spk_segments.rs:355-376).
 __device__ __forceinline__ int32_t seg_pos(const SpkDev* __restrict__ spk,
                                            const double* __restrict__ coef, int s, double jd,
-                                           Vec3& pos) {
+                                           Vec3& pos, int32_t win = 0) {
   Vec3 unused;
-  return seg_eval<false>(spk, coef, s, jd, pos, unused);
+  return seg_eval<false>(spk, coef, s, jd, pos, unused, win);
 }
 
 // Segment position (AU) and velocity (AU/day).
 __device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk,
                                                const double* __restrict__ coef, int s, double jd,
-                                               Vec3& pos, Vec3& vel) {
-  return seg_eval<true>(spk, coef, s, jd, pos, vel);
+                                               Vec3& pos, Vec3& vel, int32_t win = 0) {
+  return seg_eval<true>(spk, coef, s, jd, pos, vel, win);
 }
 
 // Sum of a chain of segment states (the center-chaining of spk.rs:99-126).
@@ -444,6 +463,19 @@ struct NonGravDev {
 
 __device__ __forceinline__ double inv_cube(double r) { return 1.0 / (r * r * r); }
 
+// 1/sqrt(x) for x > 0: v_rsq_f64 then two Newton-Raphson steps
+// y <- y (3 - x y^2) / 2, each roughly doubling the correct bits.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-hx, y * y, 0.5);
+    y = fma(y, e, y);
+  }
+  return y;
+}
+
 // j2_correction (gravity.rs:223-235)
 __device__ __forceinline__ Vec3 j2_correction(Vec3 p, double radius, double j2, double mass) {
   double r = norm(p);
@@ -504,10 +536,10 @@ __device__ __forceinline__ void nongrav_add(const NonGravDev& ng, Vec3& a, Vec3 
 __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                              const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
-                                             int32_t& impact_id) {
+                                             int32_t& impact_id, int32_t win = 0) {
   Vec3 a{0.0, 0.0, 0.0};
   Vec3 emb;
-  int32_t st = seg_pos(spk, coef, spk->emb_seg, t, emb);
+  int32_t st = seg_pos(spk, coef, spk->emb_seg, t, emb, win);
   if (st) return st;
 #pragma unroll 1
   for (int b = 0; b < N_PLANETS; ++b) {
@@ -515,20 +547,23 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     const double mass = bd.mass;
     Vec3 bp, bv{0.0, 0.0, 0.0};
     if (bd.kind == 1) {
-      st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv);
+      st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv, win);
     } else {
-      st = seg_pos(spk, coef, bd.seg, t, bp);
+      st = seg_pos(spk, coef, bd.seg, t, bp, win);
       if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
     }
     if (st) return st;
     Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
-    const double r = norm(rp);
+    // 1/r from the hardware reciprocal square root refined by two Newton
+    // steps (full FP64 accuracy), r = r^2/r, 1/r^3 = (1/r)^3; kete computes
+    // sqrt and r.powi(-3) -- the results differ by an ulp or two
+    const double r2 = dot(rp, rp);
+    const double inv_r = rsqrt_nr(r2);
+    const double r = r2 * inv_r;
     if (exact && r <= bd.radius) {
       impact_id = bd.id;
       return ST_IMPACT;
     }
-    // one division per body: 1/r, then 1/r^3 = (1/r)^3 (kete: r.powi(-3))
-    const double inv_r = 1.0 / r;
     const double inv_r3 = inv_r * inv_r * inv_r;
     const double k = mass * inv_r3;
     a.x -= rp.x * k;

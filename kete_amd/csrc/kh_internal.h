@@ -44,6 +44,9 @@ struct NBodyLaunch {
   int32_t mode;   // 0 plain, 1 persistent
   int32_t block;  // threads per block (0: default)
   int32_t grid;   // persistent grid size in blocks (0: default)
+  unsigned long long* range;  // [2] epoch-range scratch, or null: no LDS window
+  int32_t window;             // records per segment staged in LDS
+  int32_t n_seg;
 };
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket

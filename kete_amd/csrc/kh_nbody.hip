@@ -68,6 +68,8 @@ struct NBodyArgs {
   int32_t* counters;  // [n][4] or null
   unsigned long long* queue;
   int64_t first_dynamic;  // objects below this are assigned by lane id
+  unsigned long long* range;  // [2]: bits of min and max epoch over the batch
+  int32_t window;             // records per segment staged in LDS (0: none)
 };
 
 enum : int32_t { PH_INIT = 0, PH_SUB = 1, PH_EXACT = 2 };
@@ -142,15 +144,39 @@ __device__ __forceinline__ void store_result(const NBodyArgs& a, int64_t i, int3
 // output arrays, where the integrator picks it up.
 __global__ void nbody_prepare_kernel(const SpkDev* __restrict__ spk,
                                      const double* __restrict__ coef, NBodyArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i0 < a.n;
+  const int64_t i = valid ? i0 : a.n - 1;
   const double t = a.jd0[i];
   Vec3 x{a.xyz[0][i], a.xyz[1][i], a.xyz[2][i]};
   Vec3 v{a.xyz[3][i], a.xyz[4][i], a.xyz[5][i]};
   const bool finite = isfinite(t) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) &&
                       isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
-  const int32_t st = finite ? to_ssb_equatorial(spk, coef, a, t, x, v) : ST_VALUE_ERROR;
-  store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+  int32_t st = finite ? to_ssb_equatorial(spk, coef, a, t, x, v) : ST_VALUE_ERROR;
+  if (valid) store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+  if (!valid) st = ST_VALUE_ERROR;
+  if (a.range != nullptr) {
+    // epoch range of the batch (all epochs are positive JDs, so the IEEE bit
+    // patterns order like the values); reduced over the wavefront first
+    const double tf = a.jd_final[a.jd_final_stride ? i : 0];
+    double lo = st == ST_OK ? fmin(t, tf) : __builtin_inf();
+    double hi = st == ST_OK ? fmax(t, tf) : 0.0;
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, off));
+      hi = fmax(hi, __shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0 && lo <= hi && lo > 0.0) {
+      atomicMin(&a.range[0], (unsigned long long)__double_as_longlong(lo));
+      atomicMax(&a.range[1], (unsigned long long)__double_as_longlong(hi));
+    }
+  }
+}
+
+__global__ void init_range(unsigned long long* r) {
+  if (threadIdx.x == 0) {
+    r[0] = (unsigned long long)__double_as_longlong(__builtin_inf());
+    r[1] = 0ull;
+  }
 }
 
 // Post-pass: back to the input center and frame at jd_final (mod.rs:115-116).
@@ -230,7 +256,24 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk,
       lds.inv_h[c] = RT.inv_h[c];
     }
   }
+  // Stage the records covering the batch's earliest epoch onwards: a window
+  // of a.window records per segment (a one-year arc needs 13 of 32 days).
+  if (a.window > 0) {
+    const double tlo = __longlong_as_double((long long)a.range[0]);
+    const double jlo = (tlo - 2451545.0) * 86400.0;
+    if (threadIdx.x < spk->n_seg)
+      kh_win_w0[threadIdx.x] = record_index(spk->seg[threadIdx.x], jlo);
+    __syncthreads();
+    const int per_seg = a.window * 44;
+    for (int k = threadIdx.x; k < spk->n_seg * per_seg; k += blockDim.x) {
+      const int sg = k / per_seg, rem = k - sg * per_seg;
+      const SegDev& g = spk->seg[sg];
+      const int r = kh_win_w0[sg] + rem / 44;
+      kh_win_rec[k] = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 0.0;
+    }
+  }
   __syncthreads();
+  const int32_t winw = a.window;
 
   const double eps = a.epsilon;
   int64_t obj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,7 +323,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk,
     n_force++;
     const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
     int32_t st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]},
-                           !sub, ngp, acc, impact);
+                           !sub, ngp, acc, impact, winw);
     const double ev[3] = {acc.x, acc.y, acc.z};
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
@@ -457,8 +500,12 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.impact_jd = p.impact_jd;
   a.counters = p.counters;
   a.queue = p.queue;
+  a.range = p.range;
+  a.window = p.range != nullptr ? p.window : 0;
+  const size_t lds_bytes = a.window > 0 ? (size_t)p.n_seg * a.window * 44 * sizeof(double) : 0;
   const int block = p.block > 0 ? p.block : 256;
   const unsigned g1 = (unsigned)((p.n + 255) / 256);
+  if (a.range) hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range);
   hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   int64_t grid;
   if (p.mode == 1) {
@@ -467,15 +514,15 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     a.first_dynamic = grid * block;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
     if (ev0) (void)hipEventRecord(ev0, stream);
-    hipLaunchKernelGGL(radau_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), 0, stream,
-                       spk, coef, a);
+    hipLaunchKernelGGL(radau_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), lds_bytes,
+                       stream, spk, coef, a);
     if (ev1) (void)hipEventRecord(ev1, stream);
   } else {
     grid = (p.n + block - 1) / block;
     a.first_dynamic = p.n;
     if (ev0) (void)hipEventRecord(ev0, stream);
-    hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), 0, stream,
-                       spk, coef, a);
+    hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), lds_bytes,
+                       stream, spk, coef, a);
     if (ev1) (void)hipEventRecord(ev1, stream);
   }
   hipLaunchKernelGGL(nbody_finish_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
