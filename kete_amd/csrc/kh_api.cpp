@@ -379,7 +379,7 @@ static int32_t check_nbody_args(kh_ctx* c, int64_t n, const double* jd0,
     if (!xyz[k] || !out[k]) return fail(KH_E_ARG, "null state column");
   if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
   if (!(epsilon > 0.0)) return fail(KH_E_ARG, "epsilon must be positive");
-  if (mode < -1 || mode > 1) return fail(KH_E_ARG, "mode must be -1, 0 or 1");
+  if (mode < -1 || mode > 3) return fail(KH_E_ARG, "mode must be -1 .. 3");
   if (!kh_build_chain(c, center, ch))
     return fail(KH_E_NO_SPK, "center " + std::to_string(center) + " is not in the ephemeris");
   return KH_OK;
@@ -415,7 +415,7 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.impact_jd = impact_jd;
   p.counters = reinterpret_cast<int32_t*>(counters);
   p.queue = c->d_queue;
-  p.mode = mode == 0 ? 0 : 1;
+  p.mode = mode < 0 ? 1 : mode;
   p.block = 256;
   p.grid = 0;
   // LDS window of 16 records (62 KB for 11 segments) when all segments have
@@ -428,7 +428,7 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
     p.window = atoi(w);
     if (p.window <= 0 || (size_t)c->spk.n_seg * p.window * 44 * 8 > 64 * 1024) p.range = nullptr;
   }
-  if (p.mode == 1) {
+  if (p.mode == 1 || p.mode == 2) {
     // persistent: enough blocks to fill every CU at the kernel's occupancy
     int per_cu = 2;
     p.grid = c->n_cu * per_cu * 4;

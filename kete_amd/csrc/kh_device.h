@@ -529,10 +529,80 @@ __device__ __forceinline__ void nongrav_add(const NonGravDev& ng, Vec3& a, Vec3 
   }
 }
 
+// One body's contribution (GravParams::add_acceleration, gravity.rs:171-216,
+// plus the non-grav term attached to the Sun, acceleration.rs:161-165) to the
+// acceleration of a massless object at barycentric equatorial (x, v), time t.
+// emb is the Earth-Moon barycentre position when the body is given relative
+// to it.  On exact evaluations, r <= radius is an Impact.
+__device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
+                                            const double* __restrict__ coef, const BodyDev& bd,
+                                            double t, Vec3 x, Vec3 v, bool exact,
+                                            const NonGravDev* ng, Vec3 emb, Vec3& a,
+                                            int32_t win) {
+  const double mass = bd.mass;
+  Vec3 bp, bv{0.0, 0.0, 0.0};
+  int32_t st;
+  if (bd.kind == 1) {
+    st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv, win);
+  } else {
+    st = seg_pos(spk, coef, bd.seg, t, bp, win);
+    if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+  }
+  if (st) return st;
+  Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
+  // 1/r from the hardware reciprocal square root refined by two Newton
+  // steps (full FP64 accuracy), r = r^2/r, 1/r^3 = (1/r)^3; kete computes
+  // sqrt and r.powi(-3) -- the results differ by an ulp or two
+  const double r2 = dot(rp, rp);
+  const double inv_r = rsqrt_nr(r2);
+  const double r = r2 * inv_r;
+  if (exact && r <= bd.radius) return ST_IMPACT;
+  const double inv_r3 = inv_r * inv_r * inv_r;
+  const double k = mass * inv_r3;
+  a.x -= rp.x * k;
+  a.y -= rp.y * k;
+  a.z -= rp.z * k;
+  if (bd.kind == 1) {
+    // apply_gr_correction (gravity.rs:239-253)
+    const Vec3 rv{v.x - bv.x, v.y - bv.y, v.z - bv.z};
+    const double r_v = 4.0 * dot(rp, rv);
+    const double v2 = dot(rv, rv);
+    const double gc = mass * C_AU_PER_DAY_INV_SQUARED * inv_r3;
+    const double c = 4. * mass * inv_r - v2;
+    a.x += gc * (c * rp.x + r_v * rv.x);
+    a.y += gc * (c * rp.y + r_v * rv.y);
+    a.z += gc * (c * rp.z + r_v * rv.z);
+    // j2_correction in the ecliptic frame and back (gravity.rs:184-201,223-235);
+    // |rp| is the same in both frames
+    const Vec3 pe = equ_to_ecl(rp);
+    const double zr = pe.z * inv_r;
+    const double z_squared = 5.0 * (zr * zr);
+    const double rr = bd.radius * inv_r;
+    const double coef2 = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
+    const Vec3 j = ecl_to_equ(Vec3{pe.x * coef2 * (z_squared - 1.0),
+                                   pe.y * coef2 * (z_squared - 1.0),
+                                   pe.z * coef2 * (z_squared - 3.0)});
+    a.x += j.x;
+    a.y += j.y;
+    a.z += j.z;
+    if (bd.id == 10 && ng != nullptr) nongrav_add(*ng, a, rp, rv);
+  } else if (bd.kind == 2) {
+    // Earth J2 in the equatorial frame (gravity.rs:203)
+    const double zr = rp.z * inv_r;
+    const double z_squared = 5.0 * (zr * zr);
+    const double rr = bd.radius * inv_r;
+    const double coef2 = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
+    a.x += rp.x * coef2 * (z_squared - 1.0);
+    a.y += rp.y * coef2 * (z_squared - 1.0);
+    a.z += rp.z * coef2 * (z_squared - 3.0);
+  }
+  return ST_OK;
+}
+
 // spk_accel: acceleration of a massless body at barycentric equatorial
-// (x, v) and TDB JD t.  On exact evaluations, r <= radius is an Impact.
-// The Earth-Moon barycentre is evaluated once and shared by 399 and 301
-// (kete evaluates it for each, with identical results).
+// (x, v) and TDB JD t, summed over kete's PLANETS in its order.  The
+// Earth-Moon barycentre is evaluated once and shared by 399 and 301 (kete
+// evaluates it for each, with identical results).
 __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                              const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
@@ -544,64 +614,10 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
 #pragma unroll 1
   for (int b = 0; b < N_PLANETS; ++b) {
     const BodyDev& bd = spk->body[b];
-    const double mass = bd.mass;
-    Vec3 bp, bv{0.0, 0.0, 0.0};
-    if (bd.kind == 1) {
-      st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv, win);
-    } else {
-      st = seg_pos(spk, coef, bd.seg, t, bp, win);
-      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
-    }
-    if (st) return st;
-    Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
-    // 1/r from the hardware reciprocal square root refined by two Newton
-    // steps (full FP64 accuracy), r = r^2/r, 1/r^3 = (1/r)^3; kete computes
-    // sqrt and r.powi(-3) -- the results differ by an ulp or two
-    const double r2 = dot(rp, rp);
-    const double inv_r = rsqrt_nr(r2);
-    const double r = r2 * inv_r;
-    if (exact && r <= bd.radius) {
-      impact_id = bd.id;
-      return ST_IMPACT;
-    }
-    const double inv_r3 = inv_r * inv_r * inv_r;
-    const double k = mass * inv_r3;
-    a.x -= rp.x * k;
-    a.y -= rp.y * k;
-    a.z -= rp.z * k;
-    if (bd.kind == 1) {
-      // apply_gr_correction (gravity.rs:239-253)
-      const Vec3 rv{v.x - bv.x, v.y - bv.y, v.z - bv.z};
-      const double r_v = 4.0 * dot(rp, rv);
-      const double v2 = dot(rv, rv);
-      const double gc = mass * C_AU_PER_DAY_INV_SQUARED * inv_r3;
-      const double c = 4. * mass * inv_r - v2;
-      a.x += gc * (c * rp.x + r_v * rv.x);
-      a.y += gc * (c * rp.y + r_v * rv.y);
-      a.z += gc * (c * rp.z + r_v * rv.z);
-      // j2_correction in the ecliptic frame and back (gravity.rs:184-201,223-235);
-      // |rp| is the same in both frames
-      const Vec3 pe = equ_to_ecl(rp);
-      const double zr = pe.z * inv_r;
-      const double z_squared = 5.0 * (zr * zr);
-      const double rr = bd.radius * inv_r;
-      const double coef = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
-      const Vec3 j = ecl_to_equ(Vec3{pe.x * coef * (z_squared - 1.0),
-                                     pe.y * coef * (z_squared - 1.0),
-                                     pe.z * coef * (z_squared - 3.0)});
-      a.x += j.x;
-      a.y += j.y;
-      a.z += j.z;
-      if (bd.id == 10 && ng != nullptr) nongrav_add(*ng, a, rp, rv);
-    } else if (bd.kind == 2) {
-      // Earth J2 in the equatorial frame (gravity.rs:203)
-      const double zr = rp.z * inv_r;
-      const double z_squared = 5.0 * (zr * zr);
-      const double rr = bd.radius * inv_r;
-      const double coef = 1.5 * bd.j2 * mass * (rr * rr) * inv_r3;
-      a.x += rp.x * coef * (z_squared - 1.0);
-      a.y += rp.y * coef * (z_squared - 1.0);
-      a.z += rp.z * coef * (z_squared - 3.0);
+    st = add_body(spk, coef, bd, t, x, v, exact, ng, emb, a, win);
+    if (st) {
+      if (st == ST_IMPACT) impact_id = bd.id;
+      return st;
     }
   }
   acc = a;

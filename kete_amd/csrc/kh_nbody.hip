@@ -477,6 +477,284 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Three lanes per object.  Lane c of a group owns coordinate c of the
+// integrator (x_c, v_c, a_c, B[7]_c, G[7]_c), so its state is a third of the
+// one-lane kernel's and more wavefronts fit on a SIMD.  The force evaluation
+// is split across the group by body -- lane 0: Sun, Mercury, Venus, Mars;
+// lane 1: Jupiter, Saturn, Uranus, Neptune; lane 2: the Earth-Moon
+// barycentre, Earth, Moon -- and the three partial accelerations are summed
+// with wavefront shuffles.  Convergence and step-size maxima are reduced over
+// the group, so the three lanes always take the same control path.  A
+// wavefront holds 21 groups; its last lane idles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double gshfl(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int gshfl(int v, int src) { return __shfl(v, src, 64); }
+
+__device__ __forceinline__ double group_max(double v, int base) {
+  return fmax(fmax(gshfl(v, base), gshfl(v, base + 1)), gshfl(v, base + 2));
+}
+
+// kete index of the body in slot k of lane c (-1: none); PLANETS order is
+// 10, 1, 2, 399, 301, 4, 5, 6, 7, 8 (gravity.rs:145-148)
+__device__ __forceinline__ int slot_body(int c, int k) {
+  const int t0[4] = {0, 1, 2, 5};
+  const int t1[4] = {6, 7, 8, 9};
+  const int t2[4] = {3, 4, -1, -1};
+  return c == 0 ? t0[k] : (c == 1 ? t1[k] : t2[k]);
+}
+
+template <bool PERSISTENT>
+__device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t i, int c, int base,
+                                          double& t, double& tf, double& x, double& v) {
+  while (i < a.n) {
+    if (a.status[i] == ST_OK) {
+      t = a.jd0[i];
+      tf = a.jd_final[a.jd_final_stride ? i : 0];
+      x = a.out[c][i];
+      v = a.out[3 + c][i];
+      return i;
+    }
+    if (!PERSISTENT) return a.n;
+    unsigned long long k = 0;
+    if (c == 0) k = atomicAdd(a.queue, 1ull);
+    const int lo = gshfl((int)(k & 0xffffffffull), base);
+    const int hi = gshfl((int)(k >> 32), base);
+    i = a.first_dynamic + (int64_t)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
+  return i;
+}
+
+template <bool PERSISTENT>
+__global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restrict__ spk,
+                                                           const double* __restrict__ coef,
+                                                           NBodyArgs a) {
+  __shared__ LdsTables lds;
+  for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
+    const int r = k / 8, cc = k % 8;
+    lds.rdiff[r][cc] = RT.rdiff[r][cc];
+    if (r == 0) {
+      lds.h[cc] = RT.h[cc];
+      lds.inv_h[cc] = RT.inv_h[cc];
+    }
+  }
+  if (a.window > 0) {
+    const double tlo = __longlong_as_double((long long)a.range[0]);
+    const double jlo = (tlo - 2451545.0) * 86400.0;
+    if (threadIdx.x < spk->n_seg)
+      kh_win_w0[threadIdx.x] = record_index(spk->seg[threadIdx.x], jlo);
+    __syncthreads();
+    const int per_seg = a.window * 44;
+    for (int k = threadIdx.x; k < spk->n_seg * per_seg; k += blockDim.x) {
+      const int sg = k / per_seg, rem = k - sg * per_seg;
+      const SegDev& g = spk->seg[sg];
+      const int r = kh_win_w0[sg] + rem / 44;
+      kh_win_rec[k] = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int32_t winw = a.window;
+
+  const int lane = threadIdx.x & 63;
+  const int gw = lane / 3;
+  const int c = lane - 3 * gw;
+  const int base = 3 * gw;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const double eps = a.epsilon;
+  int64_t obj = gw < 21 ? wave * 21 + gw : a.n;
+  double t = 0.0, tf = 0.0, h = 0.0, h_next = 0.0, x = 0.0, v = 0.0, a0 = 0.0;
+  double B[7], G[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) B[k] = G[k] = 0.0;
+  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
+  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
+  if (gw < 21) obj = fetch3<PERSISTENT>(a, obj, c, base, t, tf, x, v);
+
+  while (obj < a.n) {
+    const bool sub = phase == PH_SUB;
+    // ---- predictor for this lane's coordinate (radau.rs:252-278) ----
+    const double s = lds.h[j];
+    double xe, ve;
+    {
+      double bw = 0.0, bu = 0.0, sp = 1.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        sp *= s;
+        bw += B[k] * (sp * RTC.w[k]);
+        bu += B[k] * (sp * RTC.u[k]);
+      }
+      const double px = x + s * v * h + (s * s) * (h * h) * (a0 / 2.0 + bw);
+      const double pv = v + s * h * (a0 + bu);
+      xe = sub ? px : x;
+      ve = sub ? pv : v;
+    }
+    const double te = sub ? t + s * h : t;
+    const Vec3 X{gshfl(xe, base), gshfl(xe, base + 1), gshfl(xe, base + 2)};
+    const Vec3 V{gshfl(ve, base), gshfl(ve, base + 1), gshfl(ve, base + 2)};
+
+    // ---- this lane's share of the force evaluation ----
+    n_force++;
+    const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
+    Vec3 part{0.0, 0.0, 0.0};
+    int key = 0x7fffffff;  // (kete index << 4) | status of this lane's first failure
+    Vec3 emb{0.0, 0.0, 0.0};
+    if (c == 2) {
+      const int32_t se = seg_pos(spk, coef, spk->emb_seg, te, emb, winw);
+      if (se) key = (3 << 4) | se;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int b = slot_body(c, k);
+      if (b >= 0 && key == 0x7fffffff) {
+        const int32_t sb = add_body(spk, coef, spk->body[b], te, X, V, !sub, ngp, emb, part, winw);
+        if (sb) key = (b << 4) | sb;
+      }
+    }
+    const int kmin = min(min(gshfl(key, base), gshfl(key, base + 1)), gshfl(key, base + 2));
+    const int32_t st = kmin == 0x7fffffff ? ST_OK : (kmin & 15);
+    // a_c = sum over the group of the partials' component c
+    const double p0 = c == 0 ? part.x : (c == 1 ? part.y : part.z);
+    const double q1 = c == 0 ? part.z : (c == 1 ? part.x : part.y);  // = part[(c+2)%3]
+    const double q2 = c == 0 ? part.y : (c == 1 ? part.z : part.x);  // = part[(c+1)%3]
+    const int l1 = base + (c == 2 ? 0 : c + 1);
+    const int l2 = base + (c == 0 ? 2 : c - 1);
+    const double ev = p0 + gshfl(q1, l1) + gshfl(q2, l2);
+
+    // ---- G column j (radau.rs:292-308) ----
+    {
+      double gk = (ev - a0) * lds.inv_h[j];
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = lds.rdiff[j][z];
+        gk = z < j ? (gk - G[z - 1]) * rd : gk;
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) G[q] = (sub && q == j - 1) ? gk : G[q];
+    }
+    // ---- end of sweep: B = G*C, convergence (radau.rs:313-319) ----
+    const bool es = sub && j == 7 && st == ST_OK;
+    const double bp6 = B[6];
+#pragma unroll
+    for (int cc = 0; cc < 7; ++cc) {
+      double acc2 = G[cc];
+#pragma unroll
+      for (int k = cc + 1; k < 7; ++k) acc2 += G[k] * RTC.c[k][cc];
+      B[cc] = es ? acc2 : B[cc];
+    }
+    const double fmx = fabs(ev) + 1e-6;
+    const double rmax = group_max(fabs(B[6] - bp6) / fmx, base);
+    const bool conv = es && rmax < 1e-14;
+    // ---- accepted step (radau.rs:320-338) ----
+    {
+      double bw = 0.0, bu = 0.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        bw += B[k] * RTC.w[k];
+        bu += B[k] * RTC.u[k];
+      }
+      const double nx = x + (v * h + (h * h) * (a0 * 0.5 + bw));
+      const double nv = v + h * (a0 + bu);
+      x = conv ? nx : x;
+      v = conv ? nv : v;
+    }
+    if (conv) {
+      const double fm = group_max(fmx, base);
+      const double bm = group_max(fabs(B[6]), base);
+      double ratio = root7(eps * fm / bm);
+      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
+      h_next = h * ratio;
+    }
+    t = conv ? t + h : t;
+    a0 = (st == ST_OK && !sub) ? ev : a0;
+
+    // ---- scalar control (radau.rs:160-218), identical in the group ----
+    int32_t done = -1;
+    bool begin = false;
+    if (st != ST_OK) {
+      done = st;
+    } else if (phase == PH_INIT) {
+      if (fabs(tf - t) < 1e-10) {
+        done = ST_OK;
+      } else {
+        h = copysign(0.1, tf - t);
+        begin = true;
+      }
+    } else if (phase == PH_EXACT) {
+      h = h_next;
+      if (fabs(t - tf) < 1e-12) {
+        done = ST_OK;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    } else if (j < 7) {
+      j = j + 1;
+    } else if (conv) {
+      n_acc++;
+      phase = PH_EXACT;
+    } else if (sweep + 1 < 10) {
+      sweep++;
+      j = 1;
+      n_sweep++;
+    } else {
+      n_fail++;
+      fails++;
+      h *= 0.7;
+      if (fails > 10) {
+        done = ST_CONVERGENCE;
+      } else {
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    }
+    if (begin) {
+      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      sweep = 0;
+      j = 1;
+      n_sweep++;
+      phase = PH_SUB;
+    }
+
+    if (done >= 0) {
+      const double nan = __builtin_nan("");
+      const bool ok = done == ST_OK;
+      a.out[c][obj] = ok ? x : nan;
+      a.out[3 + c][obj] = ok ? v : nan;
+      if (c == 0) {
+        a.status[obj] = done;
+        if (a.impact_id)
+          a.impact_id[obj] = done == ST_IMPACT ? spk->body[kmin >> 4].id : 0;
+        if (a.impact_jd) a.impact_jd[obj] = done == ST_IMPACT ? te : nan;
+        if (a.counters)
+          reinterpret_cast<int4*>(a.counters)[obj] = int4{n_force, n_sweep, n_acc, n_fail};
+      }
+      int64_t nxt = a.n;
+      if (PERSISTENT) {
+        unsigned long long k = 0;
+        if (c == 0) k = atomicAdd(a.queue, 1ull);
+        const int lo = gshfl((int)(k & 0xffffffffull), base);
+        const int hi = gshfl((int)(k >> 32), base);
+        nxt = a.first_dynamic + (int64_t)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+      }
+      double nt, ntf, nx2, nv2;
+      obj = fetch3<PERSISTENT>(a, nxt, c, base, nt, ntf, nx2, nv2);
+      t = nt;
+      tf = ntf;
+      x = nx2;
+      v = nv2;
+      phase = PH_INIT;
+      j = 1;
+      sweep = 0;
+      fails = 0;
+      n_force = n_sweep = n_acc = n_fail = 0;
+    }
+    const bool fresh = done >= 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) B[k] = fresh ? 0.0 : B[k];
+  }
+}
+
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 
 int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
@@ -508,7 +786,24 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   if (a.range) hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range);
   hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   int64_t grid;
-  if (p.mode == 1) {
+  if (p.mode == 2 || p.mode == 3) {
+    // three lanes per object, 21 objects per wavefront
+    const int64_t waves_needed = (p.n + 20) / 21;
+    const int64_t blocks_needed = (waves_needed * 64 + block - 1) / block;
+    const bool persistent = p.mode == 2;
+    grid = persistent ? (p.grid > 0 ? p.grid : 1024) : blocks_needed;
+    if (grid > blocks_needed) grid = blocks_needed;
+    a.first_dynamic = grid * (block / 64) * 21;
+    if (persistent) hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
+    if (ev0) (void)hipEventRecord(ev0, stream);
+    if (persistent)
+      hipLaunchKernelGGL(radau3_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), lds_bytes,
+                         stream, spk, coef, a);
+    else
+      hipLaunchKernelGGL(radau3_nbody_kernel<false>, dim3((unsigned)grid), dim3(block),
+                         lds_bytes, stream, spk, coef, a);
+    if (ev1) (void)hipEventRecord(ev1, stream);
+  } else if (p.mode == 1) {
     grid = p.grid > 0 ? p.grid : 1024;
     if (grid * block > p.n) grid = (p.n + block - 1) / block;
     a.first_dynamic = grid * block;
@@ -528,7 +823,7 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   hipLaunchKernelGGL(nbody_finish_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   if (used_block) *used_block = block;
   if (used_grid) *used_grid = (int32_t)grid;
-  if (used_mode) *used_mode = p.mode == 1 ? 1 : 0;
+  if (used_mode) *used_mode = p.mode;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

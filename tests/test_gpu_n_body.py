@@ -35,8 +35,10 @@ def run_both(ctx, oracle_spk, xyz, jd_final, mode=-1, frame=ka.Frames.Ecliptic, 
     return g, r
 
 
-def test_parity_one_year(ctx, oracle_spk, mix):
-    g, r = run_both(ctx, oracle_spk, mix, JD0 + 365.25)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_parity_one_year(ctx, oracle_spk, mix, mode):
+    """mode 1: one lane per object; mode 2: three lanes per object."""
+    g, r = run_both(ctx, oracle_spk, mix, JD0 + 365.25, mode=mode)
     assert np.array_equal(g["status"], r["status"])
     assert np.all(g["status"] == 0)
     dp = np.abs(g["out"][:3] - r["out"][:3]).max(axis=0)
@@ -54,13 +56,28 @@ def test_parity_one_year(ctx, oracle_spk, mix):
     assert np.mean(np.abs(gc[:, 2] - rc[:, 2]) <= 1) > 0.95
 
 
-def test_plain_and_persistent_identical(ctx, mix):
+@pytest.mark.parametrize("modes", [(0, 1), (3, 2)])
+def test_plain_and_persistent_identical(ctx, mix, modes):
     n = mix.shape[1]
     batch = ka.StateArray(np.full(n, JD0), mix)
-    a = ctx.n_body(batch, JD0 - 200.0, mode=0)
-    b = ctx.n_body(batch, JD0 - 200.0, mode=1)
+    a = ctx.n_body(batch, JD0 - 200.0, mode=modes[0])
+    b = ctx.n_body(batch, JD0 - 200.0, mode=modes[1])
     assert np.array_equal(a["status"], b["status"])
     assert np.array_equal(a["out"], b["out"])
+
+
+def test_three_lane_error_semantics(ctx, oracle_spk, mix):
+    xyz = mix[:, :6].copy()
+    xyz[0, 1] = np.nan
+    xyz[:3, 2] = [1e-3, 0.0, 0.0]
+    xyz[3:, 2] = [0.0, 0.0, 0.0]
+    g, r = run_both(ctx, oracle_spk, xyz, JD0 + 30.0, mode=2)
+    assert np.array_equal(g["status"], r["status"])
+    assert g["status"][2] == 6 and g["impact_id"][2] == 10
+    good = g["status"] == 0
+    assert np.abs(g["out"][:3, good] - r["out"][:3, good]).max() < POS_TOL
+    g, r = run_both(ctx, oracle_spk, mix[:, :4].copy(), JD0 + 5000.0, mode=2)
+    assert np.all(g["status"] == 3)
 
 
 def test_uneven_spans_parity(ctx, oracle_spk):
