@@ -98,12 +98,27 @@ def time_steps(ctx, step, steps, warmup):
     return (t1 - t0) * 1000.0 / steps, float(np.mean(kernel_ms))
 
 
-def radau_roofline(cnt, k_ms):
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    FETCH_SIZE / WRITE_SIZE measurement (profiles/hbm_traffic.json, scaled by
+    the copy-kernel calibration), or None when that workload was not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload") != workload:
+        return None
+    return float(rec["bytes_per_launch"])
+
+
+def radau_roofline(cnt, k_ms, workload):
     n_force, n_sweep, n_acc = (int(cnt[:, k].sum()) for k in range(3))
     flop = F_FORCE * n_force + F_SWEEP * n_sweep + F_ACCEPT * n_acc
     achieved = flop / (k_ms / 1000.0) / 1e12
     return {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": measured_traffic("radau_nbody_kernel", workload),
             "kernel": "radau_nbody_kernel", "kernel_ms": k_ms, "flop_per_launch": flop,
             "work": {"force_evals": n_force, "sweeps": n_sweep, "accepted_steps": n_acc}}
 
@@ -140,7 +155,10 @@ def run_nbody(args, uneven=False):
     spans = np.abs(np.broadcast_to(jdf, (n,)) - jd0)
     obj_years = float(spans.sum() / 365.25)
     value = obj_years / (ms_per_step / 1000.0)
-    roof = radau_roofline(cnt, k_ms)
+    workload = "radau15_nbody_uneven_spans_2e6" if uneven else "radau15_nbody_1e6_objects_1yr"
+    if args.n:
+        workload += f"_n{n}"
+    roof = radau_roofline(cnt, k_ms, workload)
     log(f"[bench] {ms_per_step:.1f} ms/step, kernel {k_ms:.1f} ms, {value:.3e} obj-yr/s, "
         f"{roof['achieved']:.2f} TFLOP/s FP64 ({100 * roof['frac']:.1f}% of peak)")
     extra = {}
@@ -179,7 +197,6 @@ def run_nbody(args, uneven=False):
                          f"{dt:.1f} s wall on {threads} threads (C restatement, -O3)"}
         log(f"[bench] cpu {cpu_val:.1f} obj-yr/s on {threads} threads; parity {parity}")
 
-    workload = "radau15_nbody_uneven_spans_2e6" if uneven else "radau15_nbody_1e6_objects_1yr"
     line = {
         "metric": "object-years/s, FP64 Radau (<=1e-9 AU vs ref)",
         "value": value,

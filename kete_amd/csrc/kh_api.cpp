@@ -176,6 +176,27 @@ int32_t kh_spk_upload_type2(kh_ctx* c, const kh_spk2_seg* segs, int64_t nseg) {
   }
   spk.n_seg = (int32_t)nseg;
   spk.all14 = all14 ? 1 : 0;
+  // one record grid for all segments: same start, length, count, and
+  // bitwise-identical record mid-points and radii
+  bool shared = all14;
+  for (int64_t k = 1; shared && k < nseg; ++k) {
+    const SegDev &a0 = spk.seg[0], &ak = spk.seg[k];
+    if (ak.jd_start != a0.jd_start || ak.jd_end != a0.jd_end || ak.j0s != a0.j0s ||
+        ak.intlen != a0.intlen || ak.n_rec != a0.n_rec) {
+      shared = false;
+      break;
+    }
+    for (int64_t r = 0; r < segs[k].n_rec; ++r) {
+      const double* p0 = segs[0].records + r * segs[0].rsize;
+      const double* pk = segs[k].records + r * segs[k].rsize;
+      if (std::memcmp(p0, pk, 2 * sizeof(double)) != 0) {
+        shared = false;
+        break;
+      }
+    }
+  }
+  spk.shared_grid = shared ? 1 : 0;
+  if (const char* sg = getenv("KH_SHARED_GRID")) spk.shared_grid = shared && atoi(sg) != 0;
   // PLANETS (gravity.rs:43-101,145-148): id, mass, radius, J2 and the
   // special terms of GravParams::add_acceleration (gravity.rs:171-216).
   struct PB { int64_t id; double mass, radius, j2; int kind; };

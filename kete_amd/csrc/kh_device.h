@@ -75,6 +75,7 @@ struct SpkDev {
   const double* __restrict__ coef;
   int32_t n_seg;
   int32_t all14;  // every segment has 14 coefficients
+  int32_t shared_grid;  // all segments on one record grid (see accel_shared_grid)
   int32_t emb_seg;
   int32_t sun_seg;
   SegDev seg[MAX_SEGS];
@@ -534,21 +535,12 @@ __device__ __forceinline__ void nongrav_add(const NonGravDev& ng, Vec3& a, Vec3 
 // acceleration of a massless object at barycentric equatorial (x, v), time t.
 // emb is the Earth-Moon barycentre position when the body is given relative
 // to it.  On exact evaluations, r <= radius is an Impact.
-__device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
-                                            const double* __restrict__ coef, const BodyDev& bd,
-                                            double t, Vec3 x, Vec3 v, bool exact,
-                                            const NonGravDev* ng, Vec3 emb, Vec3& a,
-                                            int32_t win) {
+// The body's barycentric equatorial position bp (and velocity bv for kind
+// 1) are given.
+__device__ __forceinline__ int32_t add_body_at(const BodyDev& bd, Vec3 bp, Vec3 bv, Vec3 x,
+                                               Vec3 v, bool exact, const NonGravDev* ng,
+                                               Vec3& a) {
   const double mass = bd.mass;
-  Vec3 bp, bv{0.0, 0.0, 0.0};
-  int32_t st;
-  if (bd.kind == 1) {
-    st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv, win);
-  } else {
-    st = seg_pos(spk, coef, bd.seg, t, bp, win);
-    if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
-  }
-  if (st) return st;
   Vec3 rp{x.x - bp.x, x.y - bp.y, x.z - bp.z};
   // 1/r from the hardware reciprocal square root refined by two Newton
   // steps (full FP64 accuracy), r = r^2/r, 1/r^3 = (1/r)^3; kete computes
@@ -599,6 +591,68 @@ __device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
   return ST_OK;
 }
 
+__device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
+                                            const double* __restrict__ coef, const BodyDev& bd,
+                                            double t, Vec3 x, Vec3 v, bool exact,
+                                            const NonGravDev* ng, Vec3 emb, Vec3& a,
+                                            int32_t win) {
+  Vec3 bp, bv{0.0, 0.0, 0.0};
+  int32_t st;
+  if (bd.kind == 1) {
+    st = seg_pos_vel(spk, coef, bd.seg, t, bp, bv, win);
+  } else {
+    st = seg_pos(spk, coef, bd.seg, t, bp, win);
+    if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+  }
+  if (st) return st;
+  return add_body_at(bd, bp, bv, x, v, exact, ng, a);
+}
+
+// Body states when every segment shares one record grid (same start, record
+// length, count, and bitwise-identical record mid-points and radii -- DE-style
+// files built on one grid, like the synthetic ephemeris) and the wave's
+// records are all in the LDS window: the record index and the normalised time
+// are computed once per force evaluation instead of once per segment.
+// slot is the record's position in the window (the same for every segment).
+__device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ spk,
+                                                     int32_t slot, int32_t win, double xs,
+                                                     double inv_radius, Vec3 x, Vec3 v,
+                                                     bool exact, const NonGravDev* ng, Vec3& acc,
+                                                     int32_t& impact_id) {
+  const double vscale = 86400.0 * inv_radius * INV_AU_KM;
+  auto coefs = [&](int s) -> const double* { return kh_win_rec + (s * win + slot) * 44 + 2; };
+  Vec3 emb = cheb_pos_n<14>(coefs(spk->emb_seg), xs);
+  emb = Vec3{emb.x * INV_AU_KM, emb.y * INV_AU_KM, emb.z * INV_AU_KM};
+  Vec3 a{0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int b = 0; b < N_PLANETS; ++b) {
+    const BodyDev& bd = spk->body[b];
+    // Opaque copy of the normalised time: keeps the compiler from hoisting
+    // the 26 loop-invariant Chebyshev T/U values out of the body loop, which
+    // would hold ~50 VGPRs for the whole force evaluation.
+    double xb = xs;
+    asm volatile("" : "+v"(xb));
+    Vec3 bp, bv{0.0, 0.0, 0.0};
+    if (bd.kind == 1) {
+      Vec3 p, d;
+      cheb_both_n<14>(coefs(bd.seg), xb, p, d);
+      bp = Vec3{p.x * INV_AU_KM, p.y * INV_AU_KM, p.z * INV_AU_KM};
+      bv = Vec3{d.x * vscale, d.y * vscale, d.z * vscale};
+    } else {
+      const Vec3 p = cheb_pos_n<14>(coefs(bd.seg), xb);
+      bp = Vec3{p.x * INV_AU_KM, p.y * INV_AU_KM, p.z * INV_AU_KM};
+      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+    }
+    const int32_t st = add_body_at(bd, bp, bv, x, v, exact, ng, a);
+    if (st) {
+      impact_id = bd.id;
+      return st;
+    }
+  }
+  acc = a;
+  return ST_OK;
+}
+
 // spk_accel: acceleration of a massless body at barycentric equatorial
 // (x, v) and TDB JD t, summed over kete's PLANETS in its order.  The
 // Earth-Moon barycentre is evaluated once and shared by 399 and 301 (kete
@@ -607,6 +661,19 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                              const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
                                              int32_t& impact_id, int32_t win = 0) {
+  if (spk->shared_grid) {
+    const SegDev& g = spk->seg[spk->emb_seg];
+    if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
+    const double js = (t - 2451545.0) * 86400.0;
+    const int32_t ri = record_index(g, js);
+    const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
+    if (win > 0 && __all(loc >= 0 && loc < win)) {
+      const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
+      const double ir = 1.0 / r0[1];
+      return accel_shared_grid(spk, loc, win, (js - r0[0]) * ir, ir, x, v, exact, ng, acc,
+                               impact_id);
+    }
+  }
   Vec3 a{0.0, 0.0, 0.0};
   Vec3 emb;
   int32_t st = seg_pos(spk, coef, spk->emb_seg, t, emb, win);
