@@ -175,6 +175,43 @@ int32_t kh_radec_vmag(kh_ctx *ctx, int64_t n, const double *const state[6],
                       const double *const obs[6], int32_t frame, const double *h_mag,
                       double g_param, double *const out[5]);
 
+/* Astrometric observables of n orbits at m epochs each and their partial
+ * derivatives with respect to 9 parameters: x, y, z, vx, vy, vz at jd0
+ * (Sun-centred, in `frame`) and A1, A2, A3 of the JplComet non-grav model.
+ * kete has no derivatives of propagate_n_body and no counterpart of this
+ * entry point; its observable chain is FovLike::check_n_body's
+ * (src/kete_core/src/fov/fov_like.rs:89-110): propagate_n_body_spk to each
+ * epoch (a new integrator per epoch, as successive propagate_n_body calls
+ * make), one two-body step back by the light time to the observer, then
+ * RA/Dec of the observer-to-object vector in the equatorial frame
+ * (src/kete_core/src/simult_states.rs:86-120).
+ * ng: [n], nullable; kind 0 is read as JplComet with the new_asteroid
+ * defaults and A1 = A2 = A3 = 0; kind 2 (dust) and JplComet with dt != 0
+ * are rejected.  jd_obs and obs[3]: [m][n]; obs is the observer position,
+ * Sun-centred, in `frame`.  force: 0 for kete's PLANETS model from the
+ * uploaded ephemeris, 1 for central_accel
+ * (src/kete_core/src/propagation/acceleration.rs:66-80: the Sun fixed at
+ * the origin, no ephemeris needed).  epsilon: Radau EPSILON (1e-6 as kete).
+ * Outputs: radec [2][m][n] (radians); jac [2][9][m][n] (dRA, dDec per
+ * parameter); state [6][m][n] and dstate [6][9][m][n] (nullable; dstate
+ * needs state): the geometric Sun-centred state at each epoch, in `frame`,
+ * and its partials; status [n] (kete's error codes; NaN outputs from the
+ * failing epoch on); counters [n] (nullable). */
+int32_t kh_radec_partials(kh_ctx *ctx, int64_t n, const double *jd0,
+                          const double *const xyz[6], int32_t frame, const kh_nongrav *ng,
+                          int32_t m, const double *jd_obs, const double *const obs[3],
+                          int32_t force, double epsilon, double *const radec[2],
+                          double *const jac[2], double *const state[6],
+                          double *const dstate[6], int32_t *status, kh_counters *counters);
+/* Device-pointer variant (all arrays in device memory). */
+int32_t kh_radec_partials_dev(kh_ctx *ctx, int64_t n, const double *jd0,
+                              const double *const xyz[6], int32_t frame, const kh_nongrav *ng,
+                              int32_t m, const double *jd_obs, const double *const obs[3],
+                              int32_t force, double epsilon, double *const radec[2],
+                              double *const jac[2], double *const state[6],
+                              double *const dstate[6], int32_t *status, kh_counters *counters,
+                              int32_t sync);
+
 /* Launch geometry last used by kh_propagate_n_body*: threads per block,
  * blocks, and mode actually run. */
 int32_t kh_last_launch(kh_ctx *ctx, int32_t *block, int32_t *grid, int32_t *mode);

@@ -219,6 +219,46 @@ class Context:
             cnt.ctypes.data_as(C.POINTER(_lib.KhCounters)) if cnt is not None else None))
         return dict(out=out, status=st, impact_id=iid, impact_jd=ijd, counters=cnt)
 
+    def radec_partials(self, jd0, xyz, jd_obs, obs, frame=FRAME_ECLIPTIC, non_gravs=None,
+                       force=0, epsilon=1e-6, states=False, counters=False):
+        """RA/Dec at each observation epoch and their partials with respect to
+        (x, y, z, vx, vy, vz, A1, A2, A3).
+
+        jd0: [n] or scalar; xyz: [6, n] Sun-centred in `frame`; jd_obs: [m]
+        or [m, n]; obs: observer position w.r.t. the Sun, [3, m] or [3, m, n].
+        -> dict(radec[2, m, n], jac[2, 9, m, n], status[n], and with
+        states=True state[6, m, n], dstate[6, 9, m, n]; counters[n, 4])"""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = xyz.shape[1]
+        jd0 = np.ascontiguousarray(np.broadcast_to(np.asarray(jd0, dtype=np.float64), (n,)))
+        jd_obs = np.asarray(jd_obs, dtype=np.float64)
+        m = jd_obs.shape[0]
+        jo = np.ascontiguousarray(np.broadcast_to(jd_obs.reshape(m, -1), (m, n)))
+        ob = np.asarray(obs, dtype=np.float64).reshape(3, m, -1)
+        ob = np.ascontiguousarray(np.broadcast_to(ob, (3, m, n)))
+        radec = np.empty((2, m, n))
+        jac = np.empty((2, 9, m, n))
+        st = np.empty(n, dtype=np.int32)
+        state = np.empty((6, m, n)) if states else None
+        dstate = np.empty((6, 9, m, n)) if states else None
+        cnt = np.zeros((n, 4), dtype=np.int32) if counters else None
+        ng = None
+        if non_gravs is not None:
+            ng_arr = non_gravs if isinstance(non_gravs, np.ndarray) else nongrav_array(non_gravs, n)
+            ng = ng_arr.ctypes.data_as(C.POINTER(_lib.KhNonGrav))
+        _lib.check(self.L.kh_radec_partials(
+            self.h, n, _lib.dptr(jd0), _lib.ptr_array(list(xyz)), frame, ng, m, _lib.dptr(jo),
+            _lib.ptr_array(list(ob)), force, epsilon, _lib.ptr_array(list(radec)),
+            _lib.ptr_array([jac[0].reshape(-1), jac[1].reshape(-1)]),
+            _lib.ptr_array(list(state)) if states else None,
+            _lib.ptr_array([dstate[k].reshape(-1) for k in range(6)]) if states else None,
+            _lib.iptr(st, C.c_int32),
+            cnt.ctypes.data_as(C.POINTER(_lib.KhCounters)) if cnt is not None else None))
+        out = dict(radec=radec, jac=jac, status=st, counters=cnt)
+        if states:
+            out.update(state=state, dstate=dstate)
+        return out
+
     def last_kernel_ms(self) -> float:
         return self.L.kh_last_kernel_ms(self.h)
 

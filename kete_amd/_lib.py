@@ -26,7 +26,7 @@ EXPORTS = [
     "kh_spk_upload_type2", "kh_spk_states", "kh_propagate_two_body",
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
-    "kh_radec_vmag",
+    "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev",
 ]
 
 
@@ -77,6 +77,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise KeteHipError(f"{LIB_PATH} is not built; run kete_amd._lib.build() "
                            "(hipcc --offload-arch=gfx950)")
+    # One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64
+    # (soname libamdhip64.so.7, linked by the unversioned name): with torch
+    # loaded first, this library binds to that copy; loaded the other way
+    # round, torch would load a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P, PP, I32, I64, D, V = (C.POINTER(C.c_double), C.POINTER(C.POINTER(C.c_double)), C.c_int32,
                              C.c_int64, C.c_double, C.c_void_p)
@@ -113,6 +121,12 @@ def load():
     L.kh_fov_state_check.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32
+    rp = [V, I64, P, PP, I32, C.POINTER(KhNonGrav), I32, P, PP, I32, D, PP, PP, PP, PP,
+          C.POINTER(I32), C.POINTER(KhCounters)]
+    L.kh_radec_partials.argtypes = rp
+    L.kh_radec_partials.restype = I32
+    L.kh_radec_partials_dev.argtypes = rp + [I32]
+    L.kh_radec_partials_dev.restype = I32
     L.kh_last_kernel_ms.argtypes = [V]
     L.kh_last_kernel_ms.restype = D
     _lib = L

@@ -94,7 +94,7 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
   }
   RadauTables t;
   fill_radau_tables(t);
-  if (upload_radau_tables(t) != 0) {
+  if (upload_radau_tables(t) != 0 || upload_grad_tables(t) != 0) {
     delete c;
     return fail(KH_E_HIP, "upload of Radau tables failed");
   }

@@ -154,4 +154,29 @@ int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s);
 int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s);
 int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s);
 
+// Observables and partial derivatives (kh_grad.hip).
+struct GradLaunch {
+  int64_t n;
+  int32_t m;
+  int32_t frame;
+  int32_t force;  // 0 spk_accel (PLANETS), 1 central_accel
+  double epsilon;
+  const double* jd0;
+  const double* xyz[6];
+  const void* ng;
+  const double* jd_obs;
+  const double* obs[3];
+  Chain sun;
+  double* radec[2];
+  double* jac[2];
+  double* state[6];
+  double* dstate[6];
+  int32_t* status;
+  int32_t* counters;
+};
+
+int32_t upload_grad_tables(const RadauTables& t);
+int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradLaunch& g,
+                              hipStream_t stream);
+
 }  // namespace kh

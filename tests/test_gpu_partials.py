@@ -1,0 +1,214 @@
+"""GPU parity of the observable/partials path (kh_radec_partials) and its
+torch.autograd wrapper.
+
+Checks, each against the oracle on the CPU:
+  - central force: the state partials equal the two-body state transition
+    matrix, taken by central differences of the analytic Kepler solution
+    (kepler.rs:254-297), which has no step-size noise;
+  - PLANETS force: values equal the oracle's observable chain (sequential
+    propagate_n_body arcs, light-time two-body step, RA/Dec) and the partials
+    equal central finite differences of that chain;
+  - the autograd gradient of chi^2 equals J^T (d chi^2 / d radec).
+Tolerances: values 1e-9 AU / 1e-9 rad; partials 1e-6 relative to the
+column's largest entry (the bar for gradient accuracy), except where noted.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import kete_amd as ka
+from kete_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+JD0 = syn.EPOCH_JD
+C_INV = 0.005775518331436995   # C_AU_PER_DAY_INV (constants/universal.rs)
+ROT_C, ROT_S = math.cos(0.40909280422232897), math.sin(0.40909280422232897)
+
+
+def ecl_to_equ(v):
+    v = np.asarray(v, dtype=np.float64)
+    return np.array([v[0], ROT_C * v[1] - ROT_S * v[2], ROT_S * v[1] + ROT_C * v[2]])
+
+
+def oracle_observables(oracle, jd0, xyz, jd_obs, obs, ng=None):
+    """The observable chain on the oracle for objects xyz [6, n] (Sun-centred
+    ecliptic at jd0): sequential propagate_n_body arcs to each epoch, then one
+    two-body step back by the light time, then RA/Dec of obs -> object.
+    -> radec [2, m, n], state [6, m, n] (Sun-centred ecliptic), status [n]."""
+    n, m = xyz.shape[1], len(jd_obs)
+    radec = np.full((2, m, n), np.nan)
+    state = np.full((6, m, n), np.nan)
+    status = np.zeros(n, dtype=np.int32)
+    cur = xyz.copy()
+    t = np.full(n, jd0)
+    for j in range(m):
+        r = oracle.batch_n_body(t, cur, jd_obs[j], center=10, frame=oracle.FRAME_ECLIPTIC,
+                                nongravs=ng, n_threads=8)
+        cur = r["out"]
+        t = np.full(n, jd_obs[j])
+        state[:, j] = cur
+        ob = ecl_to_equ(obs[:, j])
+        for i in range(n):
+            if r["status"][i] != 0:
+                status[i] = status[i] or r["status"][i]
+                continue
+            x, v = ecl_to_equ(cur[:3, i]), ecl_to_equ(cur[3:, i])
+            tau = np.linalg.norm(x - ob) * C_INV
+            st, p2, v2 = oracle.analytic_2_body(-tau, x, v)
+            assert st == 0
+            ra_dec = oracle.ra_dec_with_rates(p2, v2, ob, np.zeros(3))
+            radec[:, j, i] = ra_dec[:2]
+    return radec, state, status
+
+
+def neo_sample(n, seed_index=7):
+    return syn.n_body_workload(n=n, neo_fraction=1.0, seed_index=seed_index)
+
+
+def test_central_partials_are_two_body_stm(ctx, oracle_spk):
+    """force=1 (central_accel): d(state at t_j)/d(state at jd0) is the two-body
+    state transition matrix."""
+    oracle = oracle_spk
+    xyz = neo_sample(6)
+    jd_obs = JD0 + np.array([3.0, 17.0, 40.0, 95.0])
+    obs = np.zeros((3, len(jd_obs)))
+    obs[0] = 1.0                       # any observer; only the states are compared here
+    g = ctx.radec_partials(JD0, xyz, jd_obs, obs, force=1, states=True)
+    assert np.all(g["status"] == 0)
+    for i in range(xyz.shape[1]):
+        for j, tj in enumerate(jd_obs):
+            st, p, v = oracle.analytic_2_body(tj - JD0, xyz[:3, i], xyz[3:, i])
+            assert st == 0
+            # values: Radau on the central force vs the Kepler solution
+            assert np.abs(g["state"][:3, j, i] - p).max() < 1e-9
+            phi = np.empty((6, 6))
+            for k in range(6):
+                h = 1e-6 if k < 3 else 1e-8
+                xp, xm = xyz[:, i].copy(), xyz[:, i].copy()
+                xp[k] += h
+                xm[k] -= h
+                _, pp, vp = oracle.analytic_2_body(tj - JD0, xp[:3], xp[3:])
+                _, pm, vm = oracle.analytic_2_body(tj - JD0, xm[:3], xm[3:])
+                phi[:, k] = np.concatenate([pp - pm, vp - vm]) / (2 * h)
+            gpu = g["dstate"][:, :6, j, i]
+            scale = np.abs(phi).max(axis=0)
+            err = (np.abs(gpu - phi) / scale).max()
+            assert err < 1e-6, (i, j, err)
+            # no non-grav term in central_accel
+            assert np.all(g["dstate"][:, 6:, j, i] == 0.0)
+
+
+def test_planets_values_match_oracle_chain(ctx, oracle_spk):
+    xyz = neo_sample(64)
+    rng = np.random.default_rng(3)
+    a = rng.normal(0.0, 1e-12, (3, 64))
+    jd_obs = JD0 + np.sort(rng.uniform(0.0, 365.0, 8))
+    earth, _ = ctx.spk_state(399, jd_obs, 10, ka.Frames.Ecliptic)
+    models = [ka.NonGravModel.new_asteroid(*a[:, i]) for i in range(64)]
+    import oracle
+    ng_o = [oracle.NonGrav(1, *a[:, i], 1.0, 1.0, 2.0, 1.0, 0.0, 0.0, 0.0) for i in range(64)]
+    g = ctx.radec_partials(JD0, xyz, jd_obs, earth[:3], non_gravs=models, states=True,
+                           counters=True)
+    r_radec, r_state, r_status = oracle_observables(oracle_spk, JD0, xyz, jd_obs, earth[:3],
+                                                    ng_o)
+    assert np.array_equal(g["status"], r_status) and np.all(g["status"] == 0)
+    assert np.abs(g["state"][:3] - r_state[:3]).max() < 1e-9
+    # angular differences times the observer distance: 1e-9 AU on the sky
+    rho = np.linalg.norm(r_state[:3] - earth[:3, :, None], axis=0)
+    dra = np.abs((g["radec"][0] - r_radec[0] + np.pi) % (2 * np.pi) - np.pi)
+    assert (dra * np.cos(r_radec[1]) * rho).max() < 2e-9
+    assert (np.abs(g["radec"][1] - r_radec[1]) * rho).max() < 2e-9
+    assert np.all(g["counters"][:, 0] > 0)
+
+
+def test_planets_partials_match_central_differences(ctx, oracle_spk):
+    """All 9 columns (state and A1..A3) of d(state)/dp and d(RA, Dec)/dp
+    against central differences of the oracle's chain."""
+    n = 3
+    xyz = neo_sample(n, seed_index=9)
+    rng = np.random.default_rng(5)
+    a = rng.normal(0.0, 1e-12, (3, n))
+    jd_obs = JD0 + np.array([20.0, 75.0, 160.0])
+    earth, _ = ctx.spk_state(399, jd_obs, 10, ka.Frames.Ecliptic)
+    import oracle
+
+    def ng_list(aa):
+        return [oracle.NonGrav(1, *aa[:, i], 1.0, 1.0, 2.0, 1.0, 0.0, 0.0, 0.0)
+                for i in range(aa.shape[1])]
+
+    g = ctx.radec_partials(JD0, xyz, jd_obs, earth[:3],
+                           non_gravs=[ka.NonGravModel.new_asteroid(*a[:, i]) for i in range(n)],
+                           states=True)
+    assert np.all(g["status"] == 0)
+    # 5-point central differences, O(h^4).  The oracle's adaptive steps add
+    # ~1e-12 relative noise to each evaluation, so the FD error grows as 1/h
+    # below these steps (1e-6 AU gives 1e-6 relative); at them, 2-point and
+    # 5-point stencils agree to 3e-8.
+    steps = [1e-5] * 3 + [1e-6] * 3 + [1e-8] * 3
+    for k in range(9):
+        def chain(d):
+            x, aa = xyz.copy(), a.copy()
+            if k < 6:
+                x[k] += d
+            else:
+                aa[k - 6] += d
+            r, s_, _ = oracle_observables(oracle_spk, JD0, x, jd_obs, earth[:3], ng_list(aa))
+            return r, s_
+        h = steps[k]
+        ev = {c: chain(c * h) for c in (-2, -1, 1, 2)}
+        w = {-2: 1.0, -1: -8.0, 1: 8.0, 2: -1.0}
+        fd_state = sum(w[c] * ev[c][1] for c in w) / (12 * h)
+        base = ev[-2][0][0]
+        ra = {c: base + (ev[c][0][0] - base + np.pi) % (2 * np.pi) - np.pi for c in w}
+        fd_radec = np.stack([sum(w[c] * ra[c] for c in w),
+                             sum(w[c] * ev[c][0][1] for c in w)]) / (12 * h)
+        gs = g["dstate"][:, k]
+        gr = g["jac"][:, k]
+        for name, gpu, fd in (("state", gs, fd_state), ("radec", gr, fd_radec)):
+            scale = np.abs(fd).max()
+            err = np.abs(gpu - fd).max() / scale
+            assert err < 1e-6, (name, k, err)
+
+
+def test_autograd_chi2_gradient(ctx):
+    torch = pytest.importorskip("torch")
+    from kete_amd import astrometry as ast
+    w = ast.synthetic_observations(n=16, n_obs=10, years=1.0, seed_index=5, ctx=ctx)
+    p = w["params_true"].clone()
+    p[:, 0] += 1e-7                                    # off the truth
+    p.requires_grad_(True)
+    f = ast.chi2(p, w["jd0"], w["jd_obs"], w["obs"], w["ra_obs"], w["dec_obs"], w["sigma"])
+    f.backward()
+    # the same gradient assembled by hand from the partials
+    radec, jac, status = ast.radec_partials_dev(p.detach(), w["jd0"], w["jd_obs"], w["obs"])
+    assert int((status != 0).sum()) == 0
+    ra, dec = radec[0].t(), radec[1].t()               # [n, m]
+    dra = torch.remainder(ra - w["ra_obs"] + math.pi, 2 * math.pi) - math.pi
+    cd = torch.cos(w["dec_obs"])
+    g_ra = 2 * dra * cd ** 2 / w["sigma"] ** 2          # d chi2 / d RA   [n, m]
+    g_dec = 2 * (dec - w["dec_obs"]) / w["sigma"] ** 2  # d chi2 / d Dec  [n, m]
+    manual = (jac[0] * g_ra.t()[None]).sum(dim=1) + (jac[1] * g_dec.t()[None]).sum(dim=1)
+    manual = manual.t()                                  # [n, 9]
+    assert torch.allclose(p.grad, manual, rtol=1e-9, atol=0.0)
+    assert torch.isfinite(p.grad).all()
+
+
+def test_partials_error_semantics(ctx):
+    xyz = neo_sample(4)
+    xyz[0, 1] = np.nan
+    jd_obs = JD0 + np.array([5.0, 10.0])
+    earth, _ = ctx.spk_state(399, jd_obs, 10, ka.Frames.Ecliptic)
+    g = ctx.radec_partials(JD0, xyz, jd_obs, earth[:3])
+    assert g["status"][1] == 2 and np.all(np.isnan(g["radec"][:, :, 1]))
+    assert np.all(np.isnan(g["jac"][:, :, :, 1]))
+    assert np.all(g["status"][[0, 2, 3]] == 0)
+    # beyond the ephemeris: DAFLimits from the arc that leaves it on
+    g = ctx.radec_partials(JD0, xyz[:, [0]], JD0 + np.array([10.0, 5000.0]), earth[:3])
+    assert g["status"][0] == 3
+    assert np.all(np.isfinite(g["radec"][:, 0, 0])) and np.all(np.isnan(g["radec"][:, 1, 0]))
+    # dust models have no A1..A3: argument error
+    with pytest.raises(ka.KeteHipError):
+        ctx.radec_partials(JD0, xyz[:, [0]], jd_obs, earth[:3],
+                           non_gravs=[ka.NonGravModel.new_dust(0.01)])
