@@ -14,7 +14,9 @@ events on the library's stream.
 
 Other workloads (--config): 1 two-body (1e5 states x 64 epochs, solves/s),
 3 uneven spans (2e6 objects, |span| log-uniform in [1, 3650] d, persistent vs
-plain launch), 4 field-of-view checks (1e6 states x 3000 fields, pairs/s).
+plain launch), 4 field-of-view checks (1e6 states x 3000 fields, pairs/s),
+5 orbit-fit gradients (4096 NEO-like orbits x 60 observations, gradients/s,
+against finite differences of the plain propagators).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--config C]
 """
@@ -38,6 +40,13 @@ FP64_PEAK_TFLOPS = 78.6
 F_FORCE = 1848.0   # force evaluation: 11 Chebyshev lookups + Newton + GR + J2
 F_SWEEP = 1129.0   # predictor 7 substeps x 139 + B = G.C 147 + convergence 9
 F_ACCEPT = 111.0   # state update 107 + step-size 4
+# per parameter column of the partials kernel (derivative of the same
+# computation along one parameter): Jacobian-vector products of the force
+# terms (Newton 10 x 19, GR 2 x 60, J2 3 x 50, non-grav 100), the sweep's
+# linear predictor/corrector algebra, the accepted-step update
+F_TAN_FORCE = 560.0
+F_TAN_SWEEP = 1120.0
+F_TAN_ACCEPT = 107.0
 F_KEPLER0, F_KEPLER_IT = 97.0, 37.0   # universal Kepler: 97 + 37 per Newton iteration
 
 
@@ -228,6 +237,159 @@ def run_nbody(args, uneven=False):
     return line
 
 
+def run_adjoint(args):
+    """Gradients of an orbit-fit chi^2 (4096 NEO-like orbits, 60 observations
+    each over 2 years) with respect to the 6 state and 3 non-grav parameters,
+    through torch.autograd and the partials kernel; compared with central
+    finite differences of the plain propagators (13 evaluations: state only;
+    19: all 9 parameters), batched into one call and as separate calls."""
+    import torch
+
+    import kete_amd as ka
+    from kete_amd import astrometry as ast
+    from kete_amd import synthetic as syn
+    n = args.n or 4096
+    m = 60
+    ctx = ka.get_context()
+    log(f"[bench] adjoint workload n={n} m={m}")
+    w = ast.synthetic_observations(n=n, n_obs=m, years=2.0, noise_arcsec=0.1, seed_index=5,
+                                   ctx=ctx)
+    p0 = w["params_true"]
+    args_chi = (w["jd0"], w["jd_obs"], w["obs"], w["ra_obs"], w["dec_obs"], w["sigma"])
+
+    def grad_step():
+        p = p0.clone().requires_grad_(True)
+        f = ast.chi2(p, *args_chi)
+        f.backward()
+        return p.grad
+
+    for _ in range(max(args.warmup, 1)):
+        g_tan = grad_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k_ms = []
+    for _ in range(args.steps):
+        g_tan = grad_step()
+        k_ms.append(ctx.last_kernel_ms())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    k_ms = float(np.mean(k_ms))
+    # the same gradient with a new integrator at every epoch (successive
+    # propagate_n_body calls), for comparison
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p = p0.clone().requires_grad_(True)
+    ast.chi2(p, *args_chi, restart=True).backward()
+    torch.cuda.synchronize()
+    ms_restart = (time.perf_counter() - t0) * 1000.0
+    cnt = torch.zeros((n, 4), dtype=torch.int32, device="cuda:0")
+    _, _, status = ast.radec_partials_dev(p0, w["jd0"], w["jd_obs"], w["obs"], counters=cnt)
+    cnt = cnt.cpu().numpy().astype(np.int64)
+    n_bad = int((status != 0).sum())
+    log(f"[bench] autograd gradient: {ms:.1f} ms/step (kernel {k_ms:.1f} ms), "
+        f"{n / (ms / 1000.0):.0f} gradients/s, {n_bad} failed objects")
+
+    # finite differences of the plain propagators
+    jd_obs = w["jd_obs"][:, 0].cpu().numpy()
+    obs = w["obs"][:, :, 0].cpu().numpy()
+    ra_obs, dec_obs, sigma = w["ra_obs"], w["dec_obs"], w["sigma"]
+    steps_fd = torch.tensor([1e-6] * 3 + [1e-7] * 3 + [1e-9] * 3, dtype=torch.float64,
+                            device="cuda:0")
+
+    def wrap(x):
+        return torch.remainder(x + math.pi, 2 * math.pi) - math.pi
+
+    def fd_gradient(n_par, batched):
+        """Central-difference Jacobian of the predicted RA/Dec (2 n_par + 1
+        evaluations of the plain chain), then the chi^2 gradient
+        2 sum r J / sigma^2 from it, as a finite-difference orbit fit does."""
+        copies = [p0]
+        for k in range(n_par):
+            for sgn in (1.0, -1.0):
+                q = p0.clone()
+                q[:, k] += sgn * steps_fd[k]
+                copies.append(q)
+        if batched:
+            P = torch.cat(copies, dim=0)
+            radec, _ = ast.radec_plain(P, w["jd0"].repeat(len(copies)), jd_obs, obs, ctx=ctx)
+            rads = [radec[i * n:(i + 1) * n] for i in range(len(copies))]
+        else:
+            rads = [ast.radec_plain(q, w["jd0"], jd_obs, obs, ctx=ctx)[0] for q in copies]
+        J = []
+        for i in range(n_par):
+            d = rads[1 + 2 * i] - rads[2 + 2 * i]
+            d = torch.stack([wrap(d[..., 0]), d[..., 1]], dim=-1)
+            J.append(d / (2 * steps_fd[i]))
+        J = torch.stack(J)                                  # [n_par, n, m, 2]
+        dra = wrap(rads[0][..., 0] - ra_obs)
+        ddec = rads[0][..., 1] - dec_obs
+        g = 2.0 * ((dra * torch.cos(dec_obs) ** 2)[None] * J[..., 0] +
+                   ddec[None] * J[..., 1]).sum(dim=2) / sigma ** 2   # [n_par, n]
+        return g.t(), J
+
+    _, jac_k, _ = ast.radec_partials_dev(p0, w["jd0"], w["jd_obs"], w["obs"])
+    ok = (status == 0)
+    fd = {}
+    for n_par in (6, 9):
+        fd_gradient(n_par, True)                # warm-up (allocations, code objects)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g_fd, J_fd = fd_gradient(n_par, True)
+        torch.cuda.synchronize()
+        ms_b = (time.perf_counter() - t0) * 1000.0
+        t0 = time.perf_counter()
+        fd_gradient(n_par, False)
+        torch.cuda.synchronize()
+        ms_s = (time.perf_counter() - t0) * 1000.0
+        scale = g_tan[:, :n_par].abs().amax(dim=1).clamp_min(1e-300)
+        rel = ((g_fd - g_tan[:, :n_par]).abs().amax(dim=1) / scale)[ok]
+        # Jacobian agreement, per object relative to its largest entry per column
+        jk = jac_k[:, :n_par].permute(1, 3, 2, 0)           # [n_par, n, m, 2]
+        jscale = jk.abs().amax(dim=(2, 3), keepdim=True).clamp_min(1e-300)
+        jrel = ((J_fd - jk).abs() / jscale).amax(dim=(0, 2, 3))[ok]
+        fd[f"fd{2 * n_par + 1}"] = {
+            "evaluations": 2 * n_par + 1, "params": n_par,
+            "steps": [float(x) for x in steps_fd[:n_par]],
+            "batched_ms": ms_b, "sequential_ms": ms_s,
+            "batched_gradients_per_s": n / (ms_b / 1000.0),
+            "sequential_gradients_per_s": n / (ms_s / 1000.0),
+            "speedup_vs_batched": ms_b / ms, "speedup_vs_sequential": ms_s / ms,
+            "grad_rel_diff": {"median": float(rel.median()),
+                              "p90": float(rel.quantile(0.9)), "max": float(rel.max())},
+            "jacobian_rel_diff": {"median": float(jrel.median()),
+                                  "p90": float(jrel.quantile(0.9)),
+                                  "max": float(jrel.max())}}
+        log(f"[bench] FD-{2 * n_par + 1}: batched {ms_b:.1f} ms, sequential {ms_s:.1f} ms; "
+            f"jacobian rel. diff median {float(jrel.median()):.2e} p90 "
+            f"{float(jrel.quantile(0.9)):.2e}; gradient rel. diff median "
+            f"{float(rel.median()):.2e} p90 {float(rel.quantile(0.9)):.2e}")
+
+    n_force, n_sweep, n_acc = (int(cnt[:, k].sum()) for k in range(3))
+    flop = (F_FORCE * n_force + F_SWEEP * n_sweep + F_ACCEPT * n_acc +
+            9 * (F_TAN_FORCE * n_force + F_TAN_SWEEP * n_sweep + F_TAN_ACCEPT * n_acc))
+    achieved = flop / (k_ms / 1000.0) / 1e12
+    return {"metric": "adjoint gradients/s (chi^2 over 6 state + 3 non-grav params)",
+            "value": n / (ms / 1000.0), "unit": "gradients/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "orbit_fit_gradients_4096_neo_x_60_obs_2yr", "objects": n,
+                       "observations": m, "years": 2.0, "noise_arcsec": 0.1,
+                       "nongrav": "A1..A3 ~ N(0, 1e-12) AU/d^2, new_asteroid defaults",
+                       "seed": syn.SEED_BASE + 5,
+                       "method": "forward-mode partials in the propagation kernel, "
+                                 "torch.autograd.Function",
+                       "chain": "one integration through the 60 epochs"},
+            "restart_per_epoch_ms": ms_restart,
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "traffic": None, "kernel": "radau_partials_kernel",
+                         "kernel_ms": k_ms, "flop_per_launch": flop,
+                         "work": {"force_evals": n_force, "sweeps": n_sweep,
+                                  "accepted_steps": n_acc}},
+            "finite_differences": fd, "failed_objects": n_bad}
+
+
 def run_two_body(args):
     import torch
 
@@ -329,7 +491,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--fields", type=int, default=3000)
-    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4])
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4, 5])
     ap.add_argument("--mode", type=int, default=-1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of the baseline sample")
@@ -347,6 +509,8 @@ def main():
         line = run_two_body(args)
     elif args.config == 0:
         line = run_copy_calibration(args)
+    elif args.config == 5:
+        line = run_adjoint(args)
     else:
         line = run_fov(args)
     print(json.dumps(line))

@@ -191,7 +191,11 @@ int32_t kh_radec_vmag(kh_ctx *ctx, int64_t n, const double *const state[6],
  * Sun-centred, in `frame`.  force: 0 for kete's PLANETS model from the
  * uploaded ephemeris, 1 for central_accel
  * (src/kete_core/src/propagation/acceleration.rs:66-80: the Sun fixed at
- * the origin, no ephemeris needed).  epsilon: Radau EPSILON (1e-6 as kete).
+ * the origin, no ephemeris needed).  restart: 1 starts a new integrator at
+ * each epoch (B = 0, h = 0.1 d), as successive propagate_n_body calls do; 0
+ * integrates once through all epochs, shortening the step that reaches each
+ * epoch and keeping the integrator state.  epsilon: Radau EPSILON (1e-6 as
+ * kete).
  * Outputs: radec [2][m][n] (radians); jac [2][9][m][n] (dRA, dDec per
  * parameter); state [6][m][n] and dstate [6][9][m][n] (nullable; dstate
  * needs state): the geometric Sun-centred state at each epoch, in `frame`,
@@ -200,14 +204,15 @@ int32_t kh_radec_vmag(kh_ctx *ctx, int64_t n, const double *const state[6],
 int32_t kh_radec_partials(kh_ctx *ctx, int64_t n, const double *jd0,
                           const double *const xyz[6], int32_t frame, const kh_nongrav *ng,
                           int32_t m, const double *jd_obs, const double *const obs[3],
-                          int32_t force, double epsilon, double *const radec[2],
+                          int32_t force, int32_t restart, double epsilon, double *const radec[2],
                           double *const jac[2], double *const state[6],
                           double *const dstate[6], int32_t *status, kh_counters *counters);
 /* Device-pointer variant (all arrays in device memory). */
 int32_t kh_radec_partials_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                               const double *const xyz[6], int32_t frame, const kh_nongrav *ng,
                               int32_t m, const double *jd_obs, const double *const obs[3],
-                              int32_t force, double epsilon, double *const radec[2],
+                              int32_t force, int32_t restart, double epsilon,
+                              double *const radec[2],
                               double *const jac[2], double *const state[6],
                               double *const dstate[6], int32_t *status, kh_counters *counters,
                               int32_t sync);

@@ -220,7 +220,7 @@ class Context:
         return dict(out=out, status=st, impact_id=iid, impact_jd=ijd, counters=cnt)
 
     def radec_partials(self, jd0, xyz, jd_obs, obs, frame=FRAME_ECLIPTIC, non_gravs=None,
-                       force=0, epsilon=1e-6, states=False, counters=False):
+                       force=0, restart=False, epsilon=1e-6, states=False, counters=False):
         """RA/Dec at each observation epoch and their partials with respect to
         (x, y, z, vx, vy, vz, A1, A2, A3).
 
@@ -248,7 +248,7 @@ class Context:
             ng = ng_arr.ctypes.data_as(C.POINTER(_lib.KhNonGrav))
         _lib.check(self.L.kh_radec_partials(
             self.h, n, _lib.dptr(jd0), _lib.ptr_array(list(xyz)), frame, ng, m, _lib.dptr(jo),
-            _lib.ptr_array(list(ob)), force, epsilon, _lib.ptr_array(list(radec)),
+            _lib.ptr_array(list(ob)), force, int(restart), epsilon, _lib.ptr_array(list(radec)),
             _lib.ptr_array([jac[0].reshape(-1), jac[1].reshape(-1)]),
             _lib.ptr_array(list(state)) if states else None,
             _lib.ptr_array([dstate[k].reshape(-1) for k in range(6)]) if states else None,

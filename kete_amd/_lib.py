@@ -121,7 +121,7 @@ def load():
     L.kh_fov_state_check.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32
-    rp = [V, I64, P, PP, I32, C.POINTER(KhNonGrav), I32, P, PP, I32, D, PP, PP, PP, PP,
+    rp = [V, I64, P, PP, I32, C.POINTER(KhNonGrav), I32, P, PP, I32, I32, D, PP, PP, PP, PP,
           C.POINTER(I32), C.POINTER(KhCounters)]
     L.kh_radec_partials.argtypes = rp
     L.kh_radec_partials.restype = I32

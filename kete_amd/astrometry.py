@@ -45,10 +45,12 @@ def _asteroid_models(a123: torch.Tensor) -> torch.Tensor:
 
 
 def radec_partials_dev(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, epsilon=1e-6,
-                       ctx=None):
+                       ctx=None, counters=None, restart=False):
     """One launch on device tensors.
 
-    params: [n, 9] float64 (cuda); jd0: [n]; jd_obs: [m, n]; obs: [3, m, n].
+    params: [n, 9] float64 (cuda); jd0: [n]; jd_obs: [m, n]; obs: [3, m, n];
+    counters: optional int32 [n, 4] device tensor receiving (force
+    evaluations, sweeps, accepted steps, failed steps) per object.
     -> (radec [2, m, n], jac [2, 9, m, n], status [n] int32)."""
     ctx = ctx or get_context()
     dev = params.device
@@ -66,10 +68,11 @@ def radec_partials_dev(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, 
     _lib.check(ctx.L.kh_radec_partials_dev(
         ctx.h, n, _lib.raw(jd0.data_ptr()), _lib.ptr_array([c.data_ptr() for c in cols]), frame,
         _lib.raw(ng.data_ptr(), _lib.KhNonGrav), m, _lib.raw(jd_obs.data_ptr()),
-        _lib.ptr_array([obs[k].data_ptr() for k in range(3)]), force, epsilon,
+        _lib.ptr_array([obs[k].data_ptr() for k in range(3)]), force, int(restart), epsilon,
         _lib.ptr_array([radec[0].data_ptr(), radec[1].data_ptr()]),
         _lib.ptr_array([jac[0].data_ptr(), jac[1].data_ptr()]), None, None,
-        _lib.raw(status.data_ptr(), _lib.C.c_int32), None, 1))
+        _lib.raw(status.data_ptr(), _lib.C.c_int32),
+        _lib.raw(counters.data_ptr(), _lib.KhCounters) if counters is not None else None, 1))
     return radec, jac, status
 
 
@@ -78,9 +81,9 @@ class RaDec(torch.autograd.Function):
     partials computed in the same launch."""
 
     @staticmethod
-    def forward(fctx, params, jd0, jd_obs, obs, frame, force, epsilon):
+    def forward(fctx, params, jd0, jd_obs, obs, frame, force, epsilon, restart):
         radec, jac, status = radec_partials_dev(params.detach(), jd0, jd_obs, obs, frame, force,
-                                                epsilon)
+                                                epsilon, restart=restart)
         fctx.save_for_backward(jac)
         fctx.status = status
         return radec.permute(2, 1, 0).contiguous()
@@ -90,23 +93,83 @@ class RaDec(torch.autograd.Function):
         (jac,) = fctx.saved_tensors
         g = grad.permute(2, 1, 0).contiguous()           # [2, m, n]
         gp = (jac * g[:, None]).sum(dim=(0, 2))          # [9, n]
-        return gp.t().contiguous(), None, None, None, None, None, None
+        return gp.t().contiguous(), None, None, None, None, None, None, None
 
 
-def radec(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, epsilon=1e-6):
-    """Differentiable predicted RA/Dec [n, m, 2] (radians)."""
-    return RaDec.apply(params, jd0, jd_obs, obs, frame, force, epsilon)
+def radec(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, epsilon=1e-6, restart=False):
+    """Differentiable predicted RA/Dec [n, m, 2] (radians).  restart: a new
+    integrator at each epoch (as successive propagate_n_body calls) instead of
+    one integration through all epochs; the two agree to the integrator's
+    error level."""
+    return RaDec.apply(params, jd0, jd_obs, obs, frame, force, epsilon, restart)
 
 
 def chi2(params, jd0, jd_obs, obs, ra_obs, dec_obs, sigma, frame=FRAME_ECLIPTIC, force=0,
-         epsilon=1e-6):
+         epsilon=1e-6, restart=False):
     """Sum over objects and epochs of ((dRA cos Dec_obs)^2 + dDec^2) / sigma^2,
     with dRA wrapped to [-pi, pi).  ra_obs, dec_obs: [n, m]; sigma in radians
     (scalar or [n, m]).  Returns a scalar tensor."""
-    p = radec(params, jd0, jd_obs, obs, frame, force, epsilon)
+    p = radec(params, jd0, jd_obs, obs, frame, force, epsilon, restart)
     dra = torch.remainder(p[..., 0] - ra_obs + math.pi, 2 * math.pi) - math.pi
     ddec = p[..., 1] - dec_obs
     return (((dra * torch.cos(dec_obs)) ** 2 + ddec ** 2) / sigma ** 2).sum()
+
+
+_OBL_C, _OBL_S = math.cos(0.40909280422232897), math.sin(0.40909280422232897)
+
+
+def radec_plain(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, epsilon=1e-6, ctx=None):
+    """The same observables without derivatives, from the plain propagators:
+    per epoch one kh_propagate_n_body_dev launch (all objects to the epoch),
+    one kh_propagate_two_body_dev launch with the observer (the light-time
+    step, kepler.rs:68-108), then RA/Dec on the device.  Epochs are shared by
+    all objects here.  Used as the finite-difference baseline.
+
+    params: [N, 9] (cuda); jd0: [N]; jd_obs: [m] (host); obs: [3, m] (host,
+    observer w.r.t. the Sun in `frame`).  -> (radec [N, m, 2], status [N])."""
+    ctx = ctx or get_context()
+    L = ctx.L
+    dev = params.device
+    n = params.shape[0]
+    jd_obs = np.asarray(jd_obs, dtype=np.float64)
+    obs = np.ascontiguousarray(obs, dtype=np.float64)
+    m = len(jd_obs)
+    state = params[:, :6].t().contiguous()
+    ng = _asteroid_models(params[:, 6:9].contiguous())
+    t = jd0.to(dev, torch.float64).contiguous()
+    epochs = torch.as_tensor(jd_obs, dtype=torch.float64, device=dev)
+    t_vecs = [torch.full((n,), float(jd_obs[j]), dtype=torch.float64, device=dev)
+              for j in range(m)]
+    out = torch.empty((n, m, 2), dtype=torch.float64, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    st_nb = torch.empty(n, dtype=torch.int32, device=dev)
+    st_tb = torch.empty(n, dtype=torch.int32, device=dev)
+    new = torch.empty_like(state)
+    lt = torch.empty_like(state)
+    cols = lambda s: _lib.ptr_array([s[k].data_ptr() for k in range(6)])
+    i32 = lambda x: _lib.raw(x.data_ptr(), _lib.C.c_int32)
+    for j in range(m):
+        torch.cuda.synchronize(dev)
+        _lib.check(L.kh_propagate_n_body_dev(
+            ctx.h, n, _lib.raw(t.data_ptr()), cols(state), 10, frame,
+            _lib.raw(epochs[j:].data_ptr()), 0, _lib.raw(ng.data_ptr(), _lib.KhNonGrav),
+            epsilon, -1, cols(new), i32(st_nb), None, None, None, 0))
+        ob = np.ascontiguousarray(obs[:, j])
+        _lib.check(L.kh_propagate_two_body_dev(
+            ctx.h, n, _lib.raw(t_vecs[j].data_ptr()), cols(new), 10, frame, 1,
+            _lib.raw(epochs[j:].data_ptr()), _lib.dptr(ob), cols(lt), i32(st_tb), 1))
+        status = torch.where(status == 0, st_nb, status)
+        status = torch.where(status == 0, st_tb, status)
+        rho = lt[:3] - torch.as_tensor(ob, dtype=torch.float64, device=dev)[:, None]
+        if frame == FRAME_ECLIPTIC:
+            rho = torch.stack([rho[0], _OBL_C * rho[1] - _OBL_S * rho[2],
+                               _OBL_S * rho[1] + _OBL_C * rho[2]])
+        r = torch.linalg.vector_norm(rho, dim=0)
+        out[:, j, 0] = torch.remainder(torch.atan2(rho[1], rho[0]), 2 * math.pi)
+        out[:, j, 1] = torch.asin(torch.clamp(rho[2] / r, -1.0, 1.0))
+        state, new = new, state
+        t = t_vecs[j]
+    return out, status
 
 
 def synthetic_observations(n=4096, n_obs=60, years=2.0, noise_arcsec=0.1, seed_index=5,

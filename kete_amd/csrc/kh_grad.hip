@@ -17,8 +17,11 @@
 //    communication.  The derivative is that of the computed map with the step
 //    sequence held fixed (the step sizes depend on the parameters only at the
 //    integrator's error level, ~1e-13 relative).
-//  - Observation epochs are visited in the order given; each arc restarts the
-//    integrator (B = 0, h = 0.1 d), as successive propagate_n_body calls do.
+//  - Observation epochs are visited in the order given.  restart = 1: each arc
+//    starts a new integrator (B = 0, h = 0.1 d), as successive
+//    propagate_n_body calls do.  restart = 0: one integration through all
+//    epochs; a step is shortened to land on each epoch, and the integrator
+//    keeps B and the step size it proposed before shortening.
 //  - The loop body makes one force evaluation, as in the propagation kernel,
 //    so every live lane of a wavefront evaluates the force together.
 //  - Light-time step: the value uses the universal Kepler solver; its
@@ -210,6 +213,7 @@ struct GradArgs {
   int32_t m;
   int32_t frame;  // input/output frame: 1 equatorial, 17 ecliptic (Sun-centred)
   int32_t force;  // 0: spk_accel with PLANETS; 1: central_accel, no ephemeris
+  int32_t restart;  // 1: a new integrator per epoch; 0: one integration
   double epsilon;
   const double* jd0;        // [n]
   const double* xyz[6];     // [n], Sun-centred, input frame
@@ -319,7 +323,8 @@ void radau_partials_kernel(const SpkDev* __restrict__ spk, const double* __restr
   DVec3 B[7], G[7];
 #pragma unroll
   for (int i = 0; i < 7; ++i) B[i] = G[i] = zero3;
-  double h = 0.0, next_h = 0.0, tf = 0.0;
+  double h = 0.0, next_h = 0.0, tf = 0.0, h_prop = 0.0;
+  bool clamped = false;  // the current step was shortened to land on tf
   double bprev[3] = {0.0, 0.0, 0.0};
   int32_t phase = GP_INIT, sub = 1, sweep = 0, fails = 0, j = 0;
   int4 cnt{0, 0, 0, 0};
@@ -460,7 +465,9 @@ void radau_partials_kernel(const SpkDev* __restrict__ spk, const double* __restr
       }
     }
     if (start_step) {
-      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      h_prop = h;
+      clamped = fabs(t - tf) <= fabs(h);
+      if (clamped) h = tf - t;
       sweep = 0;
       sub = 1;
 #pragma unroll
@@ -530,13 +537,34 @@ void radau_partials_kernel(const SpkDev* __restrict__ spk, const double* __restr
           p.dstate[5][jk] = vo.z.d;
         }
       }
-      // next arc: a new integrator from this state (RadauIntegrator::new)
       ++j;
       if (j < m) {
         tf = p.jd_obs[(int64_t)j * n + o];
+        if (p.restart) {
+          // a new integrator from this state (RadauIntegrator::new)
 #pragma unroll
-        for (int i = 0; i < 7; ++i) B[i] = zero3;
-        phase = GP_INIT;
+          for (int i = 0; i < 7; ++i) B[i] = zero3;
+          phase = GP_INIT;
+        } else if (fabs(tf - t) < 1e-10) {
+          phase = GP_EXACT;  // the same epoch again: a zero-length arc
+        } else {
+          // carry on with the step size the controller proposed: before the
+          // last step was shortened to land on the epoch, else after it
+          h = copysign(fmax(fabs(clamped ? h_prop : h), 0.005), tf - t);
+          fails = 0;
+          h_prop = h;
+          clamped = fabs(t - tf) <= fabs(h);
+          if (clamped) h = tf - t;
+          sweep = 0;
+          sub = 1;
+#pragma unroll
+          for (int i = 0; i < 7; ++i) G[i] = zero3;
+          bprev[0] = B[6].x.v;
+          bprev[1] = B[6].y.v;
+          bprev[2] = B[6].z.v;
+          phase = GP_SUB;
+          cnt.y++;
+        }
       }
     }
   }
@@ -571,6 +599,7 @@ int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradL
   p.m = g.m;
   p.frame = g.frame;
   p.force = g.force;
+  p.restart = g.restart;
   p.epsilon = g.epsilon;
   p.jd0 = g.jd0;
   for (int c = 0; c < 6; ++c) {

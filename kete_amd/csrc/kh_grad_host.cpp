@@ -59,7 +59,7 @@ int32_t check_args(kh_ctx* c, int64_t n, const double* jd0, const double* const 
 
 int32_t run_partials(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],
                      int32_t frame, const kh_nongrav* ng, int32_t m, const double* jd_obs,
-                     const double* const obs[3], int32_t force, double epsilon,
+                     const double* const obs[3], int32_t force, int32_t restart, double epsilon,
                      double* const radec[2], double* const jac[2], double* const state[6],
                      double* const dstate[6], int32_t* status, kh_counters* counters,
                      const Chain& sun, int32_t sync) {
@@ -68,6 +68,7 @@ int32_t run_partials(kh_ctx* c, int64_t n, const double* jd0, const double* cons
   g.m = m;
   g.frame = frame;
   g.force = force;
+  g.restart = restart ? 1 : 0;
   g.epsilon = epsilon;
   g.jd0 = jd0;
   for (int k = 0; k < 6; ++k) {
@@ -101,7 +102,7 @@ extern "C" {
 int32_t kh_radec_partials_dev(kh_ctx* c, int64_t n, const double* jd0,
                               const double* const xyz[6], int32_t frame, const kh_nongrav* ng,
                               int32_t m, const double* jd_obs, const double* const obs[3],
-                              int32_t force, double epsilon, double* const radec[2],
+                              int32_t force, int32_t restart, double epsilon, double* const radec[2],
                               double* const jac[2], double* const state[6],
                               double* const dstate[6], int32_t* status, kh_counters* counters,
                               int32_t sync) {
@@ -120,13 +121,13 @@ int32_t kh_radec_partials_dev(kh_ctx* c, int64_t n, const double* jd0,
     r = check_ng(h.data(), n);
     if (r) return r;
   }
-  return run_partials(c, n, jd0, xyz, frame, ng, m, jd_obs, obs, force, epsilon, radec, jac,
-                      state, dstate, status, counters, sun, sync);
+  return run_partials(c, n, jd0, xyz, frame, ng, m, jd_obs, obs, force, restart, epsilon, radec,
+                      jac, state, dstate, status, counters, sun, sync);
 }
 
 int32_t kh_radec_partials(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],
                           int32_t frame, const kh_nongrav* ng, int32_t m, const double* jd_obs,
-                          const double* const obs[3], int32_t force, double epsilon,
+                          const double* const obs[3], int32_t force, int32_t restart, double epsilon,
                           double* const radec[2], double* const jac[2], double* const state[6],
                           double* const dstate[6], int32_t* status, kh_counters* counters) {
   Chain sun;
@@ -179,7 +180,8 @@ int32_t kh_radec_partials(kh_ctx* c, int64_t n, const double* jd0, const double*
     for (int k = 0; k < 3; ++k)
       HIPCHK(hipMemcpyAsync((void*)d_obs[k], obs[k], mn * 8, hipMemcpyHostToDevice, s));
   }
-  r = run_partials(c, n, d_jd0, d_xyz, frame, d_ng, m, d_jo, d_obs, force, epsilon, d_radec,
+  r = run_partials(c, n, d_jd0, d_xyz, frame, d_ng, m, d_jo, d_obs, force, restart, epsilon,
+                   d_radec,
                    d_jac, state ? d_state : nullptr, dstate ? d_dstate : nullptr, d_st,
                    counters ? d_cnt : nullptr, sun, 0);
   if (r) return r;

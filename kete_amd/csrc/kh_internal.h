@@ -160,6 +160,7 @@ struct GradLaunch {
   int32_t m;
   int32_t frame;
   int32_t force;  // 0 spk_accel (PLANETS), 1 central_accel
+  int32_t restart;
   double epsilon;
   const double* jd0;
   const double* xyz[6];

@@ -100,7 +100,11 @@ def test_central_partials_are_two_body_stm(ctx, oracle_spk):
             assert np.all(g["dstate"][:, 6:, j, i] == 0.0)
 
 
-def test_planets_values_match_oracle_chain(ctx, oracle_spk):
+@pytest.mark.parametrize("restart", [True, False])
+def test_planets_values_match_oracle_chain(ctx, oracle_spk, restart):
+    """restart=True is the oracle's chain exactly (a new integrator per
+    epoch); restart=False integrates through the epochs and agrees to the
+    integrator's error level."""
     xyz = neo_sample(64)
     rng = np.random.default_rng(3)
     a = rng.normal(0.0, 1e-12, (3, 64))
@@ -110,7 +114,7 @@ def test_planets_values_match_oracle_chain(ctx, oracle_spk):
     import oracle
     ng_o = [oracle.NonGrav(1, *a[:, i], 1.0, 1.0, 2.0, 1.0, 0.0, 0.0, 0.0) for i in range(64)]
     g = ctx.radec_partials(JD0, xyz, jd_obs, earth[:3], non_gravs=models, states=True,
-                           counters=True)
+                           counters=True, restart=restart)
     r_radec, r_state, r_status = oracle_observables(oracle_spk, JD0, xyz, jd_obs, earth[:3],
                                                     ng_o)
     assert np.array_equal(g["status"], r_status) and np.all(g["status"] == 0)
