@@ -179,6 +179,18 @@ def run_nbody(args, uneven=False):
                                  "value": obj_years / (ms_plain / 1000.0)}
         log(f"[bench] plain launch: {ms_plain:.1f} ms/step")
 
+    if not uneven:
+        # the host-array entry point (inputs and outputs cross PCIe), once
+        batch = ka.StateArray(np.full(n, jd0), xyz)
+        t0 = time.perf_counter()
+        ctx.n_body(batch, jdf[0], mode=mode)
+        ms_host = (time.perf_counter() - t0) * 1000.0
+        extra["host_arrays"] = {"ms_per_step": ms_host,
+                                "value": obj_years / (ms_host / 1000.0),
+                                "note": "kh_propagate_n_body with host arrays: staging "
+                                        "copies over PCIe included"}
+        log(f"[bench] host-array entry point: {ms_host:.1f} ms/step")
+
     res_gpu = t["out"].cpu().numpy()
     cpu = parity = None
     if not args.no_cpu:
