@@ -109,6 +109,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->d_coef) (void)hipFree(c->d_coef);
   if (c->d_spk) (void)hipFree(c->d_spk);
   if (c->dbuf) (void)hipFree(c->dbuf);
+  if (c->obuf) (void)hipFree(c->obuf);
   if (c->fbuf) (void)hipFree(c->fbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
@@ -453,6 +454,12 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
     // persistent: enough blocks to fill every CU at the kernel's occupancy
     int per_cu = 2;
     p.grid = c->n_cu * per_cu * 4;
+    // longest-first queue order (KH_ORDER=0 turns it off)
+    const char* ord = getenv("KH_ORDER");
+    if (!(ord && atoi(ord) == 0)) {
+      if (kh_grow(&c->obuf, &c->obuf_size, order_buf_bytes(n)) == KH_OK) p.order_buf = c->obuf;
+      else return KH_E_HIP;
+    }
   }
   if (launch_nbody(c->d_spk, c->d_coef, p, c->stream, &c->last_block, &c->last_grid,
                    &c->last_mode, c->ev0, c->ev1))

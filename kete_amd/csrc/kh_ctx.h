@@ -28,6 +28,8 @@ struct kh_ctx {
   std::vector<int64_t> seg_obj, seg_center;
   void* dbuf = nullptr;
   size_t dbuf_size = 0;
+  void* obuf = nullptr;  // queue-order scratch of the N-body launch
+  size_t obuf_size = 0;
   void* fbuf = nullptr;  // FOV working memory
   size_t fbuf_size = 0;
   unsigned long long* d_queue = nullptr;

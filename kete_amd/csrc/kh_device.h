@@ -614,13 +614,21 @@ __device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
 // records are all in the LDS window: the record index and the normalised time
 // are computed once per force evaluation instead of once per segment.
 // slot is the record's position in the window (the same for every segment).
+// UNI: the wavefront's record is uniform and `base` points at its
+// coefficients in segment 0's numbering (coefficients then come through
+// scalar loads); otherwise they are read from the LDS window at `slot`.
+template <bool UNI>
 __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ spk,
+                                                     const double* __restrict__ base,
                                                      int32_t slot, int32_t win, double xs,
                                                      double inv_radius, Vec3 x, Vec3 v,
                                                      bool exact, const NonGravDev* ng, Vec3& acc,
                                                      int32_t& impact_id) {
   const double vscale = 86400.0 * inv_radius * INV_AU_KM;
-  auto coefs = [&](int s) -> const double* { return kh_win_rec + (s * win + slot) * 44 + 2; };
+  auto coefs = [&](int s) -> const double* {
+    if (UNI) return base + spk->seg[s].off;
+    return kh_win_rec + (s * win + slot) * 44 + 2;
+  };
   Vec3 emb = cheb_pos_n<14>(coefs(spk->emb_seg), xs);
   emb = Vec3{emb.x * INV_AU_KM, emb.y * INV_AU_KM, emb.z * INV_AU_KM};
   Vec3 a{0.0, 0.0, 0.0};
@@ -666,12 +674,21 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
     const double js = (t - 2451545.0) * 86400.0;
     const int32_t ri = record_index(g, js);
+#ifdef KH_SCALAR_FIRST
+    const int32_t ru = __builtin_amdgcn_readfirstlane(ri);
+    if (__all(ri == ru)) {
+      const double* rec = coef + g.off + (int64_t)ru * 44;
+      const double ir = 1.0 / rec[1];
+      return accel_shared_grid<true>(spk, coef + (int64_t)ru * 44 + 2, 0, 0,
+                                     (js - rec[0]) * ir, ir, x, v, exact, ng, acc, impact_id);
+    }
+#endif
     const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
     if (win > 0 && __all(loc >= 0 && loc < win)) {
       const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
       const double ir = 1.0 / r0[1];
-      return accel_shared_grid(spk, loc, win, (js - r0[0]) * ir, ir, x, v, exact, ng, acc,
-                               impact_id);
+      return accel_shared_grid<false>(spk, nullptr, loc, win, (js - r0[0]) * ir, ir, x, v,
+                                      exact, ng, acc, impact_id);
     }
   }
   Vec3 a{0.0, 0.0, 0.0};

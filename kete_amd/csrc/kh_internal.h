@@ -47,7 +47,11 @@ struct NBodyLaunch {
   unsigned long long* range;  // [2] epoch-range scratch, or null: no LDS window
   int32_t window;             // records per segment staged in LDS
   int32_t n_seg;
+  void* order_buf;  // scratch for the longest-first queue order (order_buf_bytes), or null
 };
+
+// Bytes of NBodyLaunch::order_buf for n objects.
+inline size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket
 // the integrator kernel alone.

@@ -70,6 +70,7 @@ struct NBodyArgs {
   int64_t first_dynamic;  // objects below this are assigned by lane id
   unsigned long long* range;  // [2]: bits of min and max epoch over the batch
   int32_t window;             // records per segment staged in LDS (0: none)
+  const uint32_t* perm;       // queue slot -> object (longest first), or null
 };
 
 enum : int32_t { PH_INIT = 0, PH_SUB = 1, PH_EXACT = 2 };
@@ -201,12 +202,14 @@ __global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
   a.out[5][i] = v.z;
 }
 
-// Next object whose pre-pass succeeded, from index i on; >= n when none is
+// Next object whose pre-pass succeeded, from queue slot `slot` on (the slot
+// is the object index unless a longest-first order is given); n when none is
 // left.  Its SSB-equatorial starting state is read from the output arrays.
 template <bool PERSISTENT>
-__device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t i, double& t, double& tf,
-                                         Vec3& x, Vec3& v) {
-  while (i < a.n) {
+__device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t slot, double& t,
+                                         double& tf, Vec3& x, Vec3& v) {
+  while (slot < a.n) {
+    const int64_t i = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
     if (a.status[i] == ST_OK) {
       t = a.jd0[i];
       tf = a.jd_final[a.jd_final_stride ? i : 0];
@@ -215,9 +218,84 @@ __device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t i, double& 
       return i;
     }
     if (!PERSISTENT) return a.n;
-    i = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
+    slot = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
   }
-  return i;
+  return a.n;
+}
+
+// ---------------------------------------------------------------------------
+// Longest-first order for the persistent queue.  Each lane handles only a few
+// objects, so the slowest objects bound the launch; handing them out first
+// lets the rest fill in around them, and lanes of a wavefront then carry
+// objects of similar cost.  The cost key is |span| times the mean motion
+// a^-1.5 (vis-viva on the input state; unbound orbits count as fast), binned
+// on a log scale (eighth-octave buckets) and counting-sorted, highest first.
+// ---------------------------------------------------------------------------
+constexpr int ORDER_BUCKETS = 1024;
+
+__device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
+  const double t = a.jd0[i], tf = a.jd_final[a.jd_final_stride ? i : 0];
+  const Vec3 x{a.xyz[0][i], a.xyz[1][i], a.xyz[2][i]};
+  const Vec3 v{a.xyz[3][i], a.xyz[4][i], a.xyz[5][i]};
+  const double inv_a = 2.0 / norm(x) - dot(v, v) * (1.0 / GMS);
+  const double rate = inv_a > 0.0 ? inv_a * sqrt(inv_a) : 100.0;
+  const double key = fabs(tf - t) * rate;
+  if (!(key > 0.0) || !isfinite(key)) return 0;
+  const int b = (int)floor(log2(key) * 8.0) + ORDER_BUCKETS / 2;
+  return b < 1 ? 1 : (b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b);
+}
+
+// Bucket of each object and the bucket histogram (privatised in LDS).
+__global__ void order_hist_kernel(NBodyArgs a, uint16_t* __restrict__ bkt,
+                                  uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[ORDER_BUCKETS];
+  for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x) lh[k] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) {
+    const int32_t b = order_bucket(a, i);
+    bkt[i] = (uint16_t)b;
+    atomicAdd(&lh[b], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x)
+    if (lh[k] != 0) atomicAdd(&hist[k], lh[k]);
+}
+
+__global__ void order_zero_kernel(uint32_t* __restrict__ hist) {
+  for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x) hist[k] = 0;
+}
+
+// Start of each bucket in the order, highest bucket first (one thread: 1024
+// entries).
+__global__ void order_scan_kernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t run = 0;
+  for (int b = ORDER_BUCKETS - 1; b >= 0; --b) {
+    off[b] = run;
+    run += hist[b];
+  }
+}
+
+// perm[position] = object; positions within a bucket are claimed per block.
+__global__ void order_scatter_kernel(int64_t n, const uint16_t* __restrict__ bkt,
+                                     uint32_t* __restrict__ off, uint32_t* __restrict__ perm) {
+  __shared__ uint32_t cnt[ORDER_BUCKETS];
+  __shared__ uint32_t base[ORDER_BUCKETS];
+  for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x) cnt[k] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t b = 0;
+  uint32_t rank = 0;
+  if (i < n) {
+    b = bkt[i];
+    rank = atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x)
+    if (cnt[k] != 0) base[k] = atomicAdd(&off[k], cnt[k]);
+  __syncthreads();
+  if (i < n) perm[base[b] + rank] = (uint32_t)i;
 }
 
 // y = x^(1/7) for the step-size ratio (radau.rs:339-342), by two Newton steps
@@ -505,9 +583,10 @@ __device__ __forceinline__ int slot_body(int c, int k) {
 }
 
 template <bool PERSISTENT>
-__device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t i, int c, int base,
+__device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t slot, int c, int base,
                                           double& t, double& tf, double& x, double& v) {
-  while (i < a.n) {
+  while (slot < a.n) {
+    const int64_t i = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
     if (a.status[i] == ST_OK) {
       t = a.jd0[i];
       tf = a.jd_final[a.jd_final_stride ? i : 0];
@@ -520,9 +599,9 @@ __device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t i, int c, 
     if (c == 0) k = atomicAdd(a.queue, 1ull);
     const int lo = gshfl((int)(k & 0xffffffffull), base);
     const int hi = gshfl((int)(k >> 32), base);
-    i = a.first_dynamic + (int64_t)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    slot = a.first_dynamic + (int64_t)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
   }
-  return i;
+  return a.n;
 }
 
 template <bool PERSISTENT>
@@ -779,12 +858,25 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.counters = p.counters;
   a.queue = p.queue;
   a.range = p.range;
+  a.perm = nullptr;
   a.window = p.range != nullptr ? p.window : 0;
   const size_t lds_bytes = a.window > 0 ? (size_t)p.n_seg * a.window * 44 * sizeof(double) : 0;
   const int block = p.block > 0 ? p.block : 256;
   const unsigned g1 = (unsigned)((p.n + 255) / 256);
   if (a.range) hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range);
   hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
+  if (p.order_buf != nullptr && (p.mode == 1 || p.mode == 2) && p.n < (int64_t)1 << 31) {
+    // longest-first order of the persistent queue
+    uint32_t* hist = static_cast<uint32_t*>(p.order_buf);
+    uint32_t* off = hist + ORDER_BUCKETS;
+    uint32_t* perm = off + ORDER_BUCKETS;
+    uint16_t* bkt = reinterpret_cast<uint16_t*>(perm + p.n);
+    hipLaunchKernelGGL(order_zero_kernel, dim3(1), dim3(256), 0, stream, hist);
+    hipLaunchKernelGGL(order_hist_kernel, dim3(g1), dim3(256), 0, stream, a, bkt, hist);
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off, perm);
+    a.perm = perm;
+  }
   int64_t grid;
   if (p.mode == 2 || p.mode == 3) {
     // three lanes per object, 21 objects per wavefront
