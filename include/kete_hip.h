@@ -123,7 +123,8 @@ int32_t kh_propagate_two_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
  * mode: 0 plain launch (one lane per object), 1 persistent launch with a
  * device work queue, 2 persistent with three lanes per object (one per
  * coordinate), 3 plain with three lanes per object, -1 let the library
- * choose.
+ * choose (0).  Modes 0-2 hand out the objects longest first (|span| times
+ * mean motion), so the slowest arcs start at once.
  * ng, counters, impact_id, impact_jd are nullable. */
 int32_t kh_propagate_n_body(kh_ctx *ctx, int64_t n, const double *jd0,
                             const double *const xyz[6], int64_t center, int32_t frame,

@@ -437,7 +437,7 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.impact_jd = impact_jd;
   p.counters = reinterpret_cast<int32_t*>(counters);
   p.queue = c->d_queue;
-  p.mode = mode < 0 ? 1 : mode;
+  p.mode = mode < 0 ? 0 : mode;
   p.block = 256;
   p.grid = 0;
   // LDS window of 16 records (62 KB for 11 segments) when all segments have
@@ -450,11 +450,11 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
     p.window = atoi(w);
     if (p.window <= 0 || (size_t)c->spk.n_seg * p.window * 44 * 8 > 64 * 1024) p.range = nullptr;
   }
-  if (p.mode == 1 || p.mode == 2) {
-    // persistent: enough blocks to fill every CU at the kernel's occupancy
-    int per_cu = 2;
-    p.grid = c->n_cu * per_cu * 4;
-    // longest-first queue order (KH_ORDER=0 turns it off)
+  p.n_cu = c->n_cu;
+  if (const char* g = getenv("KH_GRID")) p.grid = atoi(g);
+  if (p.mode != 3) {
+    // longest-first order of the objects (one lane per object); KH_ORDER=0
+    // turns it off
     const char* ord = getenv("KH_ORDER");
     if (!(ord && atoi(ord) == 0)) {
       if (kh_grow(&c->obuf, &c->obuf_size, order_buf_bytes(n)) == KH_OK) p.order_buf = c->obuf;

@@ -235,6 +235,13 @@ __device__ __forceinline__ int32_t seg_eval(const SpkDev* __restrict__ spk,
     const double x = (js - rec[0]) * inv_radius;
     if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
     else p = cheb_pos_n<14>(rec + 2, x);
+  } else if (all14) {
+    // per-lane records, fixed length: unrolled, so all loads issue up front
+    const double* __restrict__ rec = coef + g.off + (int64_t)ri * 44;
+    inv_radius = 1.0 / rec[1];
+    const double x = (js - rec[0]) * inv_radius;
+    if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
+    else p = cheb_pos_n<14>(rec + 2, x);
   } else {
     const double* __restrict__ rec = coef + g.off + (int64_t)ri * g.rsize;
     inv_radius = 1.0 / rec[1];

@@ -189,7 +189,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       HIPCHK(hipMemcpyAsync(d_scalar, &jd_target, 8, hipMemcpyHostToDevice, s));
       const double* xin[6] = {x[0], x[1], x[2], x[3], x[4], x[5]};
       int32_t rr = kh_propagate_n_body_dev(c, n, xjd, xin, 10, frame, d_scalar, 0, nullptr, 1e-6,
-                                           1, x, st, nullptr, nullptr, nullptr, 0);
+                                           -1, x, st, nullptr, nullptr, nullptr, 0);
       if (rr) return rr;
       hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, xjd, n, jd_target);
       return KH_OK;
@@ -291,7 +291,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
         for (int k = 0; k < 7; ++k) nb.hit_state[k] = h_state[k];
         if (launch_fov_nb_gather(nb, s)) return kh_fail(KH_E_HIP, "gather launch failed");
         const double* gx[6] = {g_x[0], g_x[1], g_x[2], g_x[3], g_x[4], g_x[5]};
-        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, nullptr, 1e-6, 1,
+        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, nullptr, 1e-6, -1,
                                     g_out, g_st, nullptr, nullptr, nullptr, 0);
         if (r) return r;
         if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");

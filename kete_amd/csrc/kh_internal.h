@@ -48,6 +48,7 @@ struct NBodyLaunch {
   int32_t window;             // records per segment staged in LDS
   int32_t n_seg;
   void* order_buf;  // scratch for the longest-first queue order (order_buf_bytes), or null
+  int32_t n_cu;     // compute units of the device (sizes the persistent grid)
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.

@@ -836,6 +836,19 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 
+// Blocks of `kernel` that are resident at once on the whole device: the
+// persistent launch must not exceed it, or the blocks beyond it start only
+// after the first ones have drained the queue.
+template <typename K>
+static int64_t resident_grid(K kernel, int block, size_t lds_bytes, int32_t n_cu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (int64_t)per_cu * (n_cu > 0 ? n_cu : 1);
+}
+
 int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
                      hipStream_t stream, int32_t* used_block, int32_t* used_grid,
                      int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1) {
@@ -865,7 +878,7 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   const unsigned g1 = (unsigned)((p.n + 255) / 256);
   if (a.range) hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range);
   hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
-  if (p.order_buf != nullptr && (p.mode == 1 || p.mode == 2) && p.n < (int64_t)1 << 31) {
+  if (p.order_buf != nullptr && p.mode != 3 && p.n < (int64_t)1 << 31) {
     // longest-first order of the persistent queue
     uint32_t* hist = static_cast<uint32_t*>(p.order_buf);
     uint32_t* off = hist + ORDER_BUCKETS;
@@ -883,7 +896,10 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     const int64_t waves_needed = (p.n + 20) / 21;
     const int64_t blocks_needed = (waves_needed * 64 + block - 1) / block;
     const bool persistent = p.mode == 2;
-    grid = persistent ? (p.grid > 0 ? p.grid : 1024) : blocks_needed;
+    grid = persistent ? (p.grid > 0 ? p.grid
+                                    : resident_grid(radau3_nbody_kernel<true>, block, lds_bytes,
+                                                    p.n_cu))
+                      : blocks_needed;
     if (grid > blocks_needed) grid = blocks_needed;
     a.first_dynamic = grid * (block / 64) * 21;
     if (persistent) hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
@@ -896,7 +912,8 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
                          lds_bytes, stream, spk, coef, a);
     if (ev1) (void)hipEventRecord(ev1, stream);
   } else if (p.mode == 1) {
-    grid = p.grid > 0 ? p.grid : 1024;
+    grid = p.grid > 0 ? p.grid
+                      : resident_grid(radau_nbody_kernel<true>, block, lds_bytes, p.n_cu);
     if (grid * block > p.n) grid = (p.n + block - 1) / block;
     a.first_dynamic = grid * block;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);

@@ -35,9 +35,11 @@ def run_both(ctx, oracle_spk, xyz, jd_final, mode=-1, frame=ka.Frames.Ecliptic, 
     return g, r
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_parity_one_year(ctx, oracle_spk, mix, mode):
-    """mode 1: one lane per object; mode 2: three lanes per object."""
+    """mode 0: plain launch (the default), 1: persistent queue, both one lane
+    per object; mode 2: persistent, three lanes per object.  All hand out the
+    objects longest first."""
     g, r = run_both(ctx, oracle_spk, mix, JD0 + 365.25, mode=mode)
     assert np.array_equal(g["status"], r["status"])
     assert np.all(g["status"] == 0)
@@ -84,7 +86,7 @@ def test_uneven_spans_parity(ctx, oracle_spk):
     xyz = syn.n_body_workload(n=400, neo_fraction=0.1, seed_index=3)
     span = syn.uneven_spans(400) / 4.0  # keep the oracle fast: up to 912 d
     jf = JD0 + span
-    g, r = run_both(ctx, oracle_spk, xyz, jf, mode=1)
+    g, r = run_both(ctx, oracle_spk, xyz, jf)
     assert np.array_equal(g["status"], r["status"])
     ok = g["status"] == 0
     dp = np.abs(g["out"][:3, ok] - r["out"][:3, ok]).max()
