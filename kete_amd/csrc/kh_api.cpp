@@ -85,10 +85,14 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
   }
   c->n_cu = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->d_counts, 4 * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&c->d_range, 2 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&c->d_range, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_split, sizeof(int64_t)) != hipSuccess) {
     delete c;
     return fail(KH_E_HIP, "stream/event/queue allocation failed");
   }
@@ -114,8 +118,12 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);
+  if (c->d_split) (void)hipFree(c->d_split);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return KH_OK;
@@ -452,6 +460,15 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   }
   p.n_cu = c->n_cu;
   if (const char* g = getenv("KH_GRID")) p.grid = atoi(g);
+  // plain launch on the shared record grid: up to 8 x n_cu of the slowest
+  // objects run in the latency kernel beside the main launch, those whose
+  // cost outweighs a lane's share of the batch (KH_LATENCY_K sets the cap)
+  p.latency_k = c->spk.shared_grid ? (int64_t)c->n_cu * 8 : 0;
+  if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;
+  p.split = c->d_split;
+  p.stream2 = c->stream2;
+  p.fork = c->fork;
+  p.join = c->join;
   if (p.mode != 3) {
     // longest-first order of the objects (one lane per object); KH_ORDER=0
     // turns it off

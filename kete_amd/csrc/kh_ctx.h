@@ -36,6 +36,9 @@ struct kh_ctx {
   unsigned long long* d_range = nullptr;  // epoch range of the last N-body batch [2]
   unsigned long long* d_counts = nullptr;  // FOV list counters [4]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream2 = nullptr;  // the N-body latency kernel runs here
+  int64_t* d_split = nullptr;     // head of the N-body order it takes
+  hipEvent_t fork = nullptr, join = nullptr;
   bool timed = false;
   int32_t last_block = 0, last_grid = 0, last_mode = 0;
   int n_cu = 0;

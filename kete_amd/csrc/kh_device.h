@@ -668,6 +668,84 @@ __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ 
   return ST_OK;
 }
 
+#ifndef ILP_BODIES
+#define ILP_BODIES 2  // bodies whose evaluations overlap in spk_accel_ilp
+#endif
+
+// spk_accel on the shared record grid, arranged for the shortest dependency
+// chain rather than the fewest registers (the kernel variant that carries the
+// slowest objects runs one wavefront per SIMD, so its latency is exposed):
+// the Chebyshev T_k and U_k are formed once for all bodies, and the bodies are
+// unrolled so their coefficient loads and sums overlap.  The arithmetic and
+// the order of the accumulation into the acceleration are those of the
+// per-body path, so the results are bitwise the same.  Coefficients are read
+// from HBM/L2 per lane (no LDS).
+__device__ __forceinline__ int32_t spk_accel_ilp(const SpkDev* __restrict__ spk,
+                                                 const double* __restrict__ coef, double t,
+                                                 Vec3 x, Vec3 v, bool exact,
+                                                 const NonGravDev* ng, Vec3& acc,
+                                                 int32_t& impact_id) {
+  const SegDev& g = spk->seg[spk->emb_seg];
+  if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
+  const double js = (t - 2451545.0) * 86400.0;
+  const int64_t roff = (int64_t)record_index(g, js) * 44;
+  const double* __restrict__ r0 = coef + g.off + roff;
+  const double ir = 1.0 / r0[1];
+  const double xs = (js - r0[0]) * ir;
+  const double vscale = 86400.0 * ir * INV_AU_KM;
+  // T_k and U_k (interpolation.rs:25-77), the recurrences of cheb_pos_n /
+  // cheb_both_n
+  const double x2 = 2.0 * xs;
+  double T[14], U[14];
+  T[0] = 1.0;
+  T[1] = xs;
+  U[0] = 1.0;
+  U[1] = x2;
+#pragma unroll
+  for (int k = 2; k < 14; ++k) {
+    T[k] = x2 * T[k - 1] - T[k - 2];
+    U[k] = x2 * U[k - 1] - U[k - 2];
+  }
+  auto pos = [&](const double* __restrict__ c) {
+    double vx = c[0] + c[1] * xs, vy = c[14] + c[15] * xs, vz = c[28] + c[29] * xs;
+#pragma unroll
+    for (int k = 2; k < 14; ++k) {
+      vx += c[k] * T[k];
+      vy += c[14 + k] * T[k];
+      vz += c[28 + k] * T[k];
+    }
+    return Vec3{vx * INV_AU_KM, vy * INV_AU_KM, vz * INV_AU_KM};
+  };
+  auto vel = [&](const double* __restrict__ c) {
+    double dx = c[1], dy = c[15], dz = c[29];
+#pragma unroll
+    for (int k = 2; k < 14; ++k) {
+      const double kk = (double)k;
+      dx += c[k] * U[k - 1] * kk;
+      dy += c[14 + k] * U[k - 1] * kk;
+      dz += c[28 + k] * U[k - 1] * kk;
+    }
+    return Vec3{dx * vscale, dy * vscale, dz * vscale};
+  };
+  const Vec3 emb = pos(r0 + 2);
+  Vec3 a{0.0, 0.0, 0.0};
+#pragma unroll ILP_BODIES
+  for (int b = 0; b < N_PLANETS; ++b) {
+    const BodyDev& bd = spk->body[b];
+    const double* __restrict__ c = coef + spk->seg[bd.seg].off + roff + 2;
+    Vec3 bp = pos(c), bv{0.0, 0.0, 0.0};
+    if (bd.kind == 1) bv = vel(c);
+    if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+    const int32_t st = add_body_at(bd, bp, bv, x, v, exact, ng, a);
+    if (st) {
+      impact_id = bd.id;
+      return st;
+    }
+  }
+  acc = a;
+  return ST_OK;
+}
+
 // spk_accel: acceleration of a massless body at barycentric equatorial
 // (x, v) and TDB JD t, summed over kete's PLANETS in its order.  The
 // Earth-Moon barycentre is evaluated once and shared by 399 and 301 (kete

@@ -49,6 +49,12 @@ struct NBodyLaunch {
   int32_t n_seg;
   void* order_buf;  // scratch for the longest-first queue order (order_buf_bytes), or null
   int32_t n_cu;     // compute units of the device (sizes the persistent grid)
+  // mode 0: the first latency_k objects of the order run in the latency
+  // kernel on stream2 (fork/join events order it with `stream`)
+  int64_t latency_k;
+  int64_t* split;  // device int64: head of the order the latency kernel takes
+  hipStream_t stream2;
+  hipEvent_t fork, join;
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.

@@ -71,7 +71,26 @@ struct NBodyArgs {
   unsigned long long* range;  // [2]: bits of min and max epoch over the batch
   int32_t window;             // records per segment staged in LDS (0: none)
   const uint32_t* perm;       // queue slot -> object (longest first), or null
+  int64_t slot_begin, slot_end;  // the queue slots this launch takes
+  // device-chosen split of the order (head: latency kernel, tail: the rest):
+  // with split != null, part 1 takes slots [0, *split), part 2 [*split, n)
+  const int64_t* split;
+  int32_t part;
 };
+
+// The slot range of this launch (uniform: read once per lane from the
+// device-chosen split when there is one).
+__device__ __forceinline__ void slot_range(NBodyArgs& a) {
+  if (a.split == nullptr) return;
+  const int64_t k = *a.split;
+  if (a.part == 1) {
+    a.slot_begin = 0;
+    a.slot_end = k;
+  } else {
+    a.slot_begin = k;
+    a.slot_end = a.n;
+  }
+}
 
 enum : int32_t { PH_INIT = 0, PH_SUB = 1, PH_EXACT = 2 };
 
@@ -208,7 +227,7 @@ __global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
 template <bool PERSISTENT>
 __device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t slot, double& t,
                                          double& tf, Vec3& x, Vec3& v) {
-  while (slot < a.n) {
+  while (slot < a.slot_end) {
     const int64_t i = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
     if (a.status[i] == ST_OK) {
       t = a.jd0[i];
@@ -267,14 +286,28 @@ __global__ void order_zero_kernel(uint32_t* __restrict__ hist) {
 }
 
 // Start of each bucket in the order, highest bucket first (one thread: 1024
-// entries).
-__global__ void order_scan_kernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ off) {
+// entries).  With `split`, also the head of the order that goes to the
+// latency kernel: the objects whose cost key exceeds the average key load of
+// one resident lane of the main launch (`lanes`), at most `cap` of them --
+// objects that would otherwise outlast everything else.
+__global__ void order_scan_kernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ off,
+                                  int64_t* __restrict__ split, double lanes, int64_t cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t run = 0;
+  double load = 0.0;
   for (int b = ORDER_BUCKETS - 1; b >= 0; --b) {
     off[b] = run;
     run += hist[b];
+    if (b > 0) load += (double)hist[b] * exp2((b - ORDER_BUCKETS / 2 + 0.5) / 8.0);
   }
+  if (split == nullptr) return;
+  const double per_lane = load / lanes;
+  int64_t k = 0;
+  for (int b = ORDER_BUCKETS - 1; b > 0; --b) {
+    if (exp2((b - ORDER_BUCKETS / 2) / 8.0) <= per_lane || k + (int64_t)hist[b] > cap) break;
+    k += hist[b];
+  }
+  *split = k;
 }
 
 // perm[position] = object; positions within a bucket are claimed per block.
@@ -320,11 +353,14 @@ struct LdsTables {
   double rdiff[8][8];
 };
 
-template <bool PERSISTENT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-radau_nbody_kernel(const SpkDev* __restrict__ spk,
-                                                          const double* __restrict__ coef,
-                                                          NBodyArgs a) {
+// ILP: the latency variant for the slowest objects -- one wavefront per block
+// and per SIMD, the whole register file, the force evaluation arranged for
+// the shortest dependency chain (spk_accel_ilp), no LDS window.
+template <bool PERSISTENT, bool ILP = false>
+__global__ void __launch_bounds__(ILP ? 64 : 256)
+__attribute__((amdgpu_waves_per_eu(ILP ? 1 : 2, ILP ? 1 : 8)))
+radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef,
+                   NBodyArgs a) {
   __shared__ LdsTables lds;
   for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
     const int r = k / 8, c = k % 8;
@@ -354,7 +390,8 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk,
   const int32_t winw = a.window;
 
   const double eps = a.epsilon;
-  int64_t obj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  slot_range(a);
+  int64_t obj = a.slot_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double t = 0.0, tf = 0.0, h = 0.0, h_next = 0.0;
   Vec3 x{0, 0, 0}, v{0, 0, 0}, a0{0, 0, 0};
   double B[7][3], G[7][3];
@@ -400,8 +437,13 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk,
     int32_t impact = 0;
     n_force++;
     const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
-    int32_t st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]},
-                           !sub, ngp, acc, impact, winw);
+    int32_t st;
+    if constexpr (ILP)
+      st = spk_accel_ilp(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]},
+                         !sub, ngp, acc, impact);
+    else
+      st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
+                     ngp, acc, impact, winw);
     const double ev[3] = {acc.x, acc.y, acc.z};
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
@@ -585,7 +627,7 @@ __device__ __forceinline__ int slot_body(int c, int k) {
 template <bool PERSISTENT>
 __device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t slot, int c, int base,
                                           double& t, double& tf, double& x, double& v) {
-  while (slot < a.n) {
+  while (slot < a.slot_end) {
     const int64_t i = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
     if (a.status[i] == ST_OK) {
       t = a.jd0[i];
@@ -640,7 +682,7 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
   const int base = 3 * gw;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const double eps = a.epsilon;
-  int64_t obj = gw < 21 ? wave * 21 + gw : a.n;
+  int64_t obj = gw < 21 ? a.slot_begin + wave * 21 + gw : a.n;
   double t = 0.0, tf = 0.0, h = 0.0, h_next = 0.0, x = 0.0, v = 0.0, a0 = 0.0;
   double B[7], G[7];
 #pragma unroll
@@ -872,6 +914,10 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.queue = p.queue;
   a.range = p.range;
   a.perm = nullptr;
+  a.slot_begin = 0;
+  a.slot_end = p.n;
+  a.split = nullptr;
+  a.part = 0;
   a.window = p.range != nullptr ? p.window : 0;
   const size_t lds_bytes = a.window > 0 ? (size_t)p.n_seg * a.window * 44 * sizeof(double) : 0;
   const int block = p.block > 0 ? p.block : 256;
@@ -886,7 +932,11 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     uint16_t* bkt = reinterpret_cast<uint16_t*>(perm + p.n);
     hipLaunchKernelGGL(order_zero_kernel, dim3(1), dim3(256), 0, stream, hist);
     hipLaunchKernelGGL(order_hist_kernel, dim3(g1), dim3(256), 0, stream, a, bkt, hist);
-    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off);
+    int64_t* split = p.mode == 0 && p.latency_k > 0 ? p.split : nullptr;
+    const double lanes = (double)resident_grid(radau_nbody_kernel<false>, block, lds_bytes,
+                                               p.n_cu) * block;
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off, split, lanes,
+                       p.latency_k);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off, perm);
     a.perm = perm;
   }
@@ -922,11 +972,31 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
                        stream, spk, coef, a);
     if (ev1) (void)hipEventRecord(ev1, stream);
   } else {
-    grid = (p.n + block - 1) / block;
     a.first_dynamic = p.n;
+    // the slowest objects (the head of the longest-first order) go to the
+    // latency variant on the second stream, concurrently with the rest
+    const bool hybrid = a.perm != nullptr && p.latency_k > 0 && p.split && p.stream2 &&
+                        p.fork && p.join;
     if (ev0) (void)hipEventRecord(ev0, stream);
+    if (hybrid) {
+      // up to latency_k slots; the kernel reads how many from *split
+      NBodyArgs al = a;
+      al.split = p.split;
+      al.part = 1;
+      al.window = 0;
+      const int64_t k_max = p.latency_k < p.n ? p.latency_k : p.n;
+      (void)hipEventRecord(p.fork, stream);
+      (void)hipStreamWaitEvent(p.stream2, p.fork, 0);
+      hipLaunchKernelGGL((radau_nbody_kernel<false, true>), dim3((unsigned)((k_max + 63) / 64)),
+                         dim3(64), 0, p.stream2, spk, coef, al);
+      (void)hipEventRecord(p.join, p.stream2);
+      a.split = p.split;
+      a.part = 2;
+    }
+    grid = (p.n + block - 1) / block;
     hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), lds_bytes,
                        stream, spk, coef, a);
+    if (hybrid) (void)hipStreamWaitEvent(stream, p.join, 0);
     if (ev1) (void)hipEventRecord(ev1, stream);
   }
   hipLaunchKernelGGL(nbody_finish_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
