@@ -415,12 +415,14 @@ static int32_t check_nbody_args(kh_ctx* c, int64_t n, const double* jd0,
   return KH_OK;
 }
 
-int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
-                                const double* const xyz[6], int64_t center, int32_t frame,
-                                const double* jd_final, int32_t jd_final_stride,
-                                const kh_nongrav* ng, double epsilon, int32_t mode,
-                                double* const out[6], int32_t* status, int64_t* impact_id,
-                                double* impact_jd, kh_counters* counters, int32_t sync) {
+// The N-body launch behind kh_propagate_n_body_dev and the multi-epoch
+// propagation (me != null).
+static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],
+                         int64_t center, int32_t frame, const double* jd_final,
+                         int32_t jd_final_stride, const kh_nongrav* ng, double epsilon,
+                         int32_t mode, double* const out[6], int32_t* status, int64_t* impact_id,
+                         double* impact_jd, kh_counters* counters, int32_t sync,
+                         const kh::MultiEpochs* me) {
   Chain ch;
   int32_t r = check_nbody_args(c, n, jd0, xyz, center, frame, jd_final, epsilon, mode, out, status,
                                ch);
@@ -469,6 +471,14 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.stream2 = c->stream2;
   p.fork = c->fork;
   p.join = c->join;
+  if (me) {
+    p.mode = 0;
+    p.epochs = me->epochs;
+    p.n_epochs = me->n_epochs;
+    p.ep_lo = me->lo;
+    p.ep_hi = me->hi;
+    for (int k = 0; k < 6; ++k) p.mout[k] = me->out[k];
+  }
   if (p.mode != 3) {
     // longest-first order of the objects (one lane per object); KH_ORDER=0
     // turns it off
@@ -484,6 +494,16 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   c->timed = true;
   if (sync) HIPCHK(hipStreamSynchronize(c->stream));
   return KH_OK;
+}
+
+int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
+                                const double* const xyz[6], int64_t center, int32_t frame,
+                                const double* jd_final, int32_t jd_final_stride,
+                                const kh_nongrav* ng, double epsilon, int32_t mode,
+                                double* const out[6], int32_t* status, int64_t* impact_id,
+                                double* impact_jd, kh_counters* counters, int32_t sync) {
+  return nbody_dev(c, n, jd0, xyz, center, frame, jd_final, jd_final_stride, ng, epsilon, mode,
+                   out, status, impact_id, impact_jd, counters, sync, nullptr);
 }
 
 int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
@@ -567,3 +587,15 @@ double kh_last_kernel_ms(kh_ctx* c) {
 }
 
 }  // extern "C"
+
+int32_t kh_propagate_n_body_epochs_dev(kh_ctx* c, int64_t n, const double* jd0,
+                                       const double* const xyz[6], int64_t center, int32_t frame,
+                                       const kh::MultiEpochs& me, double epsilon,
+                                       double* const out[6], int32_t* status, int32_t sync) {
+  if (me.n_epochs <= 0 || !me.epochs) return fail(KH_E_ARG, "no epochs");
+  for (int k = 0; k < 6; ++k)
+    if (!me.out[k]) return fail(KH_E_ARG, "null epoch output column");
+  return nbody_dev(c, n, jd0, xyz, center, frame, me.epochs, 0, nullptr, epsilon, 0, out, status,
+                   nullptr, nullptr, nullptr, sync, &me);
+}
+

@@ -48,3 +48,12 @@ extern "C" {
 bool kh_build_chain(const kh_ctx* c, int64_t id, kh::Chain& ch);
 int32_t kh_grow(void** buf, size_t* size, size_t bytes);
 }
+
+// Multi-epoch N-body propagation on the device (the FOV driver's states at
+// its group epochs): every object from its jd0 through me.epochs in order,
+// one integration without restarts; me.out holds the states in the input
+// center and frame, NaN from a failing epoch on; out/status the final state.
+int32_t kh_propagate_n_body_epochs_dev(kh_ctx* c, int64_t n, const double* jd0,
+                                       const double* const xyz[6], int64_t center, int32_t frame,
+                                       const kh::MultiEpochs& me, double epsilon,
+                                       double* const out[6], int32_t* status, int32_t sync);

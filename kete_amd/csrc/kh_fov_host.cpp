@@ -127,6 +127,36 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
   if (max_group > 256)
     return kh_fail(KH_E_ARG, "more than 256 fields within 2*dt_limit of each other");
 
+  // The states each chunk is checked against.  kete re-propagates all states
+  // at each chunk (small steps, checks.rs:74-85) and every 50 dt_limit from a
+  // second copy (big steps, :63-72), restarting the integrator each time.  By
+  // default here the states are integrated once through the same small-step
+  // epochs (kh_propagate_n_body_epochs_dev): the same states to the
+  // integrator's error level (~1e-12 AU), far below the edge tolerance of the
+  // visible sets, at a fraction of the work.  KH_FOV_SCHEDULE=kete runs
+  // kete's schedule with restarts.
+  const char* sched = getenv("KH_FOV_SCHEDULE");
+  const bool continuous = !(sched && std::string(sched) == "kete");
+  std::vector<double> epochs;             // distinct small-step epochs, in order
+  std::vector<int64_t> chunk_epoch(chunks.size());
+  {
+    double jd_s = jd0;
+    for (size_t k = 0; k < chunks.size(); ++k) {
+      const double jd_mean =
+          (fields[chunks[k].second - 1].jd + fields[chunks[k].first].jd) / 2.0;
+      if (std::fabs(jd_mean - jd_s) >= dt_limit) {
+        jd_s = jd_mean;
+        epochs.push_back(jd_s);
+      }
+      chunk_epoch[k] = (int64_t)epochs.size() - 1;  // -1: still at jd0
+    }
+  }
+  // epochs per multi-epoch launch: at most ~4 GiB of stored states
+  const int64_t win_epochs =
+      continuous ? std::max<int64_t>(1, std::min<int64_t>((int64_t)epochs.size(),
+                                                          (4ll << 30) / (48 * n)))
+                 : 0;
+
   // device memory: states (cur, big) with per-object epochs, fields, lists
   int64_t cand_cap = std::max<int64_t>(1 << 20, 8 * n);
   int64_t nb_cap = std::max<int64_t>(1 << 16, n / 4);
@@ -134,10 +164,12 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
   std::vector<std::pair<int64_t, int64_t>> chunk_hits;  // (first sorted field, hits so far)
   for (int attempt = 0; attempt < 3; ++attempt) {
     const size_t bn = al(n * 8), bs = al(n * 4);
+    const size_t bw = al((size_t)win_epochs * n * 8);
     const size_t need = 2 * (7 * bn) + bs + al(n_fields * sizeof(FieldDev)) +
                         al(cand_cap * 4) * 2 + al(cand_cap * 8) + al(nb_cap * 4) +
                         al(nb_cap * 8) + al(nb_cap * 8) * 14 + al(nb_cap * 4) +
-                        al(hit_cap * 4) + al(hit_cap * 8) * 8 + al(8);
+                        al(hit_cap * 4) + al(hit_cap * 8) * 8 + al(8) +
+                        (continuous ? 6 * bw + bn + al(win_epochs * 8) + 6 * bn : 0);
     int32_t r = kh_grow(&c->fbuf, &c->fbuf_size, need);
     if (r) return r;
     char* q = (char*)c->fbuf;
@@ -171,6 +203,16 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
     double* h_state[7];
     for (int k = 0; k < 7; ++k) h_state[k] = (double*)take(al(hit_cap * 8));
     double* d_scalar = (double*)take(al(8));
+    double* m_out[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* ep_jd = nullptr;    // the epoch of the states a chunk uses, per object
+    double* d_epochs = nullptr;  // this window's epochs
+    double* fin[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (continuous) {
+      for (int k = 0; k < 6; ++k) m_out[k] = (double*)take(bw);
+      ep_jd = (double*)take(bn);
+      d_epochs = (double*)take(al(win_epochs * 8));
+      for (int k = 0; k < 6; ++k) fin[k] = (double*)take(bn);
+    }
 
     HIPCHK(hipMemcpyAsync(d_fields, fields.data(), n_fields * sizeof(FieldDev),
                           hipMemcpyHostToDevice, s));
@@ -197,33 +239,73 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
 
     double jd = jd0, big_epoch_ref = jd0;
     bool overflow = false;
-    for (auto& ch : chunks) {
+    int64_t w0 = -1, w1 = -1;  // epochs [w0, w1) are stored in m_out
+    for (size_t ci_ = 0; ci_ < chunks.size(); ++ci_) {
+      auto& ch = chunks[ci_];
       const int64_t f0 = ch.first, f1 = ch.second;
       const double jd_mean = (fields[f1 - 1].jd + fields[f0].jd) / 2.0;
-      // big steps (checks.rs:63-72): the big-step states move to the
-      // current small-step epoch
-      if (std::fabs(jd_mean - big_epoch_ref) >= dt_limit * 50.0) {
-        big_epoch_ref = jd_mean;
-        r = propagate(big, big_jd, jd);
-        if (r) return r;
-      }
-      // small steps (checks.rs:74-85)
-      if (std::fabs(jd_mean - jd) >= dt_limit) {
-        if (std::fabs(jd - big_epoch_ref) >= dt_limit * 25.0) {
-          for (int k = 0; k < 6; ++k)
-            HIPCHK(hipMemcpyAsync(cur[k], big[k], n * 8, hipMemcpyDeviceToDevice, s));
-          HIPCHK(hipMemcpyAsync(cur_jd, big_jd, n * 8, hipMemcpyDeviceToDevice, s));
+      double* sx[6];     // states this chunk is checked against
+      double* sjd = cur_jd;
+      for (int k = 0; k < 6; ++k) sx[k] = cur[k];
+      if (continuous) {
+        const int64_t e = chunk_epoch[ci_];
+        if (e >= 0) {
+          if (e >= w1) {
+            // next window of epochs: from the states at the last stored
+            // epoch (or the inputs), one launch through the window
+            if (w1 > 0) {
+              const int64_t last = (w1 - 1 - w0) * n;
+              for (int k = 0; k < 6; ++k)
+                HIPCHK(hipMemcpyAsync(cur[k], m_out[k] + last, n * 8, hipMemcpyDeviceToDevice,
+                                      s));
+              hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, cur_jd, n,
+                                 epochs[w1 - 1]);
+            }
+            w0 = e;
+            w1 = std::min<int64_t>((int64_t)epochs.size(), w0 + win_epochs);
+            HIPCHK(hipMemcpyAsync(d_epochs, epochs.data() + w0, (w1 - w0) * 8,
+                                  hipMemcpyHostToDevice, s));
+            MultiEpochs me{};
+            me.epochs = d_epochs;
+            me.n_epochs = (int32_t)(w1 - w0);
+            me.lo = *std::min_element(epochs.begin() + w0, epochs.begin() + w1);
+            me.hi = *std::max_element(epochs.begin() + w0, epochs.begin() + w1);
+            for (int k = 0; k < 6; ++k) me.out[k] = m_out[k];
+            const double* xin[6] = {cur[0], cur[1], cur[2], cur[3], cur[4], cur[5]};
+            r = kh_propagate_n_body_epochs_dev(c, n, cur_jd, xin, 10, frame, me, 1e-6, fin, st,
+                                               0);
+            if (r) return r;
+          }
+          for (int k = 0; k < 6; ++k) sx[k] = m_out[k] + (e - w0) * n;
+          hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, ep_jd, n, epochs[e]);
+          sjd = ep_jd;
         }
-        jd = jd_mean;
-        r = propagate(cur, cur_jd, jd);
-        if (r) return r;
+      } else {
+        // big steps (checks.rs:63-72): the big-step states move to the
+        // current small-step epoch
+        if (std::fabs(jd_mean - big_epoch_ref) >= dt_limit * 50.0) {
+          big_epoch_ref = jd_mean;
+          r = propagate(big, big_jd, jd);
+          if (r) return r;
+        }
+        // small steps (checks.rs:74-85)
+        if (std::fabs(jd_mean - jd) >= dt_limit) {
+          if (std::fabs(jd - big_epoch_ref) >= dt_limit * 25.0) {
+            for (int k = 0; k < 6; ++k)
+              HIPCHK(hipMemcpyAsync(cur[k], big[k], n * 8, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(cur_jd, big_jd, n * 8, hipMemcpyDeviceToDevice, s));
+          }
+          jd = jd_mean;
+          r = propagate(cur, cur_jd, jd);
+          if (r) return r;
+        }
       }
       // phase 1: prefilter over (object, field) pairs of this chunk
       hipLaunchKernelGGL(zero_list_counts, dim3(1), dim3(64), 0, s, c->d_counts);
       FovPrefilter pf{};
       pf.n = n;
-      pf.jd = cur_jd;
-      for (int k = 0; k < 6; ++k) pf.x[k] = cur[k];
+      pf.jd = sjd;
+      for (int k = 0; k < 6; ++k) pf.x[k] = sx[k];
       pf.fields = d_fields + f0;
       pf.nf = (int32_t)(f1 - f0);
       pf.dt_limit = dt_limit;
@@ -244,8 +326,8 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       // phase 2: two-body checks of the candidates
       FovTwoBody tb{};
       tb.n_cand = (int64_t)counts[0];
-      tb.jd = cur_jd;
-      for (int k = 0; k < 6; ++k) tb.x[k] = cur[k];
+      tb.jd = sjd;
+      for (int k = 0; k < 6; ++k) tb.x[k] = sx[k];
       tb.fields = d_fields + f0;
       tb.dt_limit = dt_limit;
       tb.cand_field = cand_field;
@@ -272,8 +354,8 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       if (counts[2] > 0) {
         FovNBody nb{};
         nb.m = (int64_t)counts[2];
-        nb.jd = cur_jd;
-        for (int k = 0; k < 6; ++k) nb.x[k] = cur[k];
+        nb.jd = sjd;
+        for (int k = 0; k < 6; ++k) nb.x[k] = sx[k];
         nb.fields = d_fields + f0;
         nb.nb_field = nb_field;
         nb.nb_obj = nb_obj;

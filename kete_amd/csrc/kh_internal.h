@@ -55,6 +55,22 @@ struct NBodyLaunch {
   int64_t* split;  // device int64: head of the order the latency kernel takes
   hipStream_t stream2;
   hipEvent_t fork, join;
+  // multi-epoch propagation (mode 0 only): n_epochs shared epochs in
+  // integration order, spanning [ep_lo, ep_hi]; states at each in mout
+  // [6][n_epochs][n], in the input center and frame
+  const double* epochs;
+  int32_t n_epochs;
+  double ep_lo, ep_hi;
+  double* mout[6];
+};
+
+// Shared epochs of a multi-epoch propagation (device array, integration
+// order) and the outputs [6][n_epochs][n].
+struct MultiEpochs {
+  const double* epochs;
+  int32_t n_epochs;
+  double lo, hi;  // smallest and largest epoch
+  double* out[6];
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.

@@ -76,6 +76,12 @@ struct NBodyArgs {
   // with split != null, part 1 takes slots [0, *split), part 2 [*split, n)
   const int64_t* split;
   int32_t part;
+  // multi-epoch propagation: every object is integrated through epochs[0..
+  // n_epochs) in order without restarting, and its SSB-equatorial state at
+  // each is stored in mout[6][n_epochs][n] (NaN from a failing epoch on)
+  const double* epochs;
+  int32_t n_epochs;
+  double* mout[6];
 };
 
 // The slot range of this launch (uniform: read once per lane from the
@@ -192,10 +198,12 @@ __global__ void nbody_prepare_kernel(const SpkDev* __restrict__ spk,
   }
 }
 
-__global__ void init_range(unsigned long long* r) {
+// Epoch range before the pre-pass folds in the objects' own epochs: empty,
+// or [lo, hi] (the epochs of a multi-epoch propagation).
+__global__ void init_range(unsigned long long* r, double lo, double hi) {
   if (threadIdx.x == 0) {
-    r[0] = (unsigned long long)__double_as_longlong(__builtin_inf());
-    r[1] = 0ull;
+    r[0] = (unsigned long long)__double_as_longlong(lo);
+    r[1] = (unsigned long long)__double_as_longlong(hi);
   }
 }
 
@@ -219,6 +227,29 @@ __global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
   a.out[3][i] = v.x;
   a.out[4][i] = v.y;
   a.out[5][i] = v.z;
+}
+
+// Post-pass of the multi-epoch propagation: each stored state back to the
+// input center and frame at its epoch, in place.  One lane per (epoch, object).
+__global__ void nbody_finish_multi_kernel(const SpkDev* __restrict__ spk,
+                                          const double* __restrict__ coef, NBodyArgs a) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)a.n_epochs * a.n) return;
+  const int64_t e = q / a.n;
+  Vec3 x{a.mout[0][q], a.mout[1][q], a.mout[2][q]};
+  Vec3 v{a.mout[3][q], a.mout[4][q], a.mout[5][q]};
+  if (!isfinite(x.x)) return;
+  const int32_t st = from_ssb_equatorial(spk, coef, a, a.epochs[e], x, v);
+  if (st != ST_OK) {
+    const double nan = __builtin_nan("");
+    x = v = Vec3{nan, nan, nan};
+  }
+  a.mout[0][q] = x.x;
+  a.mout[1][q] = x.y;
+  a.mout[2][q] = x.z;
+  a.mout[3][q] = v.x;
+  a.mout[4][q] = v.y;
+  a.mout[5][q] = v.z;
 }
 
 // Next object whose pre-pass succeeded, from queue slot `slot` on (the slot
@@ -356,7 +387,8 @@ struct LdsTables {
 // ILP: the latency variant for the slowest objects -- one wavefront per block
 // and per SIMD, the whole register file, the force evaluation arranged for
 // the shortest dependency chain (spk_accel_ilp), no LDS window.
-template <bool PERSISTENT, bool ILP = false>
+// MULTI: the multi-epoch propagation (plain launch only).
+template <bool PERSISTENT, bool ILP = false, bool MULTI = false>
 __global__ void __launch_bounds__(ILP ? 64 : 256)
 __attribute__((amdgpu_waves_per_eu(ILP ? 1 : 2, ILP ? 1 : 8)))
 radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef,
@@ -400,6 +432,10 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
   int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
   obj = fetch<PERSISTENT>(a, obj, t, tf, x, v);
+  int32_t ej = 0;                // MULTI: the epoch being integrated to
+  double h_want = 0.0;           // MULTI: the step proposed before shortening
+  bool shortened = false;        // MULTI: the step was shortened to land on tf
+  if (MULTI) tf = a.epochs[0];
 
   while (obj < a.n) {
     const bool sub = phase == PH_SUB;
@@ -562,8 +598,40 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         begin = true;
       }
     }
+    if (MULTI && done == ST_OK) {
+      // store the state at this epoch and carry on to the next one
+      for (;;) {
+        const int64_t q = (int64_t)ej * a.n + obj;
+        a.mout[0][q] = x.x;
+        a.mout[1][q] = x.y;
+        a.mout[2][q] = x.z;
+        a.mout[3][q] = v.x;
+        a.mout[4][q] = v.y;
+        a.mout[5][q] = v.z;
+        if (ej + 1 >= a.n_epochs || !(fabs(a.epochs[ej + 1] - t) < 1e-12)) break;
+        ++ej;
+      }
+      if (ej + 1 < a.n_epochs) {
+        ++ej;
+        tf = a.epochs[ej];
+        // the controller's proposal: before the last step was shortened to
+        // land on the epoch, else after it
+        const double hc = phase == PH_INIT ? 0.1 : (shortened ? h_want : h);
+        h = copysign(fmax(fabs(hc), 0.005), tf - t);
+        fails = 0;
+        done = -1;
+        begin = true;
+      }
+    } else if (MULTI && done > 0) {
+      // a failing epoch and the later ones are undefined
+      const double nan = __builtin_nan("");
+      for (int32_t e = ej; e < a.n_epochs; ++e)
+        for (int c = 0; c < 6; ++c) a.mout[c][(int64_t)e * a.n + obj] = nan;
+    }
     if (begin) {
-      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      if (MULTI) h_want = h;
+      shortened = fabs(t - tf) <= fabs(h);
+      if (shortened) h = tf - t;
       sweep = 0;
       j = 1;
       n_sweep++;
@@ -580,13 +648,14 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       Vec3 nx, nv;
       obj = fetch<PERSISTENT>(a, nxt, nt, ntf, nx, nv);
       t = nt;
-      tf = ntf;
+      tf = MULTI ? a.epochs[0] : ntf;
       x = nx;
       v = nv;
       phase = PH_INIT;
       j = 1;
       sweep = 0;
       fails = 0;
+      ej = 0;
       n_force = n_sweep = n_acc = n_fail = 0;
     }
     const bool fresh = done >= 0;
@@ -922,7 +991,13 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   const size_t lds_bytes = a.window > 0 ? (size_t)p.n_seg * a.window * 44 * sizeof(double) : 0;
   const int block = p.block > 0 ? p.block : 256;
   const unsigned g1 = (unsigned)((p.n + 255) / 256);
-  if (a.range) hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range);
+  a.epochs = p.epochs;
+  a.n_epochs = p.n_epochs;
+  for (int k = 0; k < 6; ++k) a.mout[k] = p.mout[k];
+  const bool multi = p.epochs != nullptr && p.n_epochs > 0;
+  if (a.range)
+    hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range,
+                       multi ? p.ep_lo : __builtin_inf(), multi ? p.ep_hi : 0.0);
   hipLaunchKernelGGL(nbody_prepare_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
   if (p.order_buf != nullptr && p.mode != 3 && p.n < (int64_t)1 << 31) {
     // longest-first order of the persistent queue
@@ -975,8 +1050,8 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     a.first_dynamic = p.n;
     // the slowest objects (the head of the longest-first order) go to the
     // latency variant on the second stream, concurrently with the rest
-    const bool hybrid = a.perm != nullptr && p.latency_k > 0 && p.split && p.stream2 &&
-                        p.fork && p.join;
+    const bool hybrid = !multi && a.perm != nullptr && p.latency_k > 0 && p.split &&
+                        p.stream2 && p.fork && p.join;
     if (ev0) (void)hipEventRecord(ev0, stream);
     if (hybrid) {
       // up to latency_k slots; the kernel reads how many from *split
@@ -994,12 +1069,21 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       a.part = 2;
     }
     grid = (p.n + block - 1) / block;
-    hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block), lds_bytes,
-                       stream, spk, coef, a);
+    if (multi)
+      hipLaunchKernelGGL((radau_nbody_kernel<false, false, true>), dim3((unsigned)grid),
+                         dim3(block), lds_bytes, stream, spk, coef, a);
+    else
+      hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block),
+                         lds_bytes, stream, spk, coef, a);
     if (hybrid) (void)hipStreamWaitEvent(stream, p.join, 0);
     if (ev1) (void)hipEventRecord(ev1, stream);
   }
   hipLaunchKernelGGL(nbody_finish_kernel, dim3(g1), dim3(256), 0, stream, spk, coef, a);
+  if (multi) {
+    const int64_t nq = (int64_t)p.n_epochs * p.n;
+    hipLaunchKernelGGL(nbody_finish_multi_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256),
+                       0, stream, spk, coef, a);
+  }
   if (used_block) *used_block = block;
   if (used_grid) *used_grid = (int32_t)grid;
   if (used_mode) *used_mode = p.mode;

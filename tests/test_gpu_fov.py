@@ -60,6 +60,36 @@ def test_fov_parity(ctx, oracle_spk, workload):
     assert order == list(range(len(keys)))
 
 
+@pytest.mark.parametrize("schedule", ["continuous", "kete"])
+def test_fov_parity_long_span(ctx, oracle_spk, monkeypatch, schedule):
+    """400 days of fields: many small steps and kete's big-step resets
+    (checks.rs:63-85).  'continuous' (the default) integrates the states once
+    through the small-step epochs; 'kete' restarts at every step as kete
+    does.  Both must give kete's visible sets up to the edge tolerance."""
+    if schedule == "kete":
+        monkeypatch.setenv("KH_FOV_SCHEDULE", "kete")
+    xyz, fjd, obs, pts, h = syn.fov_workload(n=800, n_fields=40, span_days=400.0, aimed=0.8,
+                                             seed_index=14)
+    import oracle
+    w = math.radians(6.9)
+    nrm = np.array([oracle.rect_normals(p, 0.0, w, w).reshape(12) for p in pts])
+    batch = ka.StateArray(np.full(xyz.shape[1], syn.EPOCH_JD), xyz)
+    hits = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    of, oo, os_ = oracle_spk.fov_state_check(syn.EPOCH_JD, xyz, fjd, obs.T.copy(), nrm,
+                                             dt_limit=3.0)
+    assert len(of) > 20, "workload should produce detections"
+    g = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(hits.field, hits.obj))}
+    r = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(of, oo))}
+    for key in set(g) ^ set(r):
+        f, i = key
+        st = os_[r[key]] if key in r else hits.state[:, g[key]]
+        assert edge_distance(nrm[f], obs[:, f], st) < 1e-9, key
+    common = sorted(set(g) & set(r))
+    gs = np.array([hits.state[:, g[k]] for k in common])
+    rs = np.array([os_[r[k]] for k in common])
+    assert np.abs(gs[:, 1:4] - rs[:, 1:4]).max() < 1e-9
+
+
 def test_radec_vmag_parity(ctx, oracle_spk, workload):
     xyz, fjd, obs, nrm, h = workload
     n = xyz.shape[1]
