@@ -472,12 +472,34 @@ def run_fov(args):
     t0 = time.perf_counter()
     hits = fov_state_check(batch, None, fields=(fjd, obs, nrm))
     dt = time.perf_counter() - t0
-    return {"metric": "fov (state x field) pairs/s", "value": n * nf / dt, "unit": "pairs/s",
+    line = {"metric": "fov (state x field) pairs/s", "value": n * nf / dt, "unit": "pairs/s",
             "n_gpus": 1, "steps": 1, "ms_per_step": dt * 1000, "higher_is_better": True,
             "dtype": "f64", "data": "synthetic",
             "config": {"workload": "fov_rect_6.9deg_1e6_states_x_3000_fields", "states": n,
-                       "fields": nf, "dt_limit": 3.0},
+                       "fields": nf, "dt_limit": 3.0,
+                       "schedule": os.environ.get("KH_FOV_SCHEDULE", "continuous")},
             "hits": len(hits)}
+    if not args.no_cpu:
+        # kete's schedule on the CPU restatement, a bounded sample of the
+        # states against all fields
+        import oracle
+        oracle.spk_load(syn.ephemeris_path())
+        m = min(n, 4000)
+        idx = np.linspace(0, n - 1, m).astype(np.int64)
+        tc = time.perf_counter()
+        of, oo, _ = oracle.fov_state_check(syn.EPOCH_JD, np.ascontiguousarray(xyz[:, idx]), fjd,
+                                           obs.T.copy(), nrm, dt_limit=3.0,
+                                           n_threads=cpu_threads())
+        dtc = time.perf_counter() - tc
+        sel = np.isin(hits.obj, idx)
+        line["cpu_baseline"] = {"value": m * nf / dtc, "unit": "pairs/s", "cores": cpu_threads(),
+                                "kind": "port",
+                                "sample": f"{m} evenly spaced states x {nf} fields, "
+                                          f"{dtc:.1f} s wall on {cpu_threads()} threads"}
+        line["parity"] = {"sample_hits_gpu": int(sel.sum()), "sample_hits_cpu": int(len(of))}
+        log(f"[bench] cpu {m * nf / dtc:.3e} pairs/s; sample hits gpu {int(sel.sum())} "
+            f"cpu {len(of)}")
+    return line
 
 
 def run_copy_calibration(args):
