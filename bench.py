@@ -418,25 +418,43 @@ def run_two_body(args):
     d_t = torch.from_numpy(targets).to(dev)
     d_out = torch.empty((6, m, n), dtype=torch.float64, device=dev)
     d_st = torch.empty((m, n), dtype=torch.int32, device=dev)
+    d_it = torch.zeros((m, n), dtype=torch.int32, device=dev)
     xyz_p = _lib.ptr_array([d_xyz[k].data_ptr() for k in range(6)])
     out_p = _lib.ptr_array([d_out[k].data_ptr() for k in range(6)])
 
-    def step():
+    def step(iters=False):
         _lib.check(ctx.L.kh_propagate_two_body_dev(
             ctx.h, n, _lib.raw(d_jd0.data_ptr()), xyz_p, 10, ka.Frames.Ecliptic, m,
-            _lib.raw(d_t.data_ptr()), None, out_p, _lib.raw(d_st.data_ptr(), C.c_int32), 0))
+            _lib.raw(d_t.data_ptr()), None, out_p, _lib.raw(d_st.data_ptr(), C.c_int32),
+            _lib.raw(d_it.data_ptr(), C.c_int32) if iters else None, 0))
 
-    ms, k_ms = time_steps(ctx, step, args.steps, args.warmup)
+    step(iters=True)                     # Newton iteration counts, untimed
+    torch.cuda.synchronize()
+    n_newton = int(d_it.sum().item())
+    steps = max(args.steps, 20)          # a 2 ms launch: time enough of them
+    ms, k_ms = time_steps(ctx, step, steps, args.warmup)
     solves = n * m
-    # Newton iterations per solve from the oracle on a sample (same algorithm)
     out = d_out.cpu().numpy()
     bytes_out = solves * 48
+    flop = F_KEPLER0 * solves + F_KEPLER_IT * n_newton
+    achieved = flop / (k_ms / 1000.0) / 1e12
     line = {"metric": "two-body solves/s", "value": solves / (ms / 1000.0),
-            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": ms, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "universal_kepler_1e5_states_x_64_epochs", "states": n,
                        "epochs": m},
-            "kernel_ms": k_ms, "hbm_write_gbs": bytes_out / (k_ms / 1000.0) / 1e9}
+            "roofline": {"bound": "fp64-valu", "achieved": achieved,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                         "kernel": "two_body_kernel", "kernel_ms": k_ms,
+                         "flop_per_launch": flop,
+                         "transcendentals_per_launch": 7 * solves + 6 * n_newton,
+                         "newton_iterations_per_solve": n_newton / solves,
+                         "hbm_write_gbs": bytes_out / (k_ms / 1000.0) / 1e9}}
+    log(f"[bench] {ms:.3f} ms/step, kernel {k_ms:.3f} ms, {solves / (ms / 1000.0):.3e} "
+        f"solves/s, {n_newton / solves:.2f} Newton iterations per solve, "
+        f"{achieved:.2f} TFLOP/s")
     if not args.no_cpu:
         import oracle
         idx = np.linspace(0, n - 1, min(n, 20000)).astype(np.int64)

@@ -108,11 +108,13 @@ int32_t kh_propagate_two_body(kh_ctx *ctx, int64_t n, const double *jd0,
                               int64_t m, const double *jd_targets,
                               const double *observer_pos, double *const out[6],
                               int32_t *status);
+/* Device-pointer variant; newton_iters (nullable, [m][n]) receives the
+ * Newton iterations each solve took (both solves with an observer). */
 int32_t kh_propagate_two_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                                   const double *const xyz[6], int64_t center, int32_t frame,
                                   int64_t m, const double *jd_targets,
                                   const double *observer_pos, double *const out[6],
-                                  int32_t *status, int32_t sync);
+                                  int32_t *status, int32_t *newton_iters, int32_t sync);
 
 /* N-body propagation with the Gauss-Radau 15 integrator and kete's SPK force
  * model.  Replaces the loop body of propagation_n_body_spk_py

@@ -104,7 +104,7 @@ def load():
     tb = [V, I64, P, PP, I64, I32, I64, P, P, PP, C.POINTER(I32)]
     L.kh_propagate_two_body.argtypes = tb
     L.kh_propagate_two_body.restype = I32
-    L.kh_propagate_two_body_dev.argtypes = tb + [I32]
+    L.kh_propagate_two_body_dev.argtypes = tb + [C.POINTER(I32), I32]
     L.kh_propagate_two_body_dev.restype = I32
     nb = [V, I64, P, PP, I64, I32, P, I32, C.POINTER(KhNonGrav), D, I32, PP, C.POINTER(I32),
           C.POINTER(I64), P, C.POINTER(KhCounters)]

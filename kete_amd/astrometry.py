@@ -157,7 +157,7 @@ def radec_plain(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, epsilon=1e-6, ct
         ob = np.ascontiguousarray(obs[:, j])
         _lib.check(L.kh_propagate_two_body_dev(
             ctx.h, n, _lib.raw(t_vecs[j].data_ptr()), cols(new), 10, frame, 1,
-            _lib.raw(epochs[j:].data_ptr()), _lib.dptr(ob), cols(lt), i32(st_tb), 1))
+            _lib.raw(epochs[j:].data_ptr()), _lib.dptr(ob), cols(lt), i32(st_tb), None, 1))
         status = torch.where(status == 0, st_nb, status)
         status = torch.where(status == 0, st_tb, status)
         rho = lt[:3] - torch.as_tensor(ob, dtype=torch.float64, device=dev)[:, None]

@@ -332,7 +332,7 @@ int32_t kh_propagate_two_body_dev(kh_ctx* c, int64_t n, const double* jd0,
                                   const double* const xyz[6], int64_t center, int32_t frame,
                                   int64_t m, const double* jd_targets,
                                   const double* observer_pos, double* const out[6],
-                                  int32_t* status, int32_t sync) {
+                                  int32_t* status, int32_t* newton_iters, int32_t sync) {
   int32_t r = check_two_body_args(c, n, jd0, xyz, center, frame, m, jd_targets, out, status);
   if (r) return r;
   if (n == 0 || m == 0) return KH_OK;
@@ -353,6 +353,7 @@ int32_t kh_propagate_two_body_dev(kh_ctx* c, int64_t n, const double* jd0,
   p.sun_seg = c->has_spk ? c->spk.sun_seg : 0;
   p.frame = frame;
   p.status = status;
+  p.iters = newton_iters;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (launch_two_body(c->d_spk, c->d_coef, p, c->stream)) return fail(KH_E_HIP, "two-body launch failed");
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -388,7 +389,7 @@ int32_t kh_propagate_two_body(kh_ctx* c, int64_t n, const double* jd0,
     HIPCHK(hipMemcpyAsync((void*)d_xyz[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_t, jd_targets, m * 8, hipMemcpyHostToDevice, c->stream));
   r = kh_propagate_two_body_dev(c, n, d_jd0, d_xyz, center, frame, m, d_t, observer_pos, d_out,
-                                d_st, 0);
+                                d_st, nullptr, 0);
   if (r) return r;
   for (int k = 0; k < 6; ++k)
     HIPCHK(hipMemcpyAsync(out[k], d_out[k], n * m * 8, hipMemcpyDeviceToHost, c->stream));

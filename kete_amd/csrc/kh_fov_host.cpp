@@ -403,23 +403,24 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       for (int k = 0; k < 7; ++k)
         HIPCHK(hipMemcpy(hs.data() + k * total, h_state[k], total * 8, hipMemcpyDeviceToHost));
     }
-    // chunk-local field index -> sorted index
-    std::vector<int64_t> gfield(total);
-    size_t ci = 0;
-    for (size_t h = 0; h < total; ++h) {
-      while (ci < chunk_hits.size() && (int64_t)h >= chunk_hits[ci].second) ++ci;
-      gfield[h] = chunk_hits[ci].first + hf[h];
-    }
-    std::vector<int64_t> idx(total);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
-      return gfield[a] != gfield[b] ? gfield[a] < gfield[b] : ho[a] < ho[b];
-    });
-    for (size_t k = 0; k < total; ++k) {
-      const int64_t h = idx[k];
-      hit_field[k] = order[gfield[h]];
-      hit_obj[k] = ho[h];
-      for (int q = 0; q < 7; ++q) hit_state[q][k] = hs[q * total + h];
+    // Hits arrive grouped by chunk (chunks in field order); within a chunk,
+    // order them by (chunk-local field, object index) on packed keys.
+    std::vector<std::pair<uint64_t, uint32_t>> keys;
+    size_t k_out = 0, h0 = 0;
+    for (size_t ci = 0; ci < chunk_hits.size(); ++ci) {
+      const size_t h1 = (size_t)chunk_hits[ci].second;
+      keys.resize(h1 - h0);
+      for (size_t h = h0; h < h1; ++h)
+        keys[h - h0] = {((uint64_t)hf[h] << 40) | (uint64_t)ho[h], (uint32_t)(h - h0)};
+      std::sort(keys.begin(), keys.end());
+      for (const auto& kv : keys) {
+        const size_t h = h0 + kv.second;
+        hit_field[k_out] = order[chunk_hits[ci].first + hf[h]];
+        hit_obj[k_out] = ho[h];
+        for (int q = 0; q < 7; ++q) hit_state[q][k_out] = hs[q * total + h];
+        ++k_out;
+      }
+      h0 = h1;
     }
     return KH_OK;
   }

@@ -94,6 +94,7 @@ struct TwoBodyLaunch {
   int32_t frame;
   double* out[6];
   int32_t* status;
+  int32_t* iters;  // Newton iterations per solve [m][n], or null
 };
 
 int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaunch& p,

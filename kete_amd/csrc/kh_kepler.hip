@@ -40,15 +40,17 @@ __global__ void __launch_bounds__(256) two_body_kernel(const SpkDev* __restrict_
       v = Vec3{v.x + (-sv.x), v.y + (-sv.y), v.z + (-sv.z)};
     }
     Vec3 po, vo;
-    if (st == ST_OK) st = analytic_2_body(jd - jd0, x, v, po, vo);
+    int it0 = 0, it1 = 0;
+    if (st == ST_OK) st = analytic_2_body(jd - jd0, x, v, po, vo, &it0);
     if (st == ST_OK && p.has_observer) {
       Vec3 d{po.x - p.observer[0], po.y - p.observer[1], po.z - p.observer[2]};
       double delay = -norm(d) / C_AU_PER_DAY;
       Vec3 p2, v2;
-      st = analytic_2_body((jd + delay) - jd, po, vo, p2, v2);
+      st = analytic_2_body((jd + delay) - jd, po, vo, p2, v2, &it1);
       po = p2;
       vo = v2;
     }
+    if (p.iters) p.iters[q] = it0 + it1;
     if (st == ST_OK && !p.center_sun) {
       // Sun -> SSB at the new epoch (spk.rs:102-103)
       double jd_new = jd;

@@ -108,7 +108,10 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
         nrm = np.ascontiguousarray(nrm, dtype=np.float64).reshape(-1, 12)
         nf = len(fjd)
     n = len(obj_state)
-    cap = cap or max(1 << 16, n // 100 * max(nf, 1) // 10 + 4096)
+    # hit capacity: a call that overflows reports the size needed and is run
+    # again, so start above the usual density (a 6.9 deg field covers 1.2e-3
+    # of the sky)
+    cap = cap or int(min(n * max(nf, 1), max(1 << 16, 2e-3 * n * nf + 4096)))
     while True:
         hf = np.empty(cap, dtype=np.int64)
         ho = np.empty(cap, dtype=np.int64)
