@@ -198,8 +198,9 @@ __device__ __forceinline__ int32_t record_index(const SegDev& g, double js) {
 
 // A window of W consecutive records of every segment, staged in LDS by the
 // N-body kernel (all segments must then have 14 coefficients, i.e. 44-double
-// records): kh_win_rec is [n_seg][W][44], kh_win_w0 the first staged record
-// of each segment.  Declared at file scope so that reads compile to ds_read.
+// records): kh_win_rec is [n_seg][W][44] with word 1 (the record radius)
+// stored as its reciprocal, kh_win_w0 the first staged record of each
+// segment.  Declared at file scope so that reads compile to ds_read.
 extern __shared__ double kh_win_rec[];
 __shared__ int32_t kh_win_w0[MAX_SEGS];
 
@@ -225,7 +226,7 @@ __device__ __forceinline__ int32_t seg_eval(const SpkDev* __restrict__ spk,
   if (win > 0 && __all(loc >= 0 && loc < win)) {
     // every lane's record is in the LDS window
     const double* rec = kh_win_rec + (s * win + loc) * 44;
-    inv_radius = 1.0 / rec[1];
+    inv_radius = rec[1];  // staged as 1 / radius
     const double x = (js - rec[0]) * inv_radius;
     if (VEL) cheb_both_n<14>(rec + 2, x, p, d);
     else p = cheb_pos_n<14>(rec + 2, x);
@@ -771,7 +772,7 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
     if (win > 0 && __all(loc >= 0 && loc < win)) {
       const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
-      const double ir = 1.0 / r0[1];
+      const double ir = r0[1];  // staged as 1 / radius
       return accel_shared_grid<false>(spk, nullptr, loc, win, (js - r0[0]) * ir, ir, x, v,
                                       exact, ng, acc, impact_id);
     }

@@ -415,7 +415,9 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       const int sg = k / per_seg, rem = k - sg * per_seg;
       const SegDev& g = spk->seg[sg];
       const int r = kh_win_w0[sg] + rem / 44;
-      kh_win_rec[k] = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 0.0;
+      // word 1 of a record (the radius) is staged as its reciprocal
+      const double w = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 1.0;
+      kh_win_rec[k] = rem % 44 == 1 ? 1.0 / w : w;
     }
   }
   __syncthreads();
@@ -516,14 +518,17 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         B[c][d] = es ? acc2 : B[c][d];
       }
     }
-    double fmx[3], rmax = 0.0;
+    // max_d |dB6_d| / (|a_d| + 1e-6) < 1e-14, tested per coordinate as
+    // |dB6_d| < 1e-14 (|a_d| + 1e-6): no divides (the two forms can differ
+    // only for ratios within an ulp of the threshold)
+    double fmx[3];
+    bool small = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       fmx[d] = fabs(ev[d]) + 1e-6;
-      const double r = fabs(B[6][d] - bp6[d]) / fmx[d];
-      rmax = d == 0 ? r : fmax(rmax, r);
+      small = small && fabs(B[6][d] - bp6[d]) < 1e-14 * fmx[d];
     }
-    const bool conv = es && rmax < 1e-14;
+    const bool conv = es && small;
     // ---- accepted step (radau.rs:320-338) ----
     {
       double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
@@ -739,7 +744,9 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
       const int sg = k / per_seg, rem = k - sg * per_seg;
       const SegDev& g = spk->seg[sg];
       const int r = kh_win_w0[sg] + rem / 44;
-      kh_win_rec[k] = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 0.0;
+      // word 1 of a record (the radius) is staged as its reciprocal
+      const double w = r < g.n_rec ? coef[g.off + (int64_t)r * 44 + rem % 44] : 1.0;
+      kh_win_rec[k] = rem % 44 == 1 ? 1.0 / w : w;
     }
   }
   __syncthreads();
