@@ -1,9 +1,13 @@
-"""DAF/SPK Type 2 files: reading, writing, and the device table layout.
+"""DAF/SPK files with Type 2 and Type 21 segments: reading, writing, and the
+device table layout.
 
-Reading follows kete's DAF container reader (``spice/daf.rs:148-351``) and its
-Type 2 segment view (``spice/spk_segments.rs:316-397``): a 1024-byte file
-record, summary records chained by a forward pointer, 8-byte words addressed
-from 1, and a ``[INIT, INTLEN, RSIZE, N]`` trailer on each Type 2 array.
+Reading follows kete's DAF container reader (``spice/daf.rs:148-351``), its
+Type 2 segment view (``spice/spk_segments.rs:316-397``) and its Type 21 view
+(``spice/spk_segments.rs:958-1003``): a 1024-byte file record, summary
+records chained by a forward pointer, 8-byte words addressed from 1, a
+``[INIT, INTLEN, RSIZE, N]`` trailer on each Type 2 array, and on each Type 21
+array ``N`` records of ``4*MAXDIM+11`` doubles, the ``N`` final epochs, a
+directory of every 100th final epoch and a ``[MAXDIM, N]`` trailer.
 
 The device table (:class:`Type2Table`) packs the records of every Type 2
 segment one after another, ``[n_rec][rsize]`` per segment, so a kernel reads a
@@ -50,9 +54,12 @@ class Segment:
     def jd_end(self) -> float:
         return spice_jds_to_jd(self.end_s)
 
-    # Type 2 trailer (spk_segments.rs:379-397)
+    # Type 2 trailer [INIT, INTLEN, RSIZE, N] (spk_segments.rs:379-397);
+    # Type 21 trailer [MAXDIM, N] (spk_segments.rs:971-986)
     @property
     def rsize(self) -> int:
+        if self.seg_type == 21:
+            return 4 * self.n_coef + 11
         return int(self.data[-2])
 
     @property
@@ -65,10 +72,35 @@ class Segment:
 
     @property
     def n_coef(self) -> int:
+        """Chebyshev coefficients per axis (Type 2) or MAXDIM (Type 21)."""
+        if self.seg_type == 21:
+            return int(self.data[-2])
         return (self.rsize - 2) // 3
 
     def records(self) -> np.ndarray:
         return self.data[: self.n_rec * self.rsize].reshape(self.n_rec, self.rsize)
+
+    def epochs(self) -> np.ndarray:
+        """Type 21 final epochs, one per record (TDB seconds past J2000)."""
+        if self.seg_type != 21:
+            raise ValueError("final epochs exist for Type 21 segments only")
+        o = self.n_rec * self.rsize
+        return self.data[o: o + self.n_rec]
+
+
+def type21_array(records: np.ndarray, epochs: np.ndarray) -> np.ndarray:
+    """Type 21 segment array from ``[N][4*MAXDIM+11]`` records and their ``N``
+    final epochs: records, epochs, the directory of every 100th final epoch,
+    then ``[MAXDIM, N]`` (NAIF SPK Type 21 layout, as kete reads it in
+    ``spk_segments.rs:971-1003``)."""
+    records = np.ascontiguousarray(records, dtype=np.float64)
+    n, rl = records.shape
+    if (rl - 11) % 4 or len(epochs) != n:
+        raise ValueError("records must be [N][4*MAXDIM+11] with N final epochs")
+    epochs = np.asarray(epochs, dtype=np.float64)
+    directory = epochs[99::100][: n // 100]
+    return np.concatenate([records.reshape(-1), epochs, directory,
+                           np.array([(rl - 11) // 4, n], dtype=np.float64)])
 
 
 def read_spk(path: str) -> list[Segment]:
@@ -106,8 +138,9 @@ def read_spk(path: str) -> list[Segment]:
     return segs
 
 
-def write_spk_type2(path: str, segments: list[Segment], name: str = "SYNTHETIC") -> None:
-    """Write Type 2 segments as a little-endian DAF/SPK file (NAIF DAF layout)."""
+def write_spk(path: str, segments: list[Segment], name: str = "SYNTHETIC") -> None:
+    """Write segments (any type; ``data`` is the segment array) as a
+    little-endian DAF/SPK file (NAIF DAF layout), in the order given."""
     nd, ni = 2, 6
     ssize = nd + (ni + 1) // 2
     if len(segments) > (1024 - 24) // (8 * ssize):
@@ -141,6 +174,9 @@ def write_spk_type2(path: str, segments: list[Segment], name: str = "SYNTHETIC")
     pad = (-len(body)) % 1024
     with open(path, "wb") as f:
         f.write(bytes(rec1) + bytes(rec2) + bytes(rec3) + body + b"\0" * pad)
+
+
+write_spk_type2 = write_spk
 
 
 class Type2Table:

@@ -69,12 +69,18 @@ def lib():
         L.ko_spk_get_state.argtypes = [I64, D, I64, I32, P, P]
         L.ko_chebyshev_evaluate_both.argtypes = [D, P, P, P, C.c_int, P, P]
         L.ko_spk_accel.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, P, C.POINTER(I64)]
+        L.ko_spk_accel_ext.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, C.c_int, P,
+                                       C.POINTER(I64)]
+        L.ko_spk21_record_eval.argtypes = [P, I64, D, P, P]
         L.ko_radau_central.argtypes = [D, D, P, P, P, P, C.POINTER(Counters)]
         L.ko_batch_two_body.argtypes = [I64, P, PP, I64, I32, I64, P, P, PP,
                                         C.POINTER(I32), C.c_int]
         L.ko_batch_n_body.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.POINTER(NonGrav),
                                       PP, C.POINTER(I32), C.POINTER(I64), P,
                                       C.POINTER(Counters), C.c_int]
+        L.ko_batch_n_body_ext.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.c_int,
+                                          C.POINTER(NonGrav), PP, C.POINTER(I32),
+                                          C.POINTER(I64), P, C.POINTER(Counters), C.c_int]
         L.ko_rect_normals.argtypes = [P, D, D, D, P]
         L.ko_rect_contains.argtypes = [P, P, P]
         L.ko_fov_check_visible.restype = I64
@@ -126,9 +132,23 @@ def elements_to_state(epoch, ecc, incl, lon_asc, peri_arg, peri_dist, peri_time)
     return st, p, v
 
 
-def spk_load(path: str) -> int:
+def spk_load(*paths: str) -> int:
+    """Replace the loaded SPK collection by the files given, in load order."""
     lib().ko_spk_reset()
-    return lib().ko_spk_load_file(path.encode())
+    for path in paths:
+        st = lib().ko_spk_load_file(path.encode())
+        if st:
+            return st
+    return OK
+
+
+def spk21_record_eval(record, n_coef, js):
+    """One Type 21 record at js (TDB seconds past J2000) -> (status, pos AU, vel AU/day)."""
+    rec = np.ascontiguousarray(record, dtype=np.float64)
+    assert len(rec) == 4 * n_coef + 11
+    p, v = np.zeros(3), np.zeros(3)
+    st = lib().ko_spk21_record_eval(_d(rec), n_coef, js, _d(p), _d(v))
+    return st, p, v
 
 
 def spk_state(naif_id, jd, center=0, frame=FRAME_EQUATORIAL):
@@ -144,12 +164,13 @@ def chebyshev_both(x, cx, cy, cz):
     return st, val, der
 
 
-def spk_accel(t, pos, vel, exact=False, nongrav=None):
+def spk_accel(t, pos, vel, exact=False, nongrav=None, include_asteroids=False):
     p, v = _v3(pos), _v3(vel)
     a = np.zeros(3)
     iid = C.c_int64(0)
     ng = C.byref(nongrav) if nongrav is not None else None
-    st = lib().ko_spk_accel(t, _d(p), _d(v), ng, int(exact), _d(a), C.byref(iid))
+    st = lib().ko_spk_accel_ext(t, _d(p), _d(v), ng, int(exact), int(include_asteroids), _d(a),
+                                C.byref(iid))
     return st, a, iid.value
 
 
@@ -189,7 +210,7 @@ def batch_two_body(jd0, xyz, targets, center=10, frame=FRAME_ECLIPTIC,
 
 
 def batch_n_body(jd0, xyz, jd_final, center=10, frame=FRAME_ECLIPTIC, nongravs=None,
-                 n_threads=None):
+                 n_threads=None, include_asteroids=False):
     """propagate_n_body over n objects -> dict(out[6,n], status, impact_id, impact_jd, counters)."""
     xyz = _soa(xyz)
     n = xyz.shape[1]
@@ -206,11 +227,12 @@ def batch_n_body(jd0, xyz, jd_final, center=10, frame=FRAME_ECLIPTIC, nongravs=N
     ng = None
     if nongravs is not None:
         ng = (NonGrav * n)(*nongravs)
-    lib().ko_batch_n_body(n, _d(jd0), _ptrs(list(xyz)), center, frame, _d(jf), stride, ng,
-                          _ptrs([out[k] for k in range(6)]),
-                          status.ctypes.data_as(C.POINTER(C.c_int32)),
-                          iid.ctypes.data_as(C.POINTER(C.c_int64)), _d(ijd), cnt,
-                          n_threads or os.cpu_count())
+    lib().ko_batch_n_body_ext(n, _d(jd0), _ptrs(list(xyz)), center, frame, _d(jf), stride,
+                              int(include_asteroids), ng,
+                              _ptrs([out[k] for k in range(6)]),
+                              status.ctypes.data_as(C.POINTER(C.c_int32)),
+                              iid.ctypes.data_as(C.POINTER(C.c_int64)), _d(ijd), cnt,
+                              n_threads or os.cpu_count())
     counters = np.array([(c.n_force, c.n_sweep, c.n_acc, c.n_fail) for c in cnt[:n]],
                         dtype=np.int64).reshape(n, 4)
     return dict(out=out, status=status, impact_id=iid, impact_jd=ijd, counters=counters)

@@ -48,8 +48,12 @@ typedef struct {
     double radius;
 } grav_params;
 
-/* PLANETS = MASSES[0..10] (constants/gravity.rs:43-101,145-148). */
-static const grav_params PLANETS[10] = {
+/* MASSES (constants/gravity.rs:43-134): the Sun, the planets and the Moon
+ * (PLANETS = MASSES[0..10], gravity.rs:145-148), then Ceres, Pallas, Vesta,
+ * Hygiea and Interamnia. */
+#define N_PLANETS 10
+#define N_MASSES 15
+static const grav_params MASSES[N_MASSES] = {
     {10, GMS, 0.004654758765894654},
     {1, 1.66012082548908e-07 * GMS, 1.63139354095098e-05},
     {2, 2.44783828779694e-06 * GMS, 4.04537843465442e-05},
@@ -60,7 +64,13 @@ static const grav_params PLANETS[10] = {
     {6, 0.00028588567002459455 * GMS, 0.0004028666966848747},
     {7, 4.36624961322212e-05 * GMS, 0.0001708513622580592},
     {8, 5.15138377265457e-05 * GMS, 0.0001655371154958558},
+    {20000001, 4.7191659919436e-10 * GMS, 6.276827e-6},
+    {20000002, 1.0259143964958e-10 * GMS, 3.415824e-6},
+    {20000004, 1.3026452498655e-10 * GMS, 3.512082e-6},
+    {20000010, 4.3953391300849e-11 * GMS, 2.894426e-6},
+    {20000704, 1.911e-11 * GMS, 2.219283e-6},
 };
+#define PLANETS MASSES
 
 double ko_const(const char *name) {
     if (!strcmp(name, "GMS")) return GMS;
@@ -365,7 +375,8 @@ int ko_elements_to_state(double epoch, double ecc, double incl, double lon_asc,
 
 /* ------------------------------------------------------------------------ */
 /* spice: DAF reader (spice/daf.rs:148-351), SPK collection (spice/spk.rs),  */
-/* Type 2 segments (spice/spk_segments.rs:105-177,316-397).                  */
+/* Type 2 segments (spice/spk_segments.rs:105-177,316-397), Type 21         */
+/* segments (spice/spk_segments.rs:958-1102).                                */
 /* ------------------------------------------------------------------------ */
 #define MAX_SEGS 256
 typedef struct {
@@ -375,7 +386,7 @@ typedef struct {
     double jd_start, jd_end;
     double *data;
     int64_t n_data;
-    /* type 2 */
+    /* type 2: n_coef Chebyshev coefficients per axis; type 21: MAXDIM */
     double jd_step;
     int64_t n_coef, record_len, n_rec;
 } spk_seg;
@@ -466,6 +477,16 @@ int ko_spk_load_file(const char *path) {
                 g->n_coef = (g->record_len - 2) / 3;
                 g->n_rec = (int64_t)g->data[g->n_data - 1];
                 if (3 * g->n_coef + 2 != g->record_len) { rc = KO_IO_ERROR; goto done; }
+            } else if (g->seg_type == 21) {
+                /* From<DafArray> for SpkSegmentType21 (spk_segments.rs:971-986):
+                 * trailer [MAXDIM, N]; records of 4*MAXDIM+11 doubles, then the
+                 * N final epochs.  kete does not check the sizes; the oracle
+                 * refuses arrays too short for them. */
+                g->n_rec = (int64_t)g->data[g->n_data - 1];
+                g->n_coef = (int64_t)g->data[g->n_data - 2];
+                g->record_len = 4 * g->n_coef + 11;
+                if (g->n_rec < 1 || g->n_coef < 1 ||
+                    g->n_rec * (g->record_len + 1) + 2 > g->n_data) { rc = KO_IO_ERROR; goto done; }
             }
             if (g->obj_id >= 0 && g->obj_id <= 1000) PLANET_ORDER[N_PLANET++] = N_SEGS;
             else OTHER_ORDER[N_OTHER++] = N_SEGS;
@@ -522,11 +543,85 @@ static inline int seg_contains(const spk_seg *g, double jd) {
     return jd >= g->jd_start && jd <= g->jd_end;
 }
 
+/* SpkSegmentType21::try_get_pos_vel (spk_segments.rs:1007-1100) on one
+ * record: the modified-difference-array (MDA) evaluation of NAIF's SPKE21.
+ * rec: [t_ref, G[n], x vx y vy z vz, DT[3][n], KQMAX1, KQ[3]] in km, s.
+ * Position in AU, velocity in AU/day.  Returns KO_IO_ERROR for a record kete
+ * would reject (a zero step size) or could not index (orders outside
+ * 1..n, which kete does not check). */
+#define MDA_MAX 64
+int ko_spk21_record_eval(const double *rec, int64_t n, double js, double pos[3],
+                         double vel[3]) {
+    const double t_ref = rec[0];
+    const double *g = rec + 1;
+    const double *ref = rec + n + 1;
+    const double *dd = rec + n + 7;
+    const int64_t kqmax1 = (int64_t)rec[4 * n + 7];
+    int64_t kq[3];
+    for (int i = 0; i < 3; ++i) kq[i] = (int64_t)rec[4 * n + 8 + i];
+    if (n > MDA_MAX || kqmax1 < 2 || kqmax1 > n + 1) return KO_IO_ERROR;
+    for (int i = 0; i < 3; ++i)
+        if (kq[i] < 0 || kq[i] > kqmax1 - 1) return KO_IO_ERROR;
+    const double dt = js - t_ref;
+    double fc[MDA_MAX], wc[MDA_MAX], w[MDA_MAX + 1];
+    double tp = dt;
+    for (int64_t j = 0; j < kqmax1 - 2; ++j) {
+        if (g[j] == 0.0) return KO_IO_ERROR;
+        fc[j] = tp / g[j];
+        wc[j] = dt / g[j];
+        tp = dt + g[j];
+    }
+    for (int64_t j = 0; j < kqmax1; ++j) w[j] = 1.0 / ((double)j + 1.0);
+    /* integration passes KS = KQMAX1-1 .. 2 */
+    int64_t ks = kqmax1 - 1, jx = 0, ks1 = ks - 1;
+    while (ks >= 2) {
+        jx += 1;
+        for (int64_t j = 0; j < jx; ++j) w[j + ks] = fc[j] * w[j + ks1] - wc[j] * w[j + ks];
+        ks = ks1;
+        ks1 -= 1;
+    }
+    for (int i = 0; i < 3; ++i) {
+        double sum = 0.0;
+        for (int64_t j = kq[i]; j >= 1; --j) sum += dd[i * n + j - 1] * w[j + ks - 1];
+        pos[i] = (ref[2 * i] + dt * (sum * dt + ref[2 * i + 1])) / AU_KM;
+    }
+    /* one more pass (KS = 1) for the velocity */
+    for (int64_t j = 0; j < jx; ++j) w[j + ks] = fc[j] * w[j + ks1] - wc[j] * w[j + ks];
+    ks -= 1;
+    for (int i = 0; i < 3; ++i) {
+        double sum = 0.0;
+        for (int64_t j = kq[i]; j >= 1; --j) sum += dd[i * n + j - 1] * w[j + ks - 1];
+        vel[i] = (ref[2 * i + 1] + dt * sum) / AU_KM * 86400.0;
+    }
+    return KO_OK;
+}
+
+/* Record selection of SpkSegmentType21::try_get_pos_vel
+ * (spk_segments.rs:1023-1030): the first record whose final epoch is >= the
+ * time (slice::binary_search_by over sorted epochs).  A time after the last
+ * final epoch -- reachable at jd == jd_end through the JD <-> seconds round
+ * trip -- is clamped to the last record, where kete reads past the array. */
+static int64_t t21_record_index(const spk_seg *g, double js) {
+    const double *ep = g->data + g->n_rec * g->record_len;
+    int64_t lo = 0, hi = g->n_rec;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (ep[mid] < js) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < g->n_rec ? lo : g->n_rec - 1;
+}
+
 /* SpkSegment::try_get_state + SpkSegmentType2::try_get_pos_vel
  * (spk_segments.rs:150-177,355-376).  The record index is clamped to the
  * last record: kete reads one record past the end at jd == jd_end. */
 static int seg_get_state(const spk_seg *g, double jd, double *pos, double *vel) {
     if (jd < g->jd_start || jd > g->jd_end) return KO_DAF_LIMITS;
+    if (g->seg_type == 21) {
+        double js = jd_to_spice_jd(jd);
+        int64_t ri = t21_record_index(g, js);
+        return ko_spk21_record_eval(g->data + ri * g->record_len, g->n_coef, js, pos, vel);
+    }
     if (g->seg_type != 2) return KO_IO_ERROR;
     double js = jd_to_spice_jd(jd);
     double j0 = jd_to_spice_jd(g->jd_start);
@@ -782,12 +877,14 @@ static void nongrav_add_acceleration(const ko_nongrav *ng, double *acc,
     }
 }
 
-/* spk_accel (acceleration.rs:121-168) with massive_obj = PLANETS. */
-int ko_spk_accel(double t, const double pos[3], const double vel[3],
-                 const ko_nongrav *ng, int exact_eval, double acc[3],
-                 int64_t *impact_id) {
+/* spk_accel (acceleration.rs:121-168) with massive_obj = PLANETS, or MASSES
+ * when include_asteroids (propagation/mod.rs:82-88). */
+int ko_spk_accel_ext(double t, const double pos[3], const double vel[3],
+                     const ko_nongrav *ng, int exact_eval, int include_asteroids,
+                     double acc[3], int64_t *impact_id) {
     acc[0] = acc[1] = acc[2] = 0.0;
-    for (int b = 0; b < 10; ++b) {
+    const int n_bodies = include_asteroids ? N_MASSES : N_PLANETS;
+    for (int b = 0; b < n_bodies; ++b) {
         const grav_params *gp = &PLANETS[b];
         ko_state st;
         int rc = spk_get_state(gp->naif_id, t, 0, KO_FRAME_EQUATORIAL, &st);
@@ -805,6 +902,12 @@ int ko_spk_accel(double t, const double pos[3], const double vel[3],
         if (gp->naif_id == 10 && ng && ng->kind) nongrav_add_acceleration(ng, acc, rp, rv);
     }
     return KO_OK;
+}
+
+int ko_spk_accel(double t, const double pos[3], const double vel[3],
+                 const ko_nongrav *ng, int exact_eval, double acc[3],
+                 int64_t *impact_id) {
+    return ko_spk_accel_ext(t, pos, vel, ng, exact_eval, 0, acc, impact_id);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -973,6 +1076,7 @@ int ko_radau_central(double t0, double t1, const double pos[3],
 
 typedef struct {
     const ko_nongrav *ng;
+    int include_asteroids;
     int64_t impact_id;
     double impact_jd;
 } spk_ctx;
@@ -981,21 +1085,22 @@ static int spk_force(double t, const double *x, const double *v, int exact,
                      double *a, void *ctx) {
     spk_ctx *c = ctx;
     int64_t id = 0;
-    int rc = ko_spk_accel(t, x, v, c->ng, exact, a, &id);
+    int rc = ko_spk_accel_ext(t, x, v, c->ng, exact, c->include_asteroids, a, &id);
     if (rc == KO_IMPACT) { c->impact_id = id; c->impact_jd = t; }
     return rc;
 }
 
 /* propagate_n_body_spk (propagation/mod.rs:70-118). */
-int ko_propagate_n_body(ko_state *s, double jd_final, const ko_nongrav *ng,
-                        ko_counters *cnt, int64_t *impact_id, double *impact_jd) {
+int ko_propagate_n_body_ext(ko_state *s, double jd_final, int include_asteroids,
+                            const ko_nongrav *ng, ko_counters *cnt, int64_t *impact_id,
+                            double *impact_jd) {
     int64_t center = s->center;
     int32_t frame = s->frame;
     int rc = ko_change_center(s, 0);
     if (rc) return rc;
     rc = ko_change_frame(s, KO_FRAME_EQUATORIAL);
     if (rc) return rc;
-    spk_ctx c = {ng, 0, 0.0};
+    spk_ctx c = {ng, include_asteroids, 0, 0.0};
     double xo[3], vo[3];
     rc = radau_integrate(spk_force, &c, s->pos, s->vel, s->jd, jd_final, xo, vo, cnt);
     if (rc) {
@@ -1013,6 +1118,11 @@ int ko_propagate_n_body(ko_state *s, double jd_final, const ko_nongrav *ng,
     rc = ko_change_center(s, center);
     if (rc) return rc;
     return ko_change_frame(s, frame);
+}
+
+int ko_propagate_n_body(ko_state *s, double jd_final, const ko_nongrav *ng,
+                        ko_counters *cnt, int64_t *impact_id, double *impact_jd) {
+    return ko_propagate_n_body_ext(s, jd_final, 0, ng, cnt, impact_id, impact_jd);
 }
 
 /* propagate_two_body (kepler.rs:306-322). */
@@ -1120,6 +1230,7 @@ typedef struct {
     int64_t *impact_id;
     double *impact_jd;
     ko_counters *cnt;
+    int include_asteroids;
 } nb_ctx;
 
 static void nb_range(int64_t lo, int64_t hi, void *vctx) {
@@ -1135,7 +1246,8 @@ static void nb_range(int64_t lo, int64_t hi, void *vctx) {
         int finite = isfinite(s.jd);
         for (int k = 0; k < 3; ++k) finite &= isfinite(s.pos[k]) && isfinite(s.vel[k]);
         if (!finite) rc = KO_VALUE_ERROR;
-        else rc = ko_propagate_n_body(&s, jf, c->ng ? &c->ng[i] : NULL, &tmp, &iid, &ijd);
+        else rc = ko_propagate_n_body_ext(&s, jf, c->include_asteroids,
+                                          c->ng ? &c->ng[i] : NULL, &tmp, &iid, &ijd);
         c->status[i] = rc;
         if (c->impact_id) c->impact_id[i] = rc == KO_IMPACT ? iid : 0;
         if (c->impact_jd) c->impact_jd[i] = rc == KO_IMPACT ? ijd : NAN;
@@ -1147,15 +1259,24 @@ static void nb_range(int64_t lo, int64_t hi, void *vctx) {
     }
 }
 
+int ko_batch_n_body_ext(int64_t n, const double *jd0, const double *const *xyz,
+                        int64_t center, int32_t frame, const double *jd_final,
+                        int jd_final_stride, int include_asteroids, const ko_nongrav *ng,
+                        double *const *out, int32_t *status, int64_t *impact_id,
+                        double *impact_jd, ko_counters *cnt, int n_threads) {
+    nb_ctx c = {jd0, xyz, center, frame, jd_final, jd_final_stride, ng, out,
+                status, impact_id, impact_jd, cnt, include_asteroids};
+    parallel_for(n, 1000, n_threads, nb_range, &c);
+    return KO_OK;
+}
+
 int ko_batch_n_body(int64_t n, const double *jd0, const double *const *xyz,
                     int64_t center, int32_t frame, const double *jd_final,
                     int jd_final_stride, const ko_nongrav *ng,
                     double *const *out, int32_t *status, int64_t *impact_id,
                     double *impact_jd, ko_counters *cnt, int n_threads) {
-    nb_ctx c = {jd0, xyz, center, frame, jd_final, jd_final_stride, ng, out,
-                status, impact_id, impact_jd, cnt};
-    parallel_for(n, 1000, n_threads, nb_range, &c);
-    return KO_OK;
+    return ko_batch_n_body_ext(n, jd0, xyz, center, frame, jd_final, jd_final_stride, 0, ng,
+                               out, status, impact_id, impact_jd, cnt, n_threads);
 }
 
 /* ------------------------------------------------------------------------ */

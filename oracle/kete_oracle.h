@@ -84,6 +84,11 @@ int ko_spk_segment_info(int idx, int64_t *obj_id, int64_t *center_id,
 /* try_get_state(id, jd, center, frame) (spk.rs:87-96). */
 int ko_spk_get_state(int64_t id, double jd, int64_t center, int32_t frame,
                      double pos[3], double vel[3]);
+/* SpkSegmentType21::try_get_pos_vel on one record (spk_segments.rs:1007-1100):
+ * rec is kete's record layout with n = MAXDIM, js in TDB seconds past J2000;
+ * position in AU, velocity in AU/day. */
+int ko_spk21_record_eval(const double *rec, int64_t n, double js, double pos[3],
+                         double vel[3]);
 /* chebyshev_evaluate_both (interpolation.rs:25-77). */
 int ko_chebyshev_evaluate_both(double x, const double *cx, const double *cy,
                                const double *cz, int n, double val[3],
@@ -97,6 +102,11 @@ int ko_change_frame(ko_state *s, int32_t new_frame);
 int ko_spk_accel(double t, const double pos[3], const double vel[3],
                  const ko_nongrav *ng, int exact_eval, double acc[3],
                  int64_t *impact_id);
+/* spk_accel with PLANETS, or with MASSES (the planets, the Moon and five
+ * asteroids, constants/gravity.rs:43-134) when include_asteroids. */
+int ko_spk_accel_ext(double t, const double pos[3], const double vel[3],
+                     const ko_nongrav *ng, int exact_eval, int include_asteroids,
+                     double acc[3], int64_t *impact_id);
 
 /* ---- integrator ---------------------------------------------------------- */
 /* RadauIntegrator::integrate with central_accel (radau.rs:160-218,
@@ -107,6 +117,10 @@ int ko_radau_central(double t0, double t1, const double pos[3],
 /* propagate_n_body_spk (propagation/mod.rs:70-118). */
 int ko_propagate_n_body(ko_state *s, double jd_final, const ko_nongrav *ng,
                         ko_counters *cnt, int64_t *impact_id, double *impact_jd);
+/* ... with kete's include_extended flag (propagation/mod.rs:73,82-88). */
+int ko_propagate_n_body_ext(ko_state *s, double jd_final, int include_asteroids,
+                            const ko_nongrav *ng, ko_counters *cnt, int64_t *impact_id,
+                            double *impact_jd);
 /* propagate_two_body (kepler.rs:306-322). */
 int ko_propagate_two_body(ko_state *s, double jd_final);
 
@@ -126,6 +140,12 @@ int ko_batch_n_body(int64_t n, const double *jd0, const double *const *xyz,
                     double *const *out, int32_t *status, int64_t *impact_id,
                     double *impact_jd, ko_counters *cnt /* [n] or NULL */,
                     int n_threads);
+/* ... with propagate_n_body's include_asteroids (propagation.rs:71-72). */
+int ko_batch_n_body_ext(int64_t n, const double *jd0, const double *const *xyz,
+                        int64_t center, int32_t frame, const double *jd_final,
+                        int jd_final_stride, int include_asteroids, const ko_nongrav *ng,
+                        double *const *out, int32_t *status, int64_t *impact_id,
+                        double *impact_jd, ko_counters *cnt, int n_threads);
 
 /* ---- FOV ----------------------------------------------------------------- */
 /* OnSkyRectangle::new (fov/patches.rs:122-155): normals[4][3]. */
