@@ -75,7 +75,7 @@ def setup_nbody(n, xyz, jd0, jdf_host, stride):
     return t
 
 
-def nbody_step_fn(ctx, t, n, stride, mode):
+def nbody_step_fn(ctx, t, n, stride, mode, include_asteroids=False):
     import kete_amd as ka
     from kete_amd import _lib
     xyz_p = _lib.ptr_array([t["xyz"][k].data_ptr() for k in range(6)])
@@ -84,7 +84,7 @@ def nbody_step_fn(ctx, t, n, stride, mode):
     def step(counters=False):
         rc = ctx.L.kh_propagate_n_body_dev(
             ctx.h, n, _lib.raw(t["jd0"].data_ptr()), xyz_p, 10, ka.Frames.Ecliptic,
-            _lib.raw(t["jf"].data_ptr()), stride, None, 1e-6, mode, out_p,
+            _lib.raw(t["jf"].data_ptr()), stride, int(include_asteroids), None, 1e-6, mode, out_p,
             _lib.raw(t["st"].data_ptr(), C.c_int32), _lib.raw(t["iid"].data_ptr(), C.c_int64),
             _lib.raw(t["ijd"].data_ptr()),
             _lib.raw(t["cnt"].data_ptr(), _lib.KhCounters) if counters else None, 0)

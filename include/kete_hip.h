@@ -78,6 +78,27 @@ typedef struct {
     const double *records;
 } kh_spk2_seg;
 
+/* One SPK Type 21 segment (spice/spk_segments.rs:958-1102): extended
+ * modified difference arrays, as JPL Horizons delivers small bodies.
+ * `records` holds n_rec records of 4*maxdim+11 doubles in kete's layout
+ * [t_ref, G[maxdim], x vx y vy z vz, DT[3][maxdim], KQMAX1, KQ[3]] (km, TDB
+ * seconds past J2000; spk_segments.rs:1010-1018), `epochs` the n_rec final
+ * epochs that follow them in the DAF array; maxdim is the array trailer's
+ * MAXDIM (spk_segments.rs:971-986).  jd_start/jd_end are the summary times
+ * converted with spice_jds_to_jd (spice/mod.rs:25-27).  Centers 10 and 0 and
+ * frames 1 and 17 are accepted. */
+typedef struct {
+    int64_t obj_id;
+    int64_t center_id;
+    int32_t frame;
+    int32_t maxdim;
+    int64_t n_rec;
+    double jd_start;
+    double jd_end;
+    const double *records;
+    const double *epochs;
+} kh_spk21_seg;
+
 /* Device selection and stream ownership.  Replaces nothing in kete (its
  * state is the process-global LOADED_SPK, spice/spk.rs:368-377); one context
  * per host thread. */
@@ -91,6 +112,18 @@ void *kh_ctx_stream(kh_ctx *ctx);
  * planet segments kete's spk_accel reads (spice/spk.rs:307-329,65-82): the
  * host side walks SpkCollection.planet_segments and passes them here. */
 int32_t kh_spk_upload_type2(kh_ctx *ctx, const kh_spk2_seg *segs, int64_t nseg);
+
+/* Upload the Type 21 segments (replacing those uploaded before; nseg may be
+ * 0), in load order.  Replaces SpkCollection::load_file for the segments it
+ * files under SpkCollection.segments (spice/spk.rs:307-329) and the
+ * From<DafArray> view of SpkSegmentType21 (spice/spk_segments.rs:971-986).
+ * Segments of kete's five MASSES asteroids (20000001, 20000002, 20000004,
+ * 20000010, 20000704; constants/gravity.rs:103-134) are the perturbers of
+ * include_asteroids; every object's are readable with kh_spk_states.
+ * Records whose difference order exceeds 16, and records kete could not
+ * evaluate (a zero step size, spk_segments.rs:1037-1042), are rejected here
+ * with KH_E_ARG.  At most 32 segments. */
+int32_t kh_spk_upload_type21(kh_ctx *ctx, const kh_spk21_seg *segs, int64_t nseg);
 
 /* State of `obj_id` relative to `center` in `frame` at m epochs, evaluated on
  * the device (SpkCollection::try_get_state, spice/spk.rs:87-96).
@@ -119,8 +152,12 @@ int32_t kh_propagate_two_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
 /* N-body propagation with the Gauss-Radau 15 integrator and kete's SPK force
  * model.  Replaces the loop body of propagation_n_body_spk_py
  * (src/kete/rust/propagation.rs:70-151) around propagate_n_body_spk
- * (src/kete_core/src/propagation/mod.rs:70-118) with include_asteroids=false.
+ * (src/kete_core/src/propagation/mod.rs:70-118).
  * jd_final: n entries, or 1 entry broadcast when jd_final_stride is 0.
+ * include_asteroids: non-zero sums kete's MASSES (PLANETS and five asteroids
+ * read from the Type 21 segments) instead of PLANETS
+ * (propagation/mod.rs:82-88); without their segments every object ends
+ * with DAFLimits, as in kete.
  * epsilon: Radau EPSILON; 1e-6 reproduces kete (radau.rs:71).
  * mode: 0 plain launch (one lane per object), 1 persistent launch with a
  * device work queue, 2 persistent with three lanes per object (one per
@@ -131,7 +168,8 @@ int32_t kh_propagate_two_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
 int32_t kh_propagate_n_body(kh_ctx *ctx, int64_t n, const double *jd0,
                             const double *const xyz[6], int64_t center, int32_t frame,
                             const double *jd_final, int32_t jd_final_stride,
-                            const kh_nongrav *ng, double epsilon, int32_t mode,
+                            int32_t include_asteroids, const kh_nongrav *ng, double epsilon,
+                            int32_t mode,
                             double *const out[6], int32_t *status, int64_t *impact_id,
                             double *impact_jd, kh_counters *counters);
 /* Device-pointer variant; ng points to device memory (nullable), counters is
@@ -139,7 +177,8 @@ int32_t kh_propagate_n_body(kh_ctx *ctx, int64_t n, const double *jd0,
 int32_t kh_propagate_n_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                                 const double *const xyz[6], int64_t center, int32_t frame,
                                 const double *jd_final, int32_t jd_final_stride,
-                                const kh_nongrav *ng, double epsilon, int32_t mode,
+                                int32_t include_asteroids, const kh_nongrav *ng,
+                                double epsilon, int32_t mode,
                                 double *const out[6], int32_t *status, int64_t *impact_id,
                                 double *impact_jd, kh_counters *counters, int32_t sync);
 
@@ -162,10 +201,13 @@ void kh_fov_rect_normals(const double pointing[3], double rotation, double lon_w
  * Outputs, ordered by (field epoch order, object index) as kete returns them:
  * the original field index, the object index, and the light-time-corrected
  * state [7][cap] (jd, pos, vel).  *n_hits receives the count; a count above
- * cap returns KH_E_ARG with *n_hits set to the size needed. */
+ * cap returns KH_E_ARG with *n_hits set to the size needed.
+ * include_asteroids: kete's flag of the same name (checks.rs:25-31): every
+ * N-body propagation sums MASSES instead of PLANETS. */
 int32_t kh_fov_state_check(kh_ctx *ctx, int64_t n, double jd0, const double *const xyz[6],
                            int32_t frame, int64_t n_fields, const double *field_jd,
                            const double *const obs[6], const double *normals, double dt_limit,
+                           int32_t include_asteroids,
                            int64_t cap, int64_t *n_hits, int64_t *hit_field, int64_t *hit_obj,
                            double *const hit_state[7]);
 

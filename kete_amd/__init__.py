@@ -152,9 +152,13 @@ class Context:
             pass
 
     # -- ephemeris ---------------------------------------------------------
-    def load_spk(self, path_or_segments):
-        segs = read_spk(path_or_segments) if isinstance(path_or_segments, str) else \
-            list(path_or_segments)
+    def load_spk(self, *paths_or_segments):
+        """Replace the ephemeris by the segments of the files (or Segment
+        lists) given, in load order: the Type 2 planet segments and the
+        Type 21 small-body segments (kete ``spk.load``, ``spice/spk.rs:307-329``)."""
+        segs = []
+        for item in paths_or_segments:
+            segs.extend(read_spk(item) if isinstance(item, str) else list(item))
         table = Type2Table(segs)
         arr = (_lib.KhSpk2Seg * len(table.segments))()
         keep = []
@@ -164,7 +168,18 @@ class Context:
             arr[k] = _lib.KhSpk2Seg(s.obj_id, s.center_id, s.frame, s.rsize, s.n_rec, s.jd_start,
                                     s.jd_end, s.intlen, _lib.dptr(rec.reshape(-1)))
         _lib.check(self.L.kh_spk_upload_type2(self.h, arr, len(table.segments)))
+        t21 = [s for s in segs if s.seg_type == 21]
+        arr21 = (_lib.KhSpk21Seg * max(len(t21), 1))()
+        for k, s in enumerate(t21):
+            rec = np.ascontiguousarray(s.records())
+            ep = np.ascontiguousarray(s.epochs())
+            keep += [rec, ep]
+            arr21[k] = _lib.KhSpk21Seg(s.obj_id, s.center_id, s.frame, s.n_coef, s.n_rec,
+                                       s.jd_start, s.jd_end, _lib.dptr(rec.reshape(-1)),
+                                       _lib.dptr(ep))
+        _lib.check(self.L.kh_spk_upload_type21(self.h, arr21, len(t21)))
         self.table = table
+        self.type21 = t21
         return table
 
     def spk_state(self, obj_id, jd, center=10, frame=FRAME_ECLIPTIC):
@@ -193,7 +208,7 @@ class Context:
         return out, st
 
     def n_body(self, batch: StateArray, jd_final, non_gravs=None, epsilon=1e-6, mode=-1,
-               counters=False):
+               counters=False, include_asteroids=False):
         """-> dict(out[6, n], status[n], impact_id[n], impact_jd[n], counters[n, 4])"""
         if self.table is None:
             raise KeteHipError("no ephemeris loaded")
@@ -213,7 +228,7 @@ class Context:
             ng = ng_arr.ctypes.data_as(C.POINTER(_lib.KhNonGrav))
         _lib.check(self.L.kh_propagate_n_body(
             self.h, n, _lib.dptr(batch.jd), _lib.ptr_array(list(batch.xyz)), batch.center_id,
-            batch.frame, _lib.dptr(jf), stride, ng, epsilon, mode,
+            batch.frame, _lib.dptr(jf), stride, int(bool(include_asteroids)), ng, epsilon, mode,
             _lib.ptr_array([out[k] for k in range(6)]), _lib.iptr(st, C.c_int32),
             _lib.iptr(iid, C.c_int64), _lib.dptr(ijd),
             cnt.ctypes.data_as(C.POINTER(_lib.KhCounters)) if cnt is not None else None))
@@ -272,22 +287,24 @@ _CTX: Context | None = None
 
 
 def get_context() -> Context:
-    """The default context, with the synthetic ephemeris loaded on first use."""
+    """The default context, with the synthetic ephemeris loaded on first use:
+    the planets (Type 2) and, as kete's core kernels hold them, the five
+    perturbing asteroids (Type 21)."""
     global _CTX
     if _CTX is None:
         ctx = Context(0)
-        from .synthetic import ephemeris_path
-        ctx.load_spk(ephemeris_path())
+        from .synthetic import asteroid_ephemeris_path, ephemeris_path
+        ctx.load_spk(ephemeris_path(), asteroid_ephemeris_path())
         _CTX = ctx
     return _CTX
 
 
-def load_spk(path: str) -> Type2Table:
+def load_spk(*paths: str) -> Type2Table:
     """Replace the default context's ephemeris (kete ``spice.spk.load``)."""
     global _CTX
     if _CTX is None:
         _CTX = Context(0)
-    return _CTX.load_spk(path)
+    return _CTX.load_spk(*paths)
 
 
 def spk_state(obj_id, jd, center=10, frame=FRAME_ECLIPTIC):
@@ -318,15 +335,14 @@ def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppre
                      suppress_impact_errors=False):
     """kete ``propagate_n_body`` (``src/kete/rust/propagation.rs:70-151``):
     Radau-15 integration with the planets, Moon and solar/Jovian GR+J2 read from
-    the loaded SPK.  Failed objects come back as NaN states; with
+    the loaded SPK, and with ``include_asteroids`` Ceres, Pallas, Vesta,
+    Hygiea and Interamnia from their Type 21 segments.  Failed objects come back as NaN states; with
     ``suppress_errors=False`` the first failure raises ``ValueError``."""
-    if include_asteroids:
-        raise NotImplementedError("include_asteroids needs SPK Type 21 perturbers, "
-                                  "which this build does not evaluate")
     batch, soa = _as_batch(states)
     if non_gravs is not None and len(non_gravs) != len(batch):
         raise ValueError("non_gravs must be the same length as states.")
-    r = get_context().n_body(batch, [jd], non_gravs=non_gravs)
+    r = get_context().n_body(batch, [jd], non_gravs=non_gravs,
+                             include_asteroids=include_asteroids)
     st = r["status"]
     if not suppress_errors and np.any(st != 0):
         i = int(np.flatnonzero(st != 0)[0])

@@ -23,7 +23,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP runtime failure", -3: "no ephemeris",
 # Symbols declared in include/kete_hip.h
 EXPORTS = [
     "kh_ctx_create", "kh_ctx_destroy", "kh_last_error", "kh_ctx_stream",
-    "kh_spk_upload_type2", "kh_spk_states", "kh_propagate_two_body",
+    "kh_spk_upload_type2", "kh_spk_upload_type21", "kh_spk_states", "kh_propagate_two_body",
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev",
@@ -35,6 +35,13 @@ class KhSpk2Seg(C.Structure):
                 ("rsize", C.c_int32), ("n_rec", C.c_int64), ("jd_start", C.c_double),
                 ("jd_end", C.c_double), ("intlen_s", C.c_double),
                 ("records", C.POINTER(C.c_double))]
+
+
+class KhSpk21Seg(C.Structure):
+    _fields_ = [("obj_id", C.c_int64), ("center_id", C.c_int64), ("frame", C.c_int32),
+                ("maxdim", C.c_int32), ("n_rec", C.c_int64), ("jd_start", C.c_double),
+                ("jd_end", C.c_double), ("records", C.POINTER(C.c_double)),
+                ("epochs", C.POINTER(C.c_double))]
 
 
 class KhNonGrav(C.Structure):
@@ -99,6 +106,8 @@ def load():
     L.kh_ctx_stream.restype = V
     L.kh_spk_upload_type2.argtypes = [V, C.POINTER(KhSpk2Seg), I64]
     L.kh_spk_upload_type2.restype = I32
+    L.kh_spk_upload_type21.argtypes = [V, C.POINTER(KhSpk21Seg), I64]
+    L.kh_spk_upload_type21.restype = I32
     L.kh_spk_states.argtypes = [V, I64, I64, I32, I64, P, PP, C.POINTER(I32)]
     L.kh_spk_states.restype = I32
     tb = [V, I64, P, PP, I64, I32, I64, P, P, PP, C.POINTER(I32)]
@@ -106,8 +115,8 @@ def load():
     L.kh_propagate_two_body.restype = I32
     L.kh_propagate_two_body_dev.argtypes = tb + [C.POINTER(I32), I32]
     L.kh_propagate_two_body_dev.restype = I32
-    nb = [V, I64, P, PP, I64, I32, P, I32, C.POINTER(KhNonGrav), D, I32, PP, C.POINTER(I32),
-          C.POINTER(I64), P, C.POINTER(KhCounters)]
+    nb = [V, I64, P, PP, I64, I32, P, I32, I32, C.POINTER(KhNonGrav), D, I32, PP,
+          C.POINTER(I32), C.POINTER(I64), P, C.POINTER(KhCounters)]
     L.kh_propagate_n_body.argtypes = nb
     L.kh_propagate_n_body.restype = I32
     L.kh_propagate_n_body_dev.argtypes = nb + [I32]
@@ -116,8 +125,8 @@ def load():
     L.kh_last_launch.restype = I32
     L.kh_fov_rect_normals.argtypes = [P, D, D, D, P]
     L.kh_fov_rect_normals.restype = None
-    L.kh_fov_state_check.argtypes = [V, I64, D, PP, I32, I64, P, PP, P, D, I64, C.POINTER(I64),
-                                     C.POINTER(I64), C.POINTER(I64), PP]
+    L.kh_fov_state_check.argtypes = [V, I64, D, PP, I32, I64, P, PP, P, D, I32, I64,
+                                     C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
     L.kh_fov_state_check.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32

@@ -152,7 +152,7 @@ def radec_plain(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, epsilon=1e-6, ct
         torch.cuda.synchronize(dev)
         _lib.check(L.kh_propagate_n_body_dev(
             ctx.h, n, _lib.raw(t.data_ptr()), cols(state), 10, frame,
-            _lib.raw(epochs[j:].data_ptr()), 0, _lib.raw(ng.data_ptr(), _lib.KhNonGrav),
+            _lib.raw(epochs[j:].data_ptr()), 0, 0, _lib.raw(ng.data_ptr(), _lib.KhNonGrav),
             epsilon, -1, cols(new), i32(st_nb), None, None, None, 0))
         ob = np.ascontiguousarray(obs[:, j])
         _lib.check(L.kh_propagate_two_body_dev(

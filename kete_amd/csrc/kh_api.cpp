@@ -22,6 +22,8 @@ int32_t kh_fail(int32_t code, const std::string& msg) {
 }
 #define fail kh_fail
 
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 namespace kh {
 void fill_radau_tables(RadauTables& t) {
   // GAUSS_RADAU_SPACINGS (radau.rs:24-33)
@@ -111,6 +113,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_coef) (void)hipFree(c->d_coef);
+  if (c->d_t21) (void)hipFree(c->d_t21);
   if (c->d_spk) (void)hipFree(c->d_spk);
   if (c->dbuf) (void)hipFree(c->dbuf);
   if (c->obuf) (void)hipFree(c->obuf);
@@ -155,7 +158,11 @@ int32_t kh_spk_upload_type2(kh_ctx* c, const kh_spk2_seg* segs, int64_t nseg) {
   if (!c || !segs || nseg <= 0) return fail(KH_E_ARG, "null context or no segments");
   if (nseg > MAX_SEGS) return fail(KH_E_ARG, "too many segments (max 24)");
   HIPCHK(hipSetDevice(c->device));
-  SpkDev spk{};
+  SpkDev spk = c->spk;  // keeps the Type 21 part
+  spk.coef = nullptr;
+  spk.n_seg = spk.all14 = spk.shared_grid = spk.emb_seg = spk.sun_seg = 0;
+  std::memset(spk.seg, 0, sizeof(spk.seg));
+  std::memset(spk.body, 0, sizeof(spk.body));
   int64_t total = 0;
   bool all14 = true;
   c->seg_obj.clear();
@@ -263,6 +270,234 @@ int32_t kh_spk_upload_type2(kh_ctx* c, const kh_spk2_seg* segs, int64_t nseg) {
   return KH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// SPK Type 21
+// ---------------------------------------------------------------------------
+// kete's MASSES after PLANETS (gravity.rs:103-134): id, mass (GM in
+// AU^3/day^2), radius (AU).
+static const struct {
+  int64_t id;
+  double mass, radius;
+} kAsteroids[N_ASTEROIDS] = {
+    {20000001, 4.7191659919436e-10 * GMS, 6.276827e-6},
+    {20000002, 1.0259143964958e-10 * GMS, 3.415824e-6},
+    {20000004, 1.3026452498655e-10 * GMS, 3.512082e-6},
+    {20000010, 4.3953391300849e-11 * GMS, 2.894426e-6},
+    {20000704, 1.911e-11 * GMS, 2.219283e-6},
+};
+
+// Rewrites one Type 21 record (kete's layout, spk_segments.rs:1010-1018) as
+// the device record: the modified-difference line of SpkSegmentType21::
+// try_get_pos_vel (spk_segments.rs:1032-1089) is a polynomial in dt; its
+// coefficients, in tau = dt / S with S the largest |dt| the record serves
+// ([lo, hi] in seconds), are formed here once in long double by running the
+// same W recursion on polynomials instead of numbers.  Returns false (why set)
+// for records kete cannot evaluate (a zero step size) or this layout cannot
+// hold (an order above T21_NP).
+static bool t21_prepare(const double* rec, int n, double lo, double hi, double* out,
+                        std::string& why) {
+  typedef long double L;
+  constexpr int MAXD = 32;
+  const int K = (int)rec[4 * n + 7];  // KQMAX1
+  int kq[3];
+  for (int c = 0; c < 3; ++c) kq[c] = (int)rec[4 * n + 8 + c];
+  if (n > 25 || K < 2 || K > n + 1) {
+    why = "MAXDIM or KQMAX1 out of range";
+    return false;
+  }
+  for (int c = 0; c < 3; ++c)
+    if (kq[c] < 0 || kq[c] > K - 1 || kq[c] > T21_NP) {
+      why = "difference order out of range (at most " + std::to_string(T21_NP) + ")";
+      return false;
+    }
+  const double* g = rec + 1;
+  for (int j = 0; j < K - 2; ++j)
+    if (g[j] == 0.0) {
+      why = "zero step size";
+      return false;
+    }
+  const double t_ref = rec[0];
+  double S = std::fmax(std::fabs(lo - t_ref), std::fabs(hi - t_ref));
+  if (!(S > 0.0) || !std::isfinite(S)) S = 1.0;
+  // w[j] as polynomials in tau; initial values 1/(j+1)
+  static thread_local L w[MAXD][MAXD];
+  for (int j = 0; j < K; ++j) {
+    for (int d = 0; d < MAXD; ++d) w[j][d] = 0.0L;
+    w[j][0] = 1.0L / (L)(j + 1);
+  }
+  // FC(j) = (G(j-1) + S tau) / G(j), WC(j) = S tau / G(j)  (G(-1) = 0)
+  auto pass = [&](int ks, int jx) {
+    for (int j = 0; j < jx; ++j) {
+      const L gj = (L)g[j];
+      const L f0 = (j > 0 ? (L)g[j - 1] : 0.0L) / gj, f1 = (L)S / gj, w1 = (L)S / gj;
+      L nw[MAXD];
+      const L* a = w[j + ks - 1];
+      const L* b = w[j + ks];
+      nw[0] = f0 * a[0];
+      for (int d = 1; d < MAXD; ++d) nw[d] = f0 * a[d] + f1 * a[d - 1] - w1 * b[d - 1];
+      for (int d = 0; d < MAXD; ++d) w[j + ks][d] = nw[d];
+    }
+  };
+  int ks = K - 1, jx = 0;
+  while (ks >= 2) {
+    jx += 1;
+    pass(ks, jx);
+    ks -= 1;
+  }
+  // position sums use w[1..kq] (ks = 1): P_c = sum_j DT_c[j-1] w[j]
+  const double* ref = rec + n + 1;
+  const double* dd = rec + n + 7;
+  out[0] = t_ref;
+  out[1] = 1.0 / S;
+  for (int k = 0; k < 6; ++k) out[2 + k] = ref[k];
+  for (int c = 0; c < 3; ++c) {
+    L P[MAXD];
+    for (int d = 0; d < MAXD; ++d) P[d] = 0.0L;
+    for (int j = kq[c]; j >= 1; --j)
+      for (int d = 0; d < MAXD; ++d) P[d] += (L)dd[c * n + j - 1] * w[j][d];
+    for (int d = T21_NP; d < MAXD; ++d)
+      if (P[d] != 0.0L) {
+        why = "internal: polynomial degree above the order";
+        return false;
+      }
+    for (int d = 0; d < T21_NP; ++d) out[8 + c * T21_NP + d] = (double)P[d];
+  }
+  return true;
+}
+
+int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) {
+  if (!c || (nseg > 0 && !segs) || nseg < 0) return fail(KH_E_ARG, "null context or segments");
+  if (nseg > MAX_T21_SEGS)
+    return fail(KH_E_ARG, "too many Type 21 segments (max " + std::to_string(MAX_T21_SEGS) + ")");
+  HIPCHK(hipSetDevice(c->device));
+  // device order: grouped by object (first appearance), load order within
+  std::vector<int64_t> order;
+  {
+    std::vector<int64_t> ids;
+    for (int64_t k = 0; k < nseg; ++k) {
+      bool seen = false;
+      for (int64_t x : ids) seen = seen || x == segs[k].obj_id;
+      if (!seen) ids.push_back(segs[k].obj_id);
+    }
+    for (int64_t id : ids)
+      for (int64_t k = 0; k < nseg; ++k)
+        if (segs[k].obj_id == id) order.push_back(k);
+  }
+  int64_t n_rec_tot = 0, n_tab_tot = 0;
+  std::vector<int32_t> nb(nseg);
+  for (int64_t k = 0; k < nseg; ++k) {
+    const kh_spk21_seg& s = segs[k];
+    const std::string tag = "Type 21 segment " + std::to_string(k) + ": ";
+    if (!s.records || !s.epochs || s.n_rec <= 0 || s.n_rec > (1 << 24) || s.maxdim < 1 ||
+        s.maxdim > 25)
+      return fail(KH_E_ARG, tag + "bad record layout");
+    if (s.frame != KH_FRAME_EQUATORIAL && s.frame != KH_FRAME_ECLIPTIC)
+      return fail(KH_E_ARG, tag + "frame must be 1 or 17");
+    if (s.center_id != 10 && s.center_id != 0)
+      return fail(KH_E_ARG, tag + "center must be the Sun (10) or the barycentre (0)");
+    if (!(s.jd_end >= s.jd_start)) return fail(KH_E_ARG, tag + "bad time range");
+    for (int64_t r = 1; r < s.n_rec; ++r)
+      if (!(s.epochs[r] >= s.epochs[r - 1])) return fail(KH_E_ARG, tag + "final epochs unsorted");
+    nb[k] = (int32_t)std::min<int64_t>(std::max<int64_t>(2 * s.n_rec, 1), 1 << 20);
+    n_rec_tot += s.n_rec;
+    n_tab_tot += nb[k];
+  }
+  if (n_rec_tot >= ((int64_t)1 << 31) / T21_REC) return fail(KH_E_ARG, "too many Type 21 records");
+  std::vector<double> rec_h((size_t)n_rec_tot * T21_REC), ep_h((size_t)n_rec_tot);
+  std::vector<int32_t> tab_h((size_t)n_tab_tot);
+  SpkDev spk = c->spk;
+  std::memset(spk.t21, 0, sizeof(spk.t21));
+  std::memset(spk.ast, 0, sizeof(spk.ast));
+  std::vector<int64_t> obj(nseg);
+  int64_t ro = 0, to = 0;
+  for (int64_t q = 0; q < nseg; ++q) {
+    const int64_t k = order[q];
+    const kh_spk21_seg& s = segs[k];
+    const int n = s.maxdim, rl = 4 * s.maxdim + 11;
+    T21SegDev& g = spk.t21[q];
+    g.jd_start = s.jd_start;
+    g.jd_end = s.jd_end;
+    g.rec_off = ro;
+    g.ep_off = ro;
+    g.tab_off = to;
+    g.n_rec = (int32_t)s.n_rec;
+    g.n_bucket = nb[k];
+    g.ecliptic = s.frame == KH_FRAME_ECLIPTIC;
+    g.sun_centred = s.center_id == 10;
+    const double seg_lo = (s.jd_start - 2451545.0) * 86400.0;  // jd_to_spice_jd
+    for (int64_t r = 0; r < s.n_rec; ++r) {
+      const double lo = r == 0 ? std::fmin(seg_lo, s.epochs[0]) : s.epochs[r - 1];
+      std::string why;
+      if (!t21_prepare(s.records + r * rl, n, lo, s.epochs[r], &rec_h[(ro + r) * T21_REC], why))
+        return fail(KH_E_ARG, "Type 21 segment " + std::to_string(k) + " record " +
+                                  std::to_string(r) + ": " + why);
+      ep_h[ro + r] = s.epochs[r];
+    }
+    // buckets over [t0, last epoch]: entry b is the first record whose final
+    // epoch is >= t0 + b*width - width*1e-6 (a lower bound for any time that
+    // rounds into bucket b)
+    const double t0 = std::fmin(seg_lo, s.epochs[0]);
+    const double t1 = s.epochs[s.n_rec - 1];
+    double bw = (t1 - t0) / nb[k];
+    if (!(bw > 0.0)) {
+      bw = 1.0;
+      g.n_bucket = nb[k] = 1;
+    }
+    g.t0 = t0;
+    g.inv_bw = 1.0 / bw;
+    int64_t i = 0;
+    for (int32_t b = 0; b < g.n_bucket; ++b) {
+      const double x = t0 + b * bw - bw * 1e-6;
+      while (i < s.n_rec - 1 && s.epochs[i] < x) ++i;
+      tab_h[to + b] = (int32_t)i;
+    }
+    obj[q] = s.obj_id;
+    ro += s.n_rec;
+    to += g.n_bucket;
+  }
+  spk.n_t21_seg = (int32_t)nseg;
+  for (int a = 0; a < N_ASTEROIDS; ++a) {
+    AstDev& ad = spk.ast[a];
+    ad.seg_begin = ad.seg_end = 0;
+    for (int64_t q = 0; q < nseg; ++q)
+      if (obj[q] == kAsteroids[a].id) {
+        if (ad.seg_end == 0) ad.seg_begin = (int32_t)q;
+        ad.seg_end = (int32_t)q + 1;
+      }
+    ad.bd.seg = -1;
+    ad.bd.emb = 0;
+    ad.bd.id = (int32_t)kAsteroids[a].id;
+    ad.bd.kind = 0;
+    ad.bd.mass = kAsteroids[a].mass;
+    ad.bd.radius = kAsteroids[a].radius;
+    ad.bd.j2 = 0.0;
+  }
+  if (c->d_t21) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->d_t21));
+    c->d_t21 = nullptr;
+  }
+  spk.t21_rec = nullptr;
+  spk.t21_ep = nullptr;
+  spk.t21_tab = nullptr;
+  if (nseg > 0) {
+    const size_t brec = align256(rec_h.size() * 8), bep = align256(ep_h.size() * 8);
+    HIPCHK(hipMalloc(&c->d_t21, brec + bep + tab_h.size() * 4));
+    char* base = (char*)c->d_t21;
+    HIPCHK(hipMemcpy(base, rec_h.data(), rec_h.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + brec, ep_h.data(), ep_h.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + brec + bep, tab_h.data(), tab_h.size() * 4, hipMemcpyHostToDevice));
+    spk.t21_rec = (const double*)base;
+    spk.t21_ep = (const double*)(base + brec);
+    spk.t21_tab = (const int32_t*)(base + brec + bep);
+  }
+  if (!c->d_spk) HIPCHK(hipMalloc(&c->d_spk, sizeof(SpkDev)));
+  HIPCHK(hipMemcpy(c->d_spk, &spk, sizeof(SpkDev), hipMemcpyHostToDevice));
+  c->spk = spk;
+  c->t21_obj = obj;
+  return KH_OK;
+}
+
 // Grow-only device buffer.
 int32_t kh_grow(void** buf, size_t* size, size_t bytes) {
   if (bytes > *size) {
@@ -282,15 +517,26 @@ static int32_t staging(kh_ctx* c, size_t bytes, char** out) {
   return r;
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 int32_t kh_spk_states(kh_ctx* c, int64_t obj_id, int64_t center, int32_t frame, int64_t m,
                       const double* jd, double* const out[6], int32_t* status) {
   if (!c || !jd || !out || !status || m < 0) return fail(KH_E_ARG, "null argument");
   if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
   if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
-  Chain body, cen;
-  if (!kh_build_chain(c, obj_id, body)) return fail(KH_E_NO_SPK, "no path from object to 0");
+  Chain body{}, cen{}, sun{};
+  // an object with Type 21 segments is read from them (kete keeps an
+  // object's segments in load order and takes the first covering the time,
+  // spk.rs:72-78); the planet tree otherwise
+  int32_t t21_begin = 0, t21_end = 0;
+  for (size_t q = 0; q < c->t21_obj.size(); ++q)
+    if (c->t21_obj[q] == obj_id) {
+      if (t21_end == 0) t21_begin = (int32_t)q;
+      t21_end = (int32_t)q + 1;
+    }
+  if (t21_end > t21_begin) {
+    if (!kh_build_chain(c, 10, sun)) return fail(KH_E_NO_SPK, "no path from the Sun to 0");
+  } else if (!kh_build_chain(c, obj_id, body)) {
+    return fail(KH_E_NO_SPK, "no path from object to 0");
+  }
   if (!kh_build_chain(c, center, cen)) return fail(KH_E_NO_SPK, "no path from center to 0");
   if (m == 0) return KH_OK;
   HIPCHK(hipSetDevice(c->device));
@@ -303,7 +549,8 @@ int32_t kh_spk_states(kh_ctx* c, int64_t obj_id, int64_t center, int32_t frame, 
   for (int k = 0; k < 6; ++k) d_out[k] = (double*)(base + bj + k * bo);
   int32_t* d_st = (int32_t*)(base + bj + 6 * bo);
   HIPCHK(hipMemcpyAsync(d_jd, jd, m * 8, hipMemcpyHostToDevice, c->stream));
-  if (launch_spk_states(c->d_spk, c->d_coef, body, cen, frame, m, d_jd, d_out, d_st, c->stream))
+  if (launch_spk_states(c->d_spk, c->d_coef, body, cen, t21_begin, t21_end, sun, center, frame, m,
+                        d_jd, d_out, d_st, c->stream))
     return fail(KH_E_HIP, "spk_states launch failed");
   for (int k = 0; k < 6; ++k)
     HIPCHK(hipMemcpyAsync(out[k], d_out[k], m * 8, hipMemcpyDeviceToHost, c->stream));
@@ -420,7 +667,8 @@ static int32_t check_nbody_args(kh_ctx* c, int64_t n, const double* jd0,
 // propagation (me != null).
 static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],
                          int64_t center, int32_t frame, const double* jd_final,
-                         int32_t jd_final_stride, const kh_nongrav* ng, double epsilon,
+                         int32_t jd_final_stride, int32_t include_asteroids,
+                         const kh_nongrav* ng, double epsilon,
                          int32_t mode, double* const out[6], int32_t* status, int64_t* impact_id,
                          double* impact_jd, kh_counters* counters, int32_t sync,
                          const kh::MultiEpochs* me) {
@@ -449,6 +697,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   p.counters = reinterpret_cast<int32_t*>(counters);
   p.queue = c->d_queue;
   p.mode = mode < 0 ? 0 : mode;
+  p.n_ast = include_asteroids ? N_ASTEROIDS : 0;
   p.block = 256;
   p.grid = 0;
   // LDS window of 16 records (62 KB for 11 segments) when all segments have
@@ -500,17 +749,19 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
 int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
                                 const double* const xyz[6], int64_t center, int32_t frame,
                                 const double* jd_final, int32_t jd_final_stride,
-                                const kh_nongrav* ng, double epsilon, int32_t mode,
-                                double* const out[6], int32_t* status, int64_t* impact_id,
-                                double* impact_jd, kh_counters* counters, int32_t sync) {
-  return nbody_dev(c, n, jd0, xyz, center, frame, jd_final, jd_final_stride, ng, epsilon, mode,
-                   out, status, impact_id, impact_jd, counters, sync, nullptr);
+                                int32_t include_asteroids, const kh_nongrav* ng, double epsilon,
+                                int32_t mode, double* const out[6], int32_t* status,
+                                int64_t* impact_id, double* impact_jd, kh_counters* counters,
+                                int32_t sync) {
+  return nbody_dev(c, n, jd0, xyz, center, frame, jd_final, jd_final_stride, include_asteroids,
+                   ng, epsilon, mode, out, status, impact_id, impact_jd, counters, sync, nullptr);
 }
 
 int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
                             const double* const xyz[6], int64_t center, int32_t frame,
                             const double* jd_final, int32_t jd_final_stride,
-                            const kh_nongrav* ng, double epsilon, int32_t mode,
+                            int32_t include_asteroids, const kh_nongrav* ng, double epsilon,
+                            int32_t mode,
                             double* const out[6], int32_t* status, int64_t* impact_id,
                             double* impact_jd, kh_counters* counters) {
   Chain ch;
@@ -555,9 +806,9 @@ int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
     HIPCHK(hipMemcpyAsync((void*)d_xyz[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_jf, jd_final, nf * 8, hipMemcpyHostToDevice, c->stream));
   if (ng) HIPCHK(hipMemcpyAsync(d_ng, ng, n * sizeof(kh_nongrav), hipMemcpyHostToDevice, c->stream));
-  r = kh_propagate_n_body_dev(c, n, d_jd0, d_xyz, center, frame, d_jf, jd_final_stride, d_ng,
-                              epsilon, mode, d_out, d_st, d_iid, d_ijd, counters ? d_cnt : nullptr,
-                              0);
+  r = kh_propagate_n_body_dev(c, n, d_jd0, d_xyz, center, frame, d_jf, jd_final_stride,
+                              include_asteroids, d_ng, epsilon, mode, d_out, d_st, d_iid, d_ijd,
+                              counters ? d_cnt : nullptr, 0);
   if (r) return r;
   for (int k = 0; k < 6; ++k)
     HIPCHK(hipMemcpyAsync(out[k], d_out[k], n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -591,12 +842,13 @@ double kh_last_kernel_ms(kh_ctx* c) {
 
 int32_t kh_propagate_n_body_epochs_dev(kh_ctx* c, int64_t n, const double* jd0,
                                        const double* const xyz[6], int64_t center, int32_t frame,
-                                       const kh::MultiEpochs& me, double epsilon,
-                                       double* const out[6], int32_t* status, int32_t sync) {
+                                       const kh::MultiEpochs& me, int32_t include_asteroids,
+                                       double epsilon, double* const out[6], int32_t* status,
+                                       int32_t sync) {
   if (me.n_epochs <= 0 || !me.epochs) return fail(KH_E_ARG, "no epochs");
   for (int k = 0; k < 6; ++k)
     if (!me.out[k]) return fail(KH_E_ARG, "null epoch output column");
-  return nbody_dev(c, n, jd0, xyz, center, frame, me.epochs, 0, nullptr, epsilon, 0, out, status,
-                   nullptr, nullptr, nullptr, sync, &me);
+  return nbody_dev(c, n, jd0, xyz, center, frame, me.epochs, 0, include_asteroids, nullptr,
+                   epsilon, 0, out, status, nullptr, nullptr, nullptr, sync, &me);
 }
 

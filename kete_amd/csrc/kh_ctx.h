@@ -26,6 +26,9 @@ struct kh_ctx {
   bool has_spk = false;
   double* d_coef = nullptr;
   std::vector<int64_t> seg_obj, seg_center;
+  // Type 21 segments (device copies in d_t21), object ids in device order
+  std::vector<int64_t> t21_obj;
+  void* d_t21 = nullptr;
   void* dbuf = nullptr;
   size_t dbuf_size = 0;
   void* obuf = nullptr;  // queue-order scratch of the N-body launch
@@ -55,5 +58,6 @@ int32_t kh_grow(void** buf, size_t* size, size_t bytes);
 // center and frame, NaN from a failing epoch on; out/status the final state.
 int32_t kh_propagate_n_body_epochs_dev(kh_ctx* c, int64_t n, const double* jd0,
                                        const double* const xyz[6], int64_t center, int32_t frame,
-                                       const kh::MultiEpochs& me, double epsilon,
+                                       const kh::MultiEpochs& me, int32_t include_asteroids,
+                                       double epsilon,
                                        double* const out[6], int32_t* status, int32_t sync);

@@ -69,6 +69,38 @@ struct BodyDev {
   double mass, radius, j2, pad;
 };
 
+// SPK Type 21 segments (spk_segments.rs:958-1102).  At upload each record's
+// modified-difference line is rewritten as three polynomials in the
+// normalised time tau = (t - t_ref) / S, S the largest |t - t_ref| over the
+// times the record serves: position = ref + dt (ref_vel + dt P(tau)),
+// velocity its derivative, P of degree < T21_NP.  Device record:
+// [t_ref, 1/S, x vx y vy z vz, P_x[T21_NP], P_y[T21_NP], P_z[T21_NP]] (km, s).
+constexpr int T21_NP = 16;
+constexpr int T21_REC = 8 + 3 * T21_NP;
+constexpr int MAX_T21_SEGS = 32;
+// kete's MASSES after PLANETS: Ceres, Pallas, Vesta, Hygiea, Interamnia
+// (gravity.rs:103-134).
+constexpr int N_ASTEROIDS = 5;
+
+struct T21SegDev {
+  double jd_start, jd_end;  // JD TDB, as kete converts the summary floats
+  double t0, inv_bw;        // bucket grid over the final epochs: origin (s), 1/width
+  int64_t rec_off;          // first record (in records) in t21_rec
+  int64_t ep_off;           // first final epoch in t21_ep
+  int64_t tab_off;          // first bucket in t21_tab
+  int32_t n_rec, n_bucket;
+  int32_t ecliptic;     // frame 17: rotated to equatorial after evaluation
+  int32_t sun_centred;  // center 10: the Sun's barycentric state is added
+};
+
+// A MASSES asteroid: its Type 21 segments t21[seg_begin, seg_end) in load
+// order (kete scans an object's segments in load order, spk.rs:72-78) and
+// its Newtonian parameters (kind 0).
+struct AstDev {
+  int32_t seg_begin, seg_end, pad0, pad1;
+  BodyDev bd;
+};
+
 // Device-resident ephemeris description; kernels take a pointer to it, so the
 // per-body and per-segment fields are read with scalar loads.
 struct SpkDev {
@@ -80,6 +112,13 @@ struct SpkDev {
   int32_t sun_seg;
   SegDev seg[MAX_SEGS];
   BodyDev body[N_PLANETS];
+  // Type 21 segments of any object, and the MASSES asteroids among them
+  const double* __restrict__ t21_rec;   // [records][T21_REC]
+  const double* __restrict__ t21_ep;    // final epochs, s past J2000
+  const int32_t* __restrict__ t21_tab;  // first candidate record of each bucket
+  int32_t n_t21_seg, pad;
+  T21SegDev t21[MAX_T21_SEGS];
+  AstDev ast[N_ASTEROIDS];
 };
 
 struct Vec3 {
@@ -274,6 +313,85 @@ __device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk,
                                                const double* __restrict__ coef, int s, double jd,
                                                Vec3& pos, Vec3& vel, int32_t win = 0) {
   return seg_eval<true>(spk, coef, s, jd, pos, vel, win);
+}
+
+// Record of Type 21 segment g serving js (s past J2000): the first whose
+// final epoch is >= js (slice::binary_search_by in spk_segments.rs:1023-1030),
+// clamped to the last record (kete reads past the array there).  The bucket
+// table gives a record at or before that one; a short forward scan ends on it.
+__device__ __forceinline__ int32_t t21_record(const SpkDev* __restrict__ spk,
+                                              const T21SegDev& g, double js) {
+  const double u = (js - g.t0) * g.inv_bw;
+  const int32_t c = u > 0.0 ? (int32_t)fmin(u, (double)(g.n_bucket - 1)) : 0;
+  int32_t i = spk->t21_tab[g.tab_off + c];
+  const double* __restrict__ ep = spk->t21_ep + g.ep_off;
+  while (i < g.n_rec - 1 && ep[i] < js) ++i;
+  return (int32_t)g.rec_off + i;
+}
+
+// One Type 21 record at js: position in AU and, if VEL, velocity in AU/day
+// (spk_segments.rs:1088-1100 divide by AU_KM and scale by 86400 s/day).
+template <bool VEL>
+__device__ __forceinline__ void t21_eval(const double* __restrict__ r, double js, Vec3& pos,
+                                         Vec3& vel) {
+  const double dt = js - r[0];
+  const double tau = dt * r[1];
+  double p[3], d[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* __restrict__ q = r + 8 + c * T21_NP;
+    double acc = q[T21_NP - 1];
+#pragma unroll
+    for (int m = T21_NP - 2; m >= 0; --m) acc = acc * tau + q[m];
+    p[c] = acc;
+    if (VEL) {
+      double der = (double)(T21_NP - 1) * q[T21_NP - 1];
+#pragma unroll
+      for (int m = T21_NP - 2; m >= 1; --m) der = der * tau + (double)m * q[m];
+      d[c] = der;
+    }
+  }
+  pos = Vec3{(r[2] + dt * (p[0] * dt + r[3])) * INV_AU_KM,
+             (r[4] + dt * (p[1] * dt + r[5])) * INV_AU_KM,
+             (r[6] + dt * (p[2] * dt + r[7])) * INV_AU_KM};
+  if (VEL) {
+    const double sc = 86400.0 * INV_AU_KM;
+    vel = Vec3{(r[3] + dt * (2.0 * p[0] + dt * d[0] * r[1])) * sc,
+               (r[5] + dt * (2.0 * p[1] + dt * d[1] * r[1])) * sc,
+               (r[7] + dt * (2.0 * p[2] + dt * d[2] * r[1])) * sc};
+  }
+}
+
+// State of the object whose Type 21 segments are t21[begin, end), at JD t,
+// relative to the segment's center and in the equatorial frame: the first
+// segment containing t (spk.rs:72-78, SpkSegment::contains), then its record.
+// When the whole wavefront reads one record, its address is uniform and the
+// coefficients arrive through scalar loads.
+template <bool VEL>
+__device__ __forceinline__ int32_t t21_state(const SpkDev* __restrict__ spk, int32_t begin,
+                                             int32_t end, double t, Vec3& pos, Vec3& vel,
+                                             bool& sun_centred) {
+  int32_t s = -1;
+  for (int32_t q = begin; q < end; ++q) {
+    const T21SegDev& g = spk->t21[q];
+    if (s < 0 && !(t < g.jd_start || t > g.jd_end)) s = q;
+  }
+  if (s < 0) return ST_DAF_LIMITS;
+  const T21SegDev& g = spk->t21[s];
+  const double js = (t - 2451545.0) * 86400.0;
+  const int32_t ri = t21_record(spk, g, js);
+  const int32_t ru = __builtin_amdgcn_readfirstlane(ri);
+  Vec3 p, v{0.0, 0.0, 0.0};
+  if (__all(ri == ru)) t21_eval<VEL>(spk->t21_rec + (int64_t)ru * T21_REC, js, p, v);
+  else t21_eval<VEL>(spk->t21_rec + (int64_t)ri * T21_REC, js, p, v);
+  if (g.ecliptic) {
+    p = ecl_to_equ(p);
+    if (VEL) v = ecl_to_equ(v);
+  }
+  pos = p;
+  vel = v;
+  sun_centred = g.sun_centred != 0;
+  return ST_OK;
 }
 
 // Sum of a chain of segment states (the center-chaining of spk.rs:99-126).
@@ -599,6 +717,32 @@ __device__ __forceinline__ int32_t add_body_at(const BodyDev& bd, Vec3 bp, Vec3 
   return ST_OK;
 }
 
+// The asteroid terms of spk_accel over MASSES (acceleration.rs:145-160 for
+// MASSES[10..15], gravity.rs:103-134; propagation/mod.rs:82-88): each
+// asteroid's barycentric position -- its Type 21 state plus, for a
+// Sun-centred segment, the Sun's (try_get_state to center 0, spk.rs:87-106)
+// -- and a Newtonian term, in MASSES order after the planets.  sun is the
+// Sun's barycentric position at t.
+__device__ __forceinline__ int32_t add_asteroids(const SpkDev* __restrict__ spk, double t,
+                                                 Vec3 sun, Vec3 x, Vec3 v, bool exact, Vec3& a,
+                                                 int32_t& impact_id) {
+#pragma unroll 1
+  for (int k = 0; k < N_ASTEROIDS; ++k) {
+    const AstDev& ad = spk->ast[k];
+    Vec3 p, unused;
+    bool sun_centred = false;
+    int32_t st = t21_state<false>(spk, ad.seg_begin, ad.seg_end, t, p, unused, sun_centred);
+    if (st) return st;
+    if (sun_centred) p = Vec3{p.x + sun.x, p.y + sun.y, p.z + sun.z};
+    st = add_body_at(ad.bd, p, Vec3{0.0, 0.0, 0.0}, x, v, exact, nullptr, a);
+    if (st) {
+      impact_id = ad.bd.id;
+      return st;
+    }
+  }
+  return ST_OK;
+}
+
 __device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
                                             const double* __restrict__ coef, const BodyDev& bd,
                                             double t, Vec3 x, Vec3 v, bool exact,
@@ -628,10 +772,10 @@ __device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
 template <bool UNI>
 __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ spk,
                                                      const double* __restrict__ base,
-                                                     int32_t slot, int32_t win, double xs,
-                                                     double inv_radius, Vec3 x, Vec3 v,
+                                                     int32_t slot, int32_t win, double t,
+                                                     double xs, double inv_radius, Vec3 x, Vec3 v,
                                                      bool exact, const NonGravDev* ng, Vec3& acc,
-                                                     int32_t& impact_id) {
+                                                     int32_t& impact_id, int32_t n_ast) {
   const double vscale = 86400.0 * inv_radius * INV_AU_KM;
   auto coefs = [&](int s) -> const double* {
     if (UNI) return base + spk->seg[s].off;
@@ -665,6 +809,12 @@ __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ 
       return st;
     }
   }
+  if (n_ast) {
+    const Vec3 sp = cheb_pos_n<14>(coefs(spk->sun_seg), xs);
+    const Vec3 sun{sp.x * INV_AU_KM, sp.y * INV_AU_KM, sp.z * INV_AU_KM};
+    const int32_t st = add_asteroids(spk, t, sun, x, v, exact, a, impact_id);
+    if (st) return st;
+  }
   acc = a;
   return ST_OK;
 }
@@ -685,7 +835,7 @@ __device__ __forceinline__ int32_t spk_accel_ilp(const SpkDev* __restrict__ spk,
                                                  const double* __restrict__ coef, double t,
                                                  Vec3 x, Vec3 v, bool exact,
                                                  const NonGravDev* ng, Vec3& acc,
-                                                 int32_t& impact_id) {
+                                                 int32_t& impact_id, int32_t n_ast) {
   const SegDev& g = spk->seg[spk->emb_seg];
   if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
   const double js = (t - 2451545.0) * 86400.0;
@@ -743,18 +893,25 @@ __device__ __forceinline__ int32_t spk_accel_ilp(const SpkDev* __restrict__ spk,
       return st;
     }
   }
+  if (n_ast) {
+    const Vec3 sun = pos(coef + spk->seg[spk->sun_seg].off + roff + 2);
+    const int32_t st = add_asteroids(spk, t, sun, x, v, exact, a, impact_id);
+    if (st) return st;
+  }
   acc = a;
   return ST_OK;
 }
 
 // spk_accel: acceleration of a massless body at barycentric equatorial
-// (x, v) and TDB JD t, summed over kete's PLANETS in its order.  The
-// Earth-Moon barycentre is evaluated once and shared by 399 and 301 (kete
-// evaluates it for each, with identical results).
+// (x, v) and TDB JD t, summed over kete's PLANETS in its order, then, with
+// n_ast != 0, the five asteroids that complete MASSES.  The Earth-Moon
+// barycentre is evaluated once and shared by 399 and 301 (kete evaluates it
+// for each, with identical results).
 __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                              const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
-                                             int32_t& impact_id, int32_t win = 0) {
+                                             int32_t& impact_id, int32_t win = 0,
+                                             int32_t n_ast = 0) {
   if (spk->shared_grid) {
     const SegDev& g = spk->seg[spk->emb_seg];
     if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
@@ -765,16 +922,17 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     if (__all(ri == ru)) {
       const double* rec = coef + g.off + (int64_t)ru * 44;
       const double ir = 1.0 / rec[1];
-      return accel_shared_grid<true>(spk, coef + (int64_t)ru * 44 + 2, 0, 0,
-                                     (js - rec[0]) * ir, ir, x, v, exact, ng, acc, impact_id);
+      return accel_shared_grid<true>(spk, coef + (int64_t)ru * 44 + 2, 0, 0, t,
+                                     (js - rec[0]) * ir, ir, x, v, exact, ng, acc, impact_id,
+                                     n_ast);
     }
 #endif
     const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
     if (win > 0 && __all(loc >= 0 && loc < win)) {
       const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
       const double ir = r0[1];  // staged as 1 / radius
-      return accel_shared_grid<false>(spk, nullptr, loc, win, (js - r0[0]) * ir, ir, x, v,
-                                      exact, ng, acc, impact_id);
+      return accel_shared_grid<false>(spk, nullptr, loc, win, t, (js - r0[0]) * ir, ir, x, v,
+                                      exact, ng, acc, impact_id, n_ast);
     }
   }
   Vec3 a{0.0, 0.0, 0.0};
@@ -789,6 +947,13 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
       if (st == ST_IMPACT) impact_id = bd.id;
       return st;
     }
+  }
+  if (n_ast) {
+    Vec3 sun;
+    st = seg_pos(spk, coef, spk->sun_seg, t, sun, win);
+    if (st) return st;
+    st = add_asteroids(spk, t, sun, x, v, exact, a, impact_id);
+    if (st) return st;
   }
   acc = a;
   return ST_OK;

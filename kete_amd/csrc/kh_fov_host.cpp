@@ -85,8 +85,8 @@ void kh_fov_rect_normals(const double pointing[3], double rotation, double lon_w
 int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const xyz[6],
                            int32_t frame, int64_t n_fields, const double* field_jd,
                            const double* const obs[6], const double* normals, double dt_limit,
-                           int64_t cap, int64_t* n_hits, int64_t* hit_field, int64_t* hit_obj,
-                           double* const hit_state[7]) {
+                           int32_t include_asteroids, int64_t cap, int64_t* n_hits,
+                           int64_t* hit_field, int64_t* hit_obj, double* const hit_state[7]) {
   if (!c || !n_hits) return kh_fail(KH_E_ARG, "null argument");
   *n_hits = 0;
   if (n < 0 || n_fields < 0 || cap < 0) return kh_fail(KH_E_ARG, "negative size");
@@ -230,7 +230,8 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
     auto propagate = [&](double* const* x, double* xjd, double jd_target) -> int32_t {
       HIPCHK(hipMemcpyAsync(d_scalar, &jd_target, 8, hipMemcpyHostToDevice, s));
       const double* xin[6] = {x[0], x[1], x[2], x[3], x[4], x[5]};
-      int32_t rr = kh_propagate_n_body_dev(c, n, xjd, xin, 10, frame, d_scalar, 0, nullptr, 1e-6,
+      int32_t rr = kh_propagate_n_body_dev(c, n, xjd, xin, 10, frame, d_scalar, 0, include_asteroids,
+                                           nullptr, 1e-6,
                                            -1, x, st, nullptr, nullptr, nullptr, 0);
       if (rr) return rr;
       hipLaunchKernelGGL(fill_f64, dim3(gb), dim3(256), 0, s, xjd, n, jd_target);
@@ -272,7 +273,8 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
             me.hi = *std::max_element(epochs.begin() + w0, epochs.begin() + w1);
             for (int k = 0; k < 6; ++k) me.out[k] = m_out[k];
             const double* xin[6] = {cur[0], cur[1], cur[2], cur[3], cur[4], cur[5]};
-            r = kh_propagate_n_body_epochs_dev(c, n, cur_jd, xin, 10, frame, me, 1e-6, fin, st,
+            r = kh_propagate_n_body_epochs_dev(c, n, cur_jd, xin, 10, frame, me, include_asteroids,
+                                               1e-6, fin, st,
                                                0);
             if (r) return r;
           }
@@ -373,7 +375,8 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
         for (int k = 0; k < 7; ++k) nb.hit_state[k] = h_state[k];
         if (launch_fov_nb_gather(nb, s)) return kh_fail(KH_E_HIP, "gather launch failed");
         const double* gx[6] = {g_x[0], g_x[1], g_x[2], g_x[3], g_x[4], g_x[5]};
-        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, nullptr, 1e-6, -1,
+        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, include_asteroids,
+                                    nullptr, 1e-6, -1,
                                     g_out, g_st, nullptr, nullptr, nullptr, 0);
         if (r) return r;
         if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");

@@ -62,6 +62,8 @@ struct NBodyLaunch {
   int32_t n_epochs;
   double ep_lo, ep_hi;
   double* mout[6];
+  // force list: 0 PLANETS, N_ASTEROIDS MASSES (include_asteroids)
+  int32_t n_ast;
 };
 
 // Shared epochs of a multi-epoch propagation (device array, integration
@@ -100,9 +102,14 @@ struct TwoBodyLaunch {
 int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaunch& p,
                         hipStream_t stream);
 
-int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center, int32_t frame, int64_t m,
-                          const double* jd, double* const out[6], int32_t* status,
-                          hipStream_t stream);
+// State of `body` relative to `center` (NAIF id center_id) at m epochs.
+// With t21_begin < t21_end the body is the object whose Type 21 segments are
+// spk.t21[t21_begin, t21_end) instead, and `sun` the Sun's chain (added for
+// Sun-centred segments when the center differs).
+int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center,
+                          int32_t t21_begin, int32_t t21_end, Chain sun, int64_t center_id,
+                          int32_t frame, int64_t m, const double* jd, double* const out[6],
+                          int32_t* status, hipStream_t stream);
 
 // One rectangular field: epoch, observer state (center 10, same frame as the
 // objects) and the four edge normals (fov/patches.rs:122-155).

@@ -86,16 +86,38 @@ int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaun
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// State of a body chain relative to a center chain, rotated to `frame`.
+// State of a body chain (or a Type 21 object) relative to a center chain,
+// rotated to `frame`.
 __global__ void spk_states_kernel(const SpkDev* __restrict__ spk,
-                                  const double* __restrict__ coef, Chain body, Chain center, int32_t frame, int64_t m,
-                                  const double* jd, double* o0, double* o1, double* o2, double* o3,
-                                  double* o4, double* o5, int32_t* status) {
+                                  const double* __restrict__ coef, Chain body, Chain center,
+                                  int32_t t21_begin, int32_t t21_end, Chain sun,
+                                  int64_t center_id, int32_t frame, int64_t m, const double* jd, double* o0, double* o1, double* o2,
+                                  double* o3, double* o4, double* o5, int32_t* status) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
   Vec3 bp, bv, cp, cv;
-  int32_t st = chain_state(spk, coef, body, jd[q], bp, bv);
-  if (st == ST_OK) st = chain_state(spk, coef, center, jd[q], cp, cv);
+  int32_t st;
+  bool same_center = false;
+  if (t21_begin < t21_end) {
+    bool sun_centred = false;
+    st = t21_state<true>(spk, t21_begin, t21_end, jd[q], bp, bv, sun_centred);
+    // kete changes the center only when it differs from the segment's
+    // (SpkCollection::try_get_state, spk.rs:87-96)
+    same_center = (sun_centred ? 10 : 0) == center_id;
+    if (st == ST_OK && sun_centred && !same_center) {
+      Vec3 sp, sv;
+      st = chain_state(spk, coef, sun, jd[q], sp, sv);
+      bp = Vec3{bp.x + sp.x, bp.y + sp.y, bp.z + sp.z};
+      bv = Vec3{bv.x + sv.x, bv.y + sv.y, bv.z + sv.z};
+    }
+  } else {
+    st = chain_state(spk, coef, body, jd[q], bp, bv);
+  }
+  if (same_center) {
+    cp = cv = Vec3{0.0, 0.0, 0.0};
+  } else if (st == ST_OK) {
+    st = chain_state(spk, coef, center, jd[q], cp, cv);
+  }
   Vec3 p{bp.x - cp.x, bp.y - cp.y, bp.z - cp.z};
   Vec3 v{bv.x - cv.x, bv.y - cv.y, bv.z - cv.z};
   if (frame == 17) {
@@ -113,14 +135,16 @@ __global__ void spk_states_kernel(const SpkDev* __restrict__ spk,
   status[q] = st;
 }
 
-int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center, int32_t frame, int64_t m,
-                          const double* jd, double* const out[6], int32_t* status,
-                          hipStream_t stream) {
+int32_t launch_spk_states(const SpkDev* spk, const double* coef, Chain body, Chain center,
+                          int32_t t21_begin, int32_t t21_end, Chain sun, int64_t center_id,
+                          int32_t frame, int64_t m, const double* jd, double* const out[6],
+                          int32_t* status, hipStream_t stream) {
   if (m <= 0) return 0;
   const int block = 256;
   hipLaunchKernelGGL(spk_states_kernel, dim3((unsigned)((m + block - 1) / block)), dim3(block), 0,
-                     stream, spk, coef, body, center, frame, m, jd, out[0], out[1], out[2], out[3],
-                     out[4], out[5], status);
+                     stream, spk, coef, body, center, t21_begin, t21_end, sun, center_id, frame, m,
+                     jd,
+                     out[0], out[1], out[2], out[3], out[4], out[5], status);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

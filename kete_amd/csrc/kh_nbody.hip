@@ -22,6 +22,8 @@
 //  - Persistent mode: the grid is sized to the machine and a lane that
 //    finishes an object takes the next one from a device-side counter, so
 //    objects with very different step counts do not leave lanes idle.
+#include <type_traits>
+
 #include "kh_device.h"
 #include "kh_internal.h"
 
@@ -82,6 +84,7 @@ struct NBodyArgs {
   const double* epochs;
   int32_t n_epochs;
   double* mout[6];
+  int32_t n_ast;  // asteroids after the planets in the force list (0 or 5)
 };
 
 // The slot range of this launch (uniform: read once per lane from the
@@ -388,7 +391,10 @@ struct LdsTables {
 // and per SIMD, the whole register file, the force evaluation arranged for
 // the shortest dependency chain (spk_accel_ilp), no LDS window.
 // MULTI: the multi-epoch propagation (plain launch only).
-template <bool PERSISTENT, bool ILP = false, bool MULTI = false>
+// AST: the force list is MASSES (the planets and five asteroids) instead of
+// PLANETS; a separate instantiation, so the planets-only kernels carry no
+// asteroid code.
+template <bool PERSISTENT, bool ILP = false, bool MULTI = false, bool AST = false>
 __global__ void __launch_bounds__(ILP ? 64 : 256)
 __attribute__((amdgpu_waves_per_eu(ILP ? 1 : 2, ILP ? 1 : 8)))
 radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef,
@@ -478,10 +484,10 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     int32_t st;
     if constexpr (ILP)
       st = spk_accel_ilp(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]},
-                         !sub, ngp, acc, impact);
+                         !sub, ngp, acc, impact, AST ? N_ASTEROIDS : 0);
     else
       st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
-                     ngp, acc, impact, winw);
+                     ngp, acc, impact, winw, AST ? N_ASTEROIDS : 0);
     const double ev[3] = {acc.x, acc.y, acc.z};
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
@@ -720,7 +726,7 @@ __device__ __forceinline__ int64_t fetch3(const NBodyArgs& a, int64_t slot, int 
   return a.n;
 }
 
-template <bool PERSISTENT>
+template <bool PERSISTENT, bool AST = false>
 __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restrict__ spk,
                                                            const double* __restrict__ coef,
                                                            NBodyArgs a) {
@@ -805,6 +811,24 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
       if (b >= 0 && key == 0x7fffffff) {
         const int32_t sb = add_body(spk, coef, spk->body[b], te, X, V, !sub, ngp, emb, part, winw);
         if (sb) key = (b << 4) | sb;
+      }
+    }
+    // the MASSES asteroids (kete indices 10..14) on the lane with the fewest bodies
+    if (AST && c == 2 && key == 0x7fffffff) {
+      Vec3 sun;
+      const int32_t ss = seg_pos(spk, coef, spk->sun_seg, te, sun, winw);
+      if (ss) key = ss;  // the Sun (index 0) was read first
+#pragma unroll 1
+      for (int k = 0; k < N_ASTEROIDS && key == 0x7fffffff; ++k) {
+        const AstDev& ad = spk->ast[k];
+        Vec3 ap, unused;
+        bool sun_centred = false;
+        int32_t sa = t21_state<false>(spk, ad.seg_begin, ad.seg_end, te, ap, unused, sun_centred);
+        if (sa == ST_OK) {
+          if (sun_centred) ap = Vec3{ap.x + sun.x, ap.y + sun.y, ap.z + sun.z};
+          sa = add_body_at(ad.bd, ap, Vec3{0.0, 0.0, 0.0}, X, V, !sub, nullptr, part);
+        }
+        if (sa) key = ((N_PLANETS + k) << 4) | sa;
       }
     }
     const int kmin = min(min(gshfl(key, base), gshfl(key, base + 1)), gshfl(key, base + 2));
@@ -921,7 +945,9 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
       if (c == 0) {
         a.status[obj] = done;
         if (a.impact_id)
-          a.impact_id[obj] = done == ST_IMPACT ? spk->body[kmin >> 4].id : 0;
+          a.impact_id[obj] = done != ST_IMPACT ? 0
+                             : (kmin >> 4) < N_PLANETS ? spk->body[kmin >> 4].id
+                                                       : spk->ast[(kmin >> 4) - N_PLANETS].bd.id;
         if (a.impact_jd) a.impact_jd[obj] = done == ST_IMPACT ? te : nan;
         if (a.counters)
           reinterpret_cast<int4*>(a.counters)[obj] = int4{n_force, n_sweep, n_acc, n_fail};
@@ -967,6 +993,14 @@ static int64_t resident_grid(K kernel, int block, size_t lds_bytes, int32_t n_cu
   return (int64_t)per_cu * (n_cu > 0 ? n_cu : 1);
 }
 
+// Calls f with std::true_type when the force list includes the asteroids,
+// std::false_type otherwise (selects the kernel instantiation).
+template <class F>
+static void with_ast(int32_t n_ast, F&& f) {
+  if (n_ast) f(std::true_type{});
+  else f(std::false_type{});
+}
+
 int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
                      hipStream_t stream, int32_t* used_block, int32_t* used_grid,
                      int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1) {
@@ -998,6 +1032,7 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   const size_t lds_bytes = a.window > 0 ? (size_t)p.n_seg * a.window * 44 * sizeof(double) : 0;
   const int block = p.block > 0 ? p.block : 256;
   const unsigned g1 = (unsigned)((p.n + 255) / 256);
+  a.n_ast = p.n_ast;
   a.epochs = p.epochs;
   a.n_epochs = p.n_epochs;
   for (int k = 0; k < 6; ++k) a.mout[k] = p.mout[k];
@@ -1036,12 +1071,15 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     a.first_dynamic = grid * (block / 64) * 21;
     if (persistent) hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
     if (ev0) (void)hipEventRecord(ev0, stream);
-    if (persistent)
-      hipLaunchKernelGGL(radau3_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), lds_bytes,
-                         stream, spk, coef, a);
-    else
-      hipLaunchKernelGGL(radau3_nbody_kernel<false>, dim3((unsigned)grid), dim3(block),
-                         lds_bytes, stream, spk, coef, a);
+    with_ast(p.n_ast, [&](auto ast) {
+      constexpr bool A = decltype(ast)::value;
+      if (persistent)
+        hipLaunchKernelGGL((radau3_nbody_kernel<true, A>), dim3((unsigned)grid), dim3(block),
+                           lds_bytes, stream, spk, coef, a);
+      else
+        hipLaunchKernelGGL((radau3_nbody_kernel<false, A>), dim3((unsigned)grid), dim3(block),
+                           lds_bytes, stream, spk, coef, a);
+    });
     if (ev1) (void)hipEventRecord(ev1, stream);
   } else if (p.mode == 1) {
     grid = p.grid > 0 ? p.grid
@@ -1050,8 +1088,11 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     a.first_dynamic = grid * block;
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, stream, p.queue);
     if (ev0) (void)hipEventRecord(ev0, stream);
-    hipLaunchKernelGGL(radau_nbody_kernel<true>, dim3((unsigned)grid), dim3(block), lds_bytes,
-                       stream, spk, coef, a);
+    with_ast(p.n_ast, [&](auto ast) {
+      constexpr bool A = decltype(ast)::value;
+      hipLaunchKernelGGL((radau_nbody_kernel<true, false, false, A>), dim3((unsigned)grid),
+                         dim3(block), lds_bytes, stream, spk, coef, a);
+    });
     if (ev1) (void)hipEventRecord(ev1, stream);
   } else {
     a.first_dynamic = p.n;
@@ -1069,19 +1110,26 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       const int64_t k_max = p.latency_k < p.n ? p.latency_k : p.n;
       (void)hipEventRecord(p.fork, stream);
       (void)hipStreamWaitEvent(p.stream2, p.fork, 0);
-      hipLaunchKernelGGL((radau_nbody_kernel<false, true>), dim3((unsigned)((k_max + 63) / 64)),
-                         dim3(64), 0, p.stream2, spk, coef, al);
+      with_ast(p.n_ast, [&](auto ast) {
+        constexpr bool A = decltype(ast)::value;
+        hipLaunchKernelGGL((radau_nbody_kernel<false, true, false, A>),
+                           dim3((unsigned)((k_max + 63) / 64)), dim3(64), 0, p.stream2, spk,
+                           coef, al);
+      });
       (void)hipEventRecord(p.join, p.stream2);
       a.split = p.split;
       a.part = 2;
     }
     grid = (p.n + block - 1) / block;
-    if (multi)
-      hipLaunchKernelGGL((radau_nbody_kernel<false, false, true>), dim3((unsigned)grid),
-                         dim3(block), lds_bytes, stream, spk, coef, a);
-    else
-      hipLaunchKernelGGL(radau_nbody_kernel<false>, dim3((unsigned)grid), dim3(block),
-                         lds_bytes, stream, spk, coef, a);
+    with_ast(p.n_ast, [&](auto ast) {
+      constexpr bool A = decltype(ast)::value;
+      if (multi)
+        hipLaunchKernelGGL((radau_nbody_kernel<false, false, true, A>), dim3((unsigned)grid),
+                           dim3(block), lds_bytes, stream, spk, coef, a);
+      else
+        hipLaunchKernelGGL((radau_nbody_kernel<false, false, false, A>), dim3((unsigned)grid),
+                           dim3(block), lds_bytes, stream, spk, coef, a);
+    });
     if (hybrid) (void)hipStreamWaitEvent(stream, p.join, 0);
     if (ev1) (void)hipEventRecord(ev1, stream);
   }

@@ -81,8 +81,6 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
     prepared tuple ``(jd[nf], obs[6, nf], normals[nf, 12])`` in the ecliptic
     frame.
     """
-    if include_asteroids:
-        raise NotImplementedError("include_asteroids needs SPK Type 21 perturbers")
     from . import get_context
     ctx = ctx or get_context()
     L = ctx.L
@@ -120,7 +118,8 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
         rc = L.kh_fov_state_check(ctx.h, n, jd0, _lib.ptr_array(list(obj_state.xyz)),
                                   FRAME_ECLIPTIC, nf, _lib.dptr(fjd),
                                   _lib.ptr_array([obs[k] for k in range(6)]),
-                                  _lib.dptr(nrm.reshape(-1)), float(dt_limit), cap, C.byref(nh),
+                                  _lib.dptr(nrm.reshape(-1)), float(dt_limit),
+                                  int(bool(include_asteroids)), cap, C.byref(nh),
                                   _lib.iptr(hf, C.c_int64), _lib.iptr(ho, C.c_int64),
                                   _lib.ptr_array([hs[k] for k in range(7)]))
         if rc == -1 and nh.value > cap:

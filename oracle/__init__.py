@@ -89,6 +89,9 @@ def lib():
         L.ko_fov_state_check.restype = I64
         L.ko_fov_state_check.argtypes = [I64, D, PP, I32, I64, P, P, P, D, I64,
                                          C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+        L.ko_fov_state_check_ext.restype = I64
+        L.ko_fov_state_check_ext.argtypes = [I64, D, PP, I32, I64, P, P, P, D, C.c_int, I64,
+                                             C.POINTER(I64), C.POINTER(I64), P, C.c_int]
         L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
         L.ko_hg_apparent_mag.restype = D
         L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
@@ -270,7 +273,7 @@ def fov_check_visible(normals, obs_jd, obs_pos, obs_vel, jd, xyz, dt_limit=3.0,
 
 
 def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
-                    frame=FRAME_ECLIPTIC, cap=None, n_threads=None):
+                    frame=FRAME_ECLIPTIC, cap=None, n_threads=None, include_asteroids=False):
     xyz = _soa(xyz)
     n = xyz.shape[1]
     field_jd = np.ascontiguousarray(field_jd, dtype=np.float64)
@@ -281,11 +284,11 @@ def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
     hf = np.empty(cap, dtype=np.int64)
     ho = np.empty(cap, dtype=np.int64)
     hs = np.empty((cap, 7))
-    nh = lib().ko_fov_state_check(n, jd0, _ptrs(list(xyz)), frame, nf, _d(field_jd),
-                                  _d(obs_xyz), _d(nrm), dt_limit, cap,
-                                  hf.ctypes.data_as(C.POINTER(C.c_int64)),
-                                  ho.ctypes.data_as(C.POINTER(C.c_int64)), _d(hs),
-                                  n_threads or os.cpu_count())
+    nh = lib().ko_fov_state_check_ext(n, jd0, _ptrs(list(xyz)), frame, nf, _d(field_jd),
+                                      _d(obs_xyz), _d(nrm), dt_limit, int(include_asteroids),
+                                      cap, hf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      ho.ctypes.data_as(C.POINTER(C.c_int64)), _d(hs),
+                                      n_threads or os.cpu_count())
     if nh < 0:
         raise RuntimeError("hit capacity exceeded")
     return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()

@@ -1333,6 +1333,7 @@ typedef struct {
     double jd;
     double pos[3], vel[3];
     int32_t frame;
+    int include_asteroids;  /* check_visible's flag (fov_like.rs:137) */
 } fov_t;
 
 /* check_linear (fov_like.rs:43-66); only the Contains result is used. */
@@ -1366,7 +1367,7 @@ static int check_two_body(const fov_t *f, const ko_state *s, int *inside,
 static int check_n_body(const fov_t *f, const ko_state *s, int *inside,
                         double *dist, ko_state *out) {
     ko_state es = *s;
-    int rc = ko_propagate_n_body(&es, f->jd, NULL, NULL, NULL, NULL);
+    int rc = ko_propagate_n_body_ext(&es, f->jd, f->include_asteroids, NULL, NULL, NULL, NULL);
     if (rc) return rc;
     double d[3];
     for (int i = 0; i < 3; ++i) d[i] = es.pos[i] - f->pos[i];
@@ -1427,7 +1428,7 @@ int64_t ko_fov_check_visible(const double normals[12], double obs_jd,
                              int64_t *vis_idx, double *const *vis_state,
                              int n_threads) {
     fov_t f = {normals, obs_jd, {obs_pos[0], obs_pos[1], obs_pos[2]},
-               {obs_vel[0], obs_vel[1], obs_vel[2]}, frame};
+               {obs_vel[0], obs_vel[1], obs_vel[2]}, frame, 0};
     int8_t *vis = calloc(n > 0 ? n : 1, 1);
     ko_state *res = calloc(n > 0 ? n : 1, sizeof(ko_state));
     cv_ctx c = {&f, jd, xyz, dt_limit, frame, vis, res};
@@ -1458,13 +1459,16 @@ typedef struct {
     ko_state *st;
     int8_t *alive;
     double jd;
+    int include_asteroids;
 } nb_prop_ctx;
 
 static void prop_range(int64_t lo, int64_t hi, void *vctx) {
     nb_prop_ctx *c = vctx;
     for (int64_t i = lo; i < hi; ++i) {
         if (!c->alive[i]) continue;
-        if (ko_propagate_n_body(&c->st[i], c->jd, NULL, NULL, NULL, NULL)) c->alive[i] = 0;
+        if (ko_propagate_n_body_ext(&c->st[i], c->jd, c->include_asteroids, NULL, NULL, NULL,
+                                    NULL))
+            c->alive[i] = 0;
     }
 }
 
@@ -1489,12 +1493,12 @@ static void grp_range(int64_t lo, int64_t hi, void *vctx) {
     }
 }
 
-int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
-                           int32_t frame, int64_t n_fields,
-                           const double *field_jd, const double *obs_xyz,
-                           const double *normals, double dt_limit, int64_t cap,
-                           int64_t *hit_field, int64_t *hit_obj, double *hit_state,
-                           int n_threads) {
+int64_t ko_fov_state_check_ext(int64_t n, double jd0, const double *const *xyz,
+                               int32_t frame, int64_t n_fields,
+                               const double *field_jd, const double *obs_xyz,
+                               const double *normals, double dt_limit, int include_asteroids,
+                               int64_t cap, int64_t *hit_field, int64_t *hit_obj,
+                               double *hit_state, int n_threads) {
     ko_state *states = malloc(sizeof(ko_state) * (n > 0 ? n : 1));
     ko_state *big = malloc(sizeof(ko_state) * (n > 0 ? n : 1));
     int8_t *alive = malloc(n > 0 ? n : 1), *big_alive = malloc(n > 0 ? n : 1);
@@ -1513,6 +1517,7 @@ int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
             fovs[f].vel[k] = obs_xyz[6 * f + 3 + k];
         }
         fovs[f].frame = frame;
+        fovs[f].include_asteroids = include_asteroids;
     }
     double jd = jd0, big_jd = jd0;
     int64_t nh = 0;
@@ -1524,7 +1529,7 @@ int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
         double jd_mean = (field_jd[f1 - 1] + field_jd[f0]) / 2.0;
         if (fabs(jd_mean - big_jd) >= dt_limit * 50.0) {
             big_jd = jd_mean;
-            nb_prop_ctx pc = {n, big, big_alive, jd};
+            nb_prop_ctx pc = {n, big, big_alive, jd, include_asteroids};
             parallel_for(n, 1000, n_threads, prop_range, &pc);
         }
         if (fabs(jd_mean - jd) >= dt_limit) {
@@ -1533,7 +1538,7 @@ int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
                 memcpy(alive, big_alive, n);
             }
             jd = jd_mean;
-            nb_prop_ctx pc = {n, states, alive, jd};
+            nb_prop_ctx pc = {n, states, alive, jd, include_asteroids};
             parallel_for(n, 1000, n_threads, prop_range, &pc);
         }
         int64_t nf = f1 - f0;
@@ -1558,6 +1563,16 @@ int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
 out:
     free(states); free(big); free(alive); free(big_alive); free(fovs);
     return nh;
+}
+
+int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
+                           int32_t frame, int64_t n_fields,
+                           const double *field_jd, const double *obs_xyz,
+                           const double *normals, double dt_limit, int64_t cap,
+                           int64_t *hit_field, int64_t *hit_obj, double *hit_state,
+                           int n_threads) {
+    return ko_fov_state_check_ext(n, jd0, xyz, frame, n_fields, field_jd, obs_xyz, normals,
+                                  dt_limit, 0, cap, hit_field, hit_obj, hit_state, n_threads);
 }
 
 /* ------------------------------------------------------------------------ */

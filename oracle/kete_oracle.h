@@ -174,6 +174,12 @@ int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
                            double dt_limit, int64_t cap, int64_t *hit_field,
                            int64_t *hit_obj, double *hit_state /*[cap][7] jd,pos,vel*/,
                            int n_threads);
+/* ... with fov_state_check's include_asteroids (checks.rs:25-31). */
+int64_t ko_fov_state_check_ext(int64_t n, double jd0, const double *const *xyz,
+                               int32_t frame, int64_t n_fields, const double *field_jd,
+                               const double *obs_xyz, const double *normals, double dt_limit,
+                               int include_asteroids, int64_t cap, int64_t *hit_field,
+                               int64_t *hit_obj, double *hit_state, int n_threads);
 
 /* ---- post-detection epilogue --------------------------------------------- */
 /* ra_dec_with_rates for one state (simult_states.rs:86-120), equatorial. */
