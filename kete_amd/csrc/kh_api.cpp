@@ -404,7 +404,7 @@ int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) 
   }
   if (n_rec_tot >= ((int64_t)1 << 31) / T21_REC) return fail(KH_E_ARG, "too many Type 21 records");
   std::vector<double> rec_h((size_t)n_rec_tot * T21_REC), ep_h((size_t)n_rec_tot);
-  std::vector<int32_t> tab_h((size_t)n_tab_tot);
+  std::vector<T21Bucket> tab_h((size_t)n_tab_tot);
   SpkDev spk = c->spk;
   std::memset(spk.t21, 0, sizeof(spk.t21));
   std::memset(spk.ast, 0, sizeof(spk.ast));
@@ -433,9 +433,9 @@ int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) 
                                   std::to_string(r) + ": " + why);
       ep_h[ro + r] = s.epochs[r];
     }
-    // buckets over [t0, last epoch]: entry b is the first record whose final
-    // epoch is >= t0 + b*width - width*1e-6 (a lower bound for any time that
-    // rounds into bucket b)
+    // buckets over [t0, last epoch]: entry b holds the first record whose
+    // final epoch is >= t0 + b*width - width*1e-6 (a lower bound for any time
+    // that rounds into bucket b) and the final epochs of it and the next
     const double t0 = std::fmin(seg_lo, s.epochs[0]);
     const double t1 = s.epochs[s.n_rec - 1];
     double bw = (t1 - t0) / nb[k];
@@ -449,7 +449,12 @@ int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) 
     for (int32_t b = 0; b < g.n_bucket; ++b) {
       const double x = t0 + b * bw - bw * 1e-6;
       while (i < s.n_rec - 1 && s.epochs[i] < x) ++i;
-      tab_h[to + b] = (int32_t)i;
+      T21Bucket& e = tab_h[to + b];
+      e.rec = (int32_t)i;
+      e.ep0 = s.epochs[i];
+      e.ep1 = i + 1 < s.n_rec ? s.epochs[i + 1] : INFINITY;
+      e.pad0 = 0;
+      e.pad1 = 0;
     }
     obj[q] = s.obj_id;
     ro += s.n_rec;
@@ -482,14 +487,15 @@ int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) 
   spk.t21_tab = nullptr;
   if (nseg > 0) {
     const size_t brec = align256(rec_h.size() * 8), bep = align256(ep_h.size() * 8);
-    HIPCHK(hipMalloc(&c->d_t21, brec + bep + tab_h.size() * 4));
+    const size_t btab = tab_h.size() * sizeof(T21Bucket);
+    HIPCHK(hipMalloc(&c->d_t21, brec + bep + btab));
     char* base = (char*)c->d_t21;
     HIPCHK(hipMemcpy(base, rec_h.data(), rec_h.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(base + brec, ep_h.data(), ep_h.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(base + brec + bep, tab_h.data(), tab_h.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + brec + bep, tab_h.data(), btab, hipMemcpyHostToDevice));
     spk.t21_rec = (const double*)base;
     spk.t21_ep = (const double*)(base + brec);
-    spk.t21_tab = (const int32_t*)(base + brec + bep);
+    spk.t21_tab = (const T21Bucket*)(base + brec + bep);
   }
   if (!c->d_spk) HIPCHK(hipMalloc(&c->d_spk, sizeof(SpkDev)));
   HIPCHK(hipMemcpy(c->d_spk, &spk, sizeof(SpkDev), hipMemcpyHostToDevice));

@@ -93,6 +93,16 @@ struct T21SegDev {
   int32_t sun_centred;  // center 10: the Sun's barycentric state is added
 };
 
+// A bucket of the final-epoch grid: rec is a record at or before the first
+// record serving any time in the bucket, ep0/ep1 the final epochs of rec and
+// rec + 1 (+inf past the last record), so that the lookup usually ends
+// without touching the epoch array.
+struct T21Bucket {
+  double ep0, ep1;
+  int32_t rec, pad0;
+  int64_t pad1;
+};
+
 // A MASSES asteroid: its Type 21 segments t21[seg_begin, seg_end) in load
 // order (kete scans an object's segments in load order, spk.rs:72-78) and
 // its Newtonian parameters (kind 0).
@@ -115,7 +125,7 @@ struct SpkDev {
   // Type 21 segments of any object, and the MASSES asteroids among them
   const double* __restrict__ t21_rec;   // [records][T21_REC]
   const double* __restrict__ t21_ep;    // final epochs, s past J2000
-  const int32_t* __restrict__ t21_tab;  // first candidate record of each bucket
+  const T21Bucket* __restrict__ t21_tab;  // final-epoch buckets
   int32_t n_t21_seg, pad;
   T21SegDev t21[MAX_T21_SEGS];
   AstDev ast[N_ASTEROIDS];
@@ -315,18 +325,55 @@ __device__ __forceinline__ int32_t seg_pos_vel(const SpkDev* __restrict__ spk,
   return seg_eval<true>(spk, coef, s, jd, pos, vel, win);
 }
 
-// Record of Type 21 segment g serving js (s past J2000): the first whose
-// final epoch is >= js (slice::binary_search_by in spk_segments.rs:1023-1030),
-// clamped to the last record (kete reads past the array there).  The bucket
-// table gives a record at or before that one; a short forward scan ends on it.
-__device__ __forceinline__ int32_t t21_record(const SpkDev* __restrict__ spk,
-                                              const T21SegDev& g, double js) {
-  const double u = (js - g.t0) * g.inv_bw;
-  const int32_t c = u > 0.0 ? (int32_t)fmin(u, (double)(g.n_bucket - 1)) : 0;
-  int32_t i = spk->t21_tab[g.tab_off + c];
-  const double* __restrict__ ep = spk->t21_ep + g.ep_off;
-  while (i < g.n_rec - 1 && ep[i] < js) ++i;
-  return (int32_t)g.rec_off + i;
+// Where a Type 21 object's state at JD t (js = seconds past J2000) comes
+// from: the first of its segments t21[begin, end) containing t (kete scans
+// an object's segments in load order, spk.rs:72-78), and in it the first
+// record whose final epoch is >= js (slice::binary_search_by,
+// spk_segments.rs:1023-1030), clamped to the last record (kete reads past
+// the array there).  The segment fields are selected from scalar loads; the
+// bucket of js gives a record at or before the answer and the final epochs of
+// it and its successor, so one load usually settles the record.
+struct T21Ref {
+  int32_t rec;    // global record index (valid when found)
+  int32_t found;  // 0: no segment contains t (DAFLimits)
+  int32_t sun_centred, ecliptic;
+};
+
+__device__ __forceinline__ T21Ref t21_find(const SpkDev* __restrict__ spk, int32_t begin,
+                                           int32_t end, double t, double js) {
+  T21Ref r{0, 0, 0, 0};
+  double t0 = 0.0, inv_bw = 0.0;
+  int64_t tab_off = 0, ep_off = 0, rec_off = 0;
+  int32_t n_rec = 1, n_bucket = 1;
+  for (int32_t q = begin; q < end; ++q) {
+    const T21SegDev& g = spk->t21[q];
+    if (!r.found && !(t < g.jd_start || t > g.jd_end)) {
+      r.found = 1;
+      t0 = g.t0;
+      inv_bw = g.inv_bw;
+      tab_off = g.tab_off;
+      ep_off = g.ep_off;
+      rec_off = g.rec_off;
+      n_rec = g.n_rec;
+      n_bucket = g.n_bucket;
+      r.sun_centred = g.sun_centred;
+      r.ecliptic = g.ecliptic;
+    }
+  }
+  if (!r.found) return r;
+  const double u = (js - t0) * inv_bw;
+  const int32_t c = u > 0.0 ? (int32_t)fmin(u, (double)(n_bucket - 1)) : 0;
+  const T21Bucket b = spk->t21_tab[tab_off + c];
+  int32_t i = b.rec;
+  if (b.ep0 < js && i < n_rec - 1) {
+    ++i;
+    if (b.ep1 < js) {
+      const double* __restrict__ ep = spk->t21_ep + ep_off;
+      while (i < n_rec - 1 && ep[i] < js) ++i;
+    }
+  }
+  r.rec = (int32_t)rec_off + i;
+  return r;
 }
 
 // One Type 21 record at js: position in AU and, if VEL, velocity in AU/day
@@ -362,35 +409,31 @@ __device__ __forceinline__ void t21_eval(const double* __restrict__ r, double js
   }
 }
 
-// State of the object whose Type 21 segments are t21[begin, end), at JD t,
-// relative to the segment's center and in the equatorial frame: the first
-// segment containing t (spk.rs:72-78, SpkSegment::contains), then its record.
-// When the whole wavefront reads one record, its address is uniform and the
-// coefficients arrive through scalar loads.
+// The equatorial state of a found Type 21 reference at js, relative to its
+// segment's center.
 template <bool VEL>
-__device__ __forceinline__ int32_t t21_state(const SpkDev* __restrict__ spk, int32_t begin,
-                                             int32_t end, double t, Vec3& pos, Vec3& vel,
-                                             bool& sun_centred) {
-  int32_t s = -1;
-  for (int32_t q = begin; q < end; ++q) {
-    const T21SegDev& g = spk->t21[q];
-    if (s < 0 && !(t < g.jd_start || t > g.jd_end)) s = q;
-  }
-  if (s < 0) return ST_DAF_LIMITS;
-  const T21SegDev& g = spk->t21[s];
-  const double js = (t - 2451545.0) * 86400.0;
-  const int32_t ri = t21_record(spk, g, js);
-  const int32_t ru = __builtin_amdgcn_readfirstlane(ri);
+__device__ __forceinline__ void t21_state_at(const SpkDev* __restrict__ spk, const T21Ref& ref,
+                                             double js, Vec3& pos, Vec3& vel) {
   Vec3 p, v{0.0, 0.0, 0.0};
-  if (__all(ri == ru)) t21_eval<VEL>(spk->t21_rec + (int64_t)ru * T21_REC, js, p, v);
-  else t21_eval<VEL>(spk->t21_rec + (int64_t)ri * T21_REC, js, p, v);
-  if (g.ecliptic) {
+  t21_eval<VEL>(spk->t21_rec + (int64_t)ref.rec * T21_REC, js, p, v);
+  if (ref.ecliptic) {
     p = ecl_to_equ(p);
     if (VEL) v = ecl_to_equ(v);
   }
   pos = p;
   vel = v;
-  sun_centred = g.sun_centred != 0;
+}
+
+// State of the object whose Type 21 segments are t21[begin, end), at JD t.
+template <bool VEL>
+__device__ __forceinline__ int32_t t21_state(const SpkDev* __restrict__ spk, int32_t begin,
+                                             int32_t end, double t, Vec3& pos, Vec3& vel,
+                                             bool& sun_centred) {
+  const double js = (t - 2451545.0) * 86400.0;
+  const T21Ref ref = t21_find(spk, begin, end, t, js);
+  if (!ref.found) return ST_DAF_LIMITS;
+  t21_state_at<VEL>(spk, ref, js, pos, vel);
+  sun_centred = ref.sun_centred != 0;
   return ST_OK;
 }
 
@@ -726,15 +769,16 @@ __device__ __forceinline__ int32_t add_body_at(const BodyDev& bd, Vec3 bp, Vec3 
 __device__ __forceinline__ int32_t add_asteroids(const SpkDev* __restrict__ spk, double t,
                                                  Vec3 sun, Vec3 x, Vec3 v, bool exact, Vec3& a,
                                                  int32_t& impact_id) {
+  const double js = (t - 2451545.0) * 86400.0;
 #pragma unroll 1
   for (int k = 0; k < N_ASTEROIDS; ++k) {
     const AstDev& ad = spk->ast[k];
+    const T21Ref ref = t21_find(spk, ad.seg_begin, ad.seg_end, t, js);
+    if (!ref.found) return ST_DAF_LIMITS;
     Vec3 p, unused;
-    bool sun_centred = false;
-    int32_t st = t21_state<false>(spk, ad.seg_begin, ad.seg_end, t, p, unused, sun_centred);
-    if (st) return st;
-    if (sun_centred) p = Vec3{p.x + sun.x, p.y + sun.y, p.z + sun.z};
-    st = add_body_at(ad.bd, p, Vec3{0.0, 0.0, 0.0}, x, v, exact, nullptr, a);
+    t21_state_at<false>(spk, ref, js, p, unused);
+    if (ref.sun_centred) p = Vec3{p.x + sun.x, p.y + sun.y, p.z + sun.z};
+    const int32_t st = add_body_at(ad.bd, p, Vec3{0.0, 0.0, 0.0}, x, v, exact, nullptr, a);
     if (st) {
       impact_id = ad.bd.id;
       return st;
