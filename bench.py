@@ -40,6 +40,7 @@ FP64_PEAK_TFLOPS = 78.6
 F_FORCE = 1848.0   # force evaluation: 11 Chebyshev lookups + Newton + GR + J2
 F_SWEEP = 1129.0   # predictor 7 substeps x 139 + B = G.C 147 + convergence 9
 F_ACCEPT = 111.0   # state update 107 + step-size 4
+F_FORCE_AST = 820.0  # include_asteroids: 5 x (Type 21 record 114 + Newton 29) + Sun 105
 # per parameter column of the partials kernel (derivative of the same
 # computation along one parameter): Jacobian-vector products of the force
 # terms (Newton 10 x 19, GR 2 x 60, J2 3 x 50, non-grav 100), the sweep's
@@ -121,9 +122,10 @@ def measured_traffic(kernel, workload):
     return float(rec["bytes_per_launch"])
 
 
-def radau_roofline(cnt, k_ms, workload):
+def radau_roofline(cnt, k_ms, workload, include_asteroids=False):
     n_force, n_sweep, n_acc = (int(cnt[:, k].sum()) for k in range(3))
-    flop = F_FORCE * n_force + F_SWEEP * n_sweep + F_ACCEPT * n_acc
+    f_force = F_FORCE + (F_FORCE_AST if include_asteroids else 0.0)
+    flop = f_force * n_force + F_SWEEP * n_sweep + F_ACCEPT * n_acc
     achieved = flop / (k_ms / 1000.0) / 1e12
     return {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
@@ -170,6 +172,7 @@ def run_nbody(args, uneven=False):
     roof = radau_roofline(cnt, k_ms, workload)
     log(f"[bench] {ms_per_step:.1f} ms/step, kernel {k_ms:.1f} ms, {value:.3e} obj-yr/s, "
         f"{roof['achieved']:.2f} TFLOP/s FP64 ({100 * roof['frac']:.1f}% of peak)")
+    res_gpu = t["out"].cpu().numpy()
     extra = {}
     if uneven:
         # the same batch through the persistent work queue, for comparison
@@ -190,8 +193,26 @@ def run_nbody(args, uneven=False):
                                 "note": "kh_propagate_n_body with host arrays: staging "
                                         "copies over PCIe included"}
         log(f"[bench] host-array entry point: {ms_host:.1f} ms/step")
+        # the same batch with include_asteroids (kete's MASSES: five Type 21
+        # perturbers after the planets)
+        ast_step = nbody_step_fn(ctx, t, n, stride, mode, include_asteroids=True)
+        ast_step(counters=True)
+        torch.cuda.synchronize()
+        cnt_a = t["cnt"].cpu().numpy().astype(np.int64)
+        st_a = t["st"].cpu().numpy()
+        out_a = t["out"].cpu().numpy()
+        ms_a, k_a = time_steps(ctx, ast_step, max(1, args.steps // 2), 0)
+        roof_a = radau_roofline(cnt_a, k_a, workload + "_masses", include_asteroids=True)
+        roof_a.pop("traffic")
+        extra["include_asteroids"] = {
+            "ms_per_step": ms_a, "kernel_ms": k_a, "value": obj_years / (ms_a / 1000.0),
+            "cost_vs_planets": ms_a / ms_per_step, "roofline": roof_a,
+            "status_counts": {str(k): int(v) for k, v in zip(*np.unique(st_a, return_counts=True))},
+            "note": "same workload, force list MASSES (PLANETS + Ceres, Pallas, Vesta, "
+                    "Hygiea, Interamnia from synthetic SPK Type 21 segments)"}
+        log(f"[bench] include_asteroids: {ms_a:.1f} ms/step ({ms_a / ms_per_step:.2f}x), "
+            f"{100 * roof_a['frac']:.1f}% of FP64 peak")
 
-    res_gpu = t["out"].cpu().numpy()
     cpu = parity = None
     if not args.no_cpu:
         import oracle
@@ -217,6 +238,26 @@ def run_nbody(args, uneven=False):
                "sample": f"{len(idx)} evenly spaced objects of the same workload and arcs, "
                          f"{dt:.1f} s wall on {threads} threads (C restatement, -O3)"}
         log(f"[bench] cpu {cpu_val:.1f} obj-yr/s on {threads} threads; parity {parity}")
+        if "include_asteroids" in extra:
+            # include_asteroids on a quarter of the sample: CPU rate and parity
+            ia = idx[::4]
+            oracle.spk_load(syn.ephemeris_path(), syn.asteroid_ephemeris_path())
+            tc = time.perf_counter()
+            ra = oracle.batch_n_body(jd0, xyz[:, ia], jdf, n_threads=threads,
+                                     include_asteroids=True)
+            dta = time.perf_counter() - tc
+            oka = (ra["status"] == 0) & (st_a[ia] == 0)
+            ea = extra["include_asteroids"]
+            ea["parity"] = {"sample": int(len(ia)),
+                            "max_pos_err_au": float(np.abs(out_a[:3, ia][:, oka] -
+                                                           ra["out"][:3, oka]).max()),
+                            "status_match": bool(np.array_equal(ra["status"], st_a[ia]))}
+            ea["cpu_baseline"] = {"value": float(spans[ia].sum() / 365.25) / dta,
+                                  "unit": "object-years/s", "cores": threads, "kind": "port",
+                                  "sample": f"{len(ia)} objects, {dta:.1f} s wall"}
+            log(f"[bench] include_asteroids cpu {ea['cpu_baseline']['value']:.1f} obj-yr/s; "
+                f"parity {ea['parity']}")
+            oracle.spk_load(syn.ephemeris_path())
 
     line = {
         "metric": "object-years/s, FP64 Radau (<=1e-9 AU vs ref)",

@@ -262,6 +262,34 @@ int32_t kh_radec_partials_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                               double *const dstate[6], int32_t *status, kh_counters *counters,
                               int32_t sync);
 
+/* Launch shape picked by kh_autotune_n_body. */
+typedef struct {
+    int32_t mode;       /* 0-3 as kh_propagate_n_body's mode */
+    int32_t block;      /* objects (mode 0/1) or lanes (2/3) per block */
+    int32_t window;     /* records per segment in the LDS ephemeris cache (0: none) */
+    int32_t n_tried;    /* shapes measured */
+    int64_t latency_k;  /* most objects the latency kernel takes (mode 0) */
+    int64_t n_sample;   /* objects the shapes were timed on */
+    double ms;          /* best time of one call on the sample */
+    double default_ms;  /* the library default's time on the sample */
+} kh_tune_result;
+
+/* Autotuner (no kete counterpart; kete has one CPU code path).  Times the
+ * N-body call (device pointers, arguments as kh_propagate_n_body_dev) on
+ * every (n/sample)-th object of the batch (all when sample is 0) over
+ * launch shapes -- threads per object and plain or persistent launch
+ * (mode), objects per block, the LDS ephemeris window, the latency kernel's
+ * share -- by coordinate descent from the default, and reports the fastest.
+ * With apply != 0 the context uses it for every later mode -1 call until
+ * kh_autotune_clear.  Results of a call do not depend on the shape beyond
+ * last-ulp differences between the one- and three-lane kernels.  Allocates
+ * and frees a scratch buffer of 20 x 8 x sample bytes. */
+int32_t kh_autotune_n_body(kh_ctx *ctx, int64_t n, const double *jd0, const double *const xyz[6],
+                           int64_t center, int32_t frame, const double *jd_final,
+                           int32_t jd_final_stride, int32_t include_asteroids, int64_t sample,
+                           int32_t apply, kh_tune_result *out);
+int32_t kh_autotune_clear(kh_ctx *ctx);
+
 /* Launch geometry last used by kh_propagate_n_body*: threads per block,
  * blocks, and mode actually run. */
 int32_t kh_last_launch(kh_ctx *ctx, int32_t *block, int32_t *grid, int32_t *mode);

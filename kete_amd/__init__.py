@@ -134,6 +134,7 @@ class Context:
     ``LOADED_SPK``, ``spice/spk.rs:368-377``, becomes per-context state)."""
 
     def __init__(self, device: int = 0):
+        self.device = device
         self.L = _lib.load()
         h = C.c_void_p()
         _lib.check(self.L.kh_ctx_create(device, C.byref(h)))
@@ -233,6 +234,33 @@ class Context:
             _lib.iptr(iid, C.c_int64), _lib.dptr(ijd),
             cnt.ctypes.data_as(C.POINTER(_lib.KhCounters)) if cnt is not None else None))
         return dict(out=out, status=st, impact_id=iid, impact_jd=ijd, counters=cnt)
+
+    def autotune(self, batch: StateArray, jd_final, include_asteroids=False, sample=0,
+                 apply=True):
+        """Pick the N-body launch shape for batches like this one
+        (kh_autotune_n_body): the batch goes to the device once, every
+        (n/sample)-th object is timed over the candidate shapes, and with
+        ``apply`` later ``mode=-1`` calls on this context use the fastest.
+        -> dict(mode, block, window, latency_k, ms, default_ms, n_tried, n_sample)"""
+        import torch
+        dev = torch.device("cuda", self.device)
+        n = len(batch)
+        jf = np.ascontiguousarray(np.atleast_1d(jd_final), dtype=np.float64)
+        stride = 0 if len(jf) == 1 else 1
+        t_jd0 = torch.from_numpy(np.array(batch.jd, dtype=np.float64)).to(dev)
+        t_xyz = torch.from_numpy(np.array(batch.xyz, dtype=np.float64)).to(dev)
+        t_jf = torch.from_numpy(jf).to(dev)
+        torch.cuda.synchronize(dev)
+        res = _lib.KhTuneResult()
+        _lib.check(self.L.kh_autotune_n_body(
+            self.h, n, _lib.raw(t_jd0.data_ptr()), _lib.ptr_array([t_xyz[k].data_ptr()
+                                                                  for k in range(6)]),
+            batch.center_id, batch.frame, _lib.raw(t_jf.data_ptr()), stride,
+            int(bool(include_asteroids)), int(sample), int(bool(apply)), C.byref(res)))
+        return {k: getattr(res, k) for k, _ in _lib.KhTuneResult._fields_}
+
+    def autotune_clear(self):
+        _lib.check(self.L.kh_autotune_clear(self.h))
 
     def radec_partials(self, jd0, xyz, jd_obs, obs, frame=FRAME_ECLIPTIC, non_gravs=None,
                        force=0, restart=False, epsilon=1e-6, states=False, counters=False):

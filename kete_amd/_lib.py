@@ -26,7 +26,8 @@ EXPORTS = [
     "kh_spk_upload_type2", "kh_spk_upload_type21", "kh_spk_states", "kh_propagate_two_body",
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
-    "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev",
+    "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
+    "kh_autotune_clear",
 ]
 
 
@@ -53,6 +54,12 @@ class KhNonGrav(C.Structure):
 class KhCounters(C.Structure):
     _fields_ = [("n_force", C.c_int32), ("n_sweep", C.c_int32), ("n_acc", C.c_int32),
                 ("n_fail", C.c_int32)]
+
+
+class KhTuneResult(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("window", C.c_int32),
+                ("n_tried", C.c_int32), ("latency_k", C.c_int64), ("n_sample", C.c_int64),
+                ("ms", C.c_double), ("default_ms", C.c_double)]
 
 
 NONGRAV_DTYPE = np.dtype([("kind", np.int32), ("pad", np.int32)] +
@@ -136,6 +143,11 @@ def load():
     L.kh_radec_partials.restype = I32
     L.kh_radec_partials_dev.argtypes = rp + [I32]
     L.kh_radec_partials_dev.restype = I32
+    L.kh_autotune_n_body.argtypes = [V, I64, P, PP, I64, I32, P, I32, I32, I64, I32,
+                                     C.POINTER(KhTuneResult)]
+    L.kh_autotune_n_body.restype = I32
+    L.kh_autotune_clear.argtypes = [V]
+    L.kh_autotune_clear.restype = I32
     L.kh_last_kernel_ms.argtypes = [V]
     L.kh_last_kernel_ms.restype = D
     _lib = L

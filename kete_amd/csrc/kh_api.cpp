@@ -461,6 +461,7 @@ int32_t kh_spk_upload_type21(kh_ctx* c, const kh_spk21_seg* segs, int64_t nseg) 
     to += g.n_bucket;
   }
   spk.n_t21_seg = (int32_t)nseg;
+  spk.n_t21_rec = (int32_t)n_rec_tot;
   for (int a = 0; a < N_ASTEROIDS; ++a) {
     AstDev& ad = spk.ast[a];
     ad.seg_begin = ad.seg_end = 0;
@@ -705,6 +706,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   p.mode = mode < 0 ? 0 : mode;
   p.n_ast = include_asteroids ? N_ASTEROIDS : 0;
   p.block = 256;
+  const bool tuned = mode < 0 && c->tune.valid && !me;
   p.grid = 0;
   // LDS window of 16 records (62 KB for 11 segments) when all segments have
   // 14 coefficients and the window fits next to a second block on the CU
@@ -723,6 +725,17 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // cost outweighs a lane's share of the batch (KH_LATENCY_K sets the cap)
   p.latency_k = c->spk.shared_grid ? (int64_t)c->n_cu * 8 : 0;
   if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;
+  if (tuned) {
+    p.mode = c->tune.mode;
+    p.block = c->tune.block;
+    p.latency_k = c->spk.shared_grid ? c->tune.latency_k : 0;
+    p.window = c->tune.window;
+    p.range = (c->spk.all14 && p.window > 0 &&
+               (size_t)c->spk.n_seg * p.window * 44 * 8 <= 64 * 1024)
+                  ? c->d_range
+                  : nullptr;
+    if (!p.range) p.window = 0;
+  }
   p.split = c->d_split;
   p.stream2 = c->stream2;
   p.fork = c->fork;
@@ -825,6 +838,150 @@ int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
     HIPCHK(hipMemcpyAsync(counters, d_cnt, n * sizeof(kh_counters), hipMemcpyDeviceToHost,
                           c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: launch shape of the N-body path measured on the caller's batch
+// ---------------------------------------------------------------------------
+int32_t kh_autotune_n_body(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],
+                           int64_t center, int32_t frame, const double* jd_final,
+                           int32_t jd_final_stride, int32_t include_asteroids, int64_t sample,
+                           int32_t apply, kh_tune_result* out) {
+  if (!c || !out) return fail(KH_E_ARG, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  if (n <= 0 || !jd0 || !xyz || !jd_final) return fail(KH_E_ARG, "empty batch or null array");
+  for (int k = 0; k < 6; ++k)
+    if (!xyz[k]) return fail(KH_E_ARG, "null state column");
+  HIPCHK(hipSetDevice(c->device));
+  // the objects tuned on: every stride-th of the batch (all when sample is 0
+  // or >= n), so the sample keeps the batch's mix of orbits and spans
+  const int64_t m = (sample <= 0 || sample >= n) ? n : sample;
+  const int64_t stride = n / m;
+  const size_t b = align256(m * 8);
+  void* buf = nullptr;
+  HIPCHK(hipMalloc(&buf, 20 * b));
+  char* q = (char*)buf;
+  double* s_jd0 = (double*)q;
+  double* s_jf = (double*)(q + b);
+  const double* s_xyz[6];
+  double* s_in[6];
+  double* s_out[6];
+  for (int k = 0; k < 6; ++k) {
+    s_in[k] = (double*)(q + (2 + k) * b);
+    s_xyz[k] = s_in[k];
+    s_out[k] = (double*)(q + (8 + k) * b);
+  }
+  int32_t* s_st = (int32_t*)(q + 14 * b);
+  int32_t r = launch_gather_states(m, stride, jd0, xyz, jd_final, jd_final_stride, s_jd0, s_in,
+                                   s_jf, c->stream)
+                  ? fail(KH_E_HIP, "gather launch failed")
+                  : KH_OK;
+  struct Cand {
+    int32_t mode, block, window;
+    int64_t latency_k;
+  };
+  const bool shared = c->spk.shared_grid != 0;
+  const int32_t win_max = c->spk.all14 ? (int32_t)std::min<int64_t>(
+                                             16, 64 * 1024 / ((int64_t)c->spk.n_seg * 352))
+                                       : 0;
+  const int64_t lat_default = shared ? (int64_t)c->n_cu * 8 : 0;
+  Cand best{0, 256, win_max, lat_default};
+  double best_ms = INFINITY, default_ms = INFINITY;
+  int32_t tried = 0;
+  const bool keep = c->tune.valid;
+  const auto saved = c->tune;
+  auto run = [&](const Cand& cd) -> double {
+    c->tune.valid = true;
+    c->tune.mode = cd.mode;
+    c->tune.block = cd.block;
+    c->tune.window = cd.window;
+    c->tune.latency_k = cd.latency_k;
+    double ms = INFINITY;
+    // one untimed run (first use of a shape), then the timed one
+    for (int rep = 0; rep < 2 && r == KH_OK; ++rep) {
+      if (hipEventRecord(c->ev0, c->stream) != hipSuccess) r = fail(KH_E_HIP, "event record");
+      if (r == KH_OK)
+        r = nbody_dev(c, m, s_jd0, s_xyz, center, frame, s_jf, 1, include_asteroids, nullptr,
+                      1e-6, -1, s_out, s_st, nullptr, nullptr, nullptr, 0, nullptr);
+      hipEvent_t e1 = nullptr;
+      if (r == KH_OK && hipEventCreate(&e1) == hipSuccess &&
+          hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+        float f = 0.f;
+        if (hipEventElapsedTime(&f, c->ev0, e1) == hipSuccess) ms = f;
+      } else if (r == KH_OK) {
+        r = fail(KH_E_HIP, "timing event failed");
+      }
+      if (e1) (void)hipEventDestroy(e1);
+    }
+    ++tried;
+    return ms;
+  };
+  auto consider = [&](const Cand& cd) {
+    const double ms = run(cd);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cd;
+    }
+  };
+  // coordinate descent from the default shape: threads per object and
+  // launch kind, then objects per block, then the LDS window, then the
+  // latency-kernel head
+  default_ms = run(best);
+  best_ms = default_ms;
+  const Cand base = best;
+  for (int32_t md : {1, 2, 3}) {
+    Cand cd = base;
+    cd.mode = md;
+    consider(cd);
+  }
+  if (best.mode <= 1)
+    for (int32_t bl : {64, 128}) {
+      Cand cd = best;
+      cd.block = bl;
+      consider(cd);
+    }
+  if (win_max > 0) {
+    for (int32_t w : {0, 8, 12}) {
+      if (w >= win_max) continue;
+      Cand cd = best;
+      cd.window = w;
+      consider(cd);
+    }
+  }
+  if (best.mode == 0 && shared)
+    for (int64_t lk : {(int64_t)0, (int64_t)c->n_cu * 2, (int64_t)c->n_cu * 32}) {
+      Cand cd = best;
+      cd.latency_k = lk;
+      consider(cd);
+    }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(buf);
+  c->tune = saved;
+  c->tune.valid = keep;
+  if (r != KH_OK) return r;
+  out->mode = best.mode;
+  out->block = best.block;
+  out->window = best.window;
+  out->latency_k = best.latency_k;
+  out->ms = best_ms;
+  out->default_ms = default_ms;
+  out->n_tried = tried;
+  out->n_sample = m;
+  if (apply) {
+    c->tune.valid = true;
+    c->tune.mode = best.mode;
+    c->tune.block = best.block;
+    c->tune.window = best.window;
+    c->tune.latency_k = best.latency_k;
+  }
+  return KH_OK;
+}
+
+int32_t kh_autotune_clear(kh_ctx* c) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  c->tune.valid = false;
   return KH_OK;
 }
 

@@ -42,6 +42,12 @@ struct kh_ctx {
   hipStream_t stream2 = nullptr;  // the N-body latency kernel runs here
   int64_t* d_split = nullptr;     // head of the N-body order it takes
   hipEvent_t fork = nullptr, join = nullptr;
+  // launch shape chosen by kh_autotune_n_body, used by mode -1 calls
+  struct {
+    bool valid = false;
+    int32_t mode = 0, block = 256, window = 16;
+    int64_t latency_k = 0;
+  } tune;
   bool timed = false;
   int32_t last_block = 0, last_grid = 0, last_mode = 0;
   int n_cu = 0;

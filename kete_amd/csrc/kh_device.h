@@ -126,7 +126,7 @@ struct SpkDev {
   const double* __restrict__ t21_rec;   // [records][T21_REC]
   const double* __restrict__ t21_ep;    // final epochs, s past J2000
   const T21Bucket* __restrict__ t21_tab;  // final-epoch buckets
-  int32_t n_t21_seg, pad;
+  int32_t n_t21_seg, n_t21_rec;
   T21SegDev t21[MAX_T21_SEGS];
   AstDev ast[N_ASTEROIDS];
 };

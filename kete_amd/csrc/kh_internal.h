@@ -102,6 +102,13 @@ struct TwoBodyLaunch {
 int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaunch& p,
                         hipStream_t stream);
 
+// Every stride-th object of a batch (m of them) into contiguous arrays;
+// jd_final broadcast when jf_stride is 0.
+int32_t launch_gather_states(int64_t m, int64_t stride, const double* jd0,
+                             const double* const xyz[6], const double* jd_final,
+                             int32_t jf_stride, double* o_jd0, double* const o_xyz[6],
+                             double* o_jf, hipStream_t stream);
+
 // State of `body` relative to `center` (NAIF id center_id) at m epochs.
 // With t21_begin < t21_end the body is the object whose Type 21 segments are
 // spk.t21[t21_begin, t21_end) instead, and `sun` the Sun's chain (added for

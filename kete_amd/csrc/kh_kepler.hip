@@ -86,6 +86,37 @@ int32_t launch_two_body(const SpkDev* spk, const double* coef, const TwoBodyLaun
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+__global__ void gather_states_kernel(int64_t m, int64_t stride, const double* __restrict__ jd0,
+                                     const double* x0, const double* x1, const double* x2,
+                                     const double* x3, const double* x4, const double* x5,
+                                     const double* __restrict__ jf, int32_t jf_stride,
+                                     double* o_jd0, double* y0, double* y1, double* y2,
+                                     double* y3, double* y4, double* y5, double* o_jf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t j = i * stride;
+  o_jd0[i] = jd0[j];
+  y0[i] = x0[j];
+  y1[i] = x1[j];
+  y2[i] = x2[j];
+  y3[i] = x3[j];
+  y4[i] = x4[j];
+  y5[i] = x5[j];
+  o_jf[i] = jf[jf_stride ? j : 0];
+}
+
+int32_t launch_gather_states(int64_t m, int64_t stride, const double* jd0,
+                             const double* const xyz[6], const double* jd_final,
+                             int32_t jf_stride, double* o_jd0, double* const o_xyz[6],
+                             double* o_jf, hipStream_t stream) {
+  if (m <= 0) return 0;
+  hipLaunchKernelGGL(gather_states_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+                     stream, m, stride, jd0, xyz[0], xyz[1], xyz[2], xyz[3], xyz[4], xyz[5],
+                     jd_final, jf_stride, o_jd0, o_xyz[0], o_xyz[1], o_xyz[2], o_xyz[3],
+                     o_xyz[4], o_xyz[5], o_jf);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // State of a body chain (or a Type 21 object) relative to a center chain,
 // rotated to `frame`.
 __global__ void spk_states_kernel(const SpkDev* __restrict__ spk,
