@@ -118,6 +118,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->dbuf) (void)hipFree(c->dbuf);
   if (c->obuf) (void)hipFree(c->obuf);
   if (c->fbuf) (void)hipFree(c->fbuf);
+  if (c->hbuf) (void)hipFree(c->hbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);

@@ -15,6 +15,8 @@
 //  3. n-body (only for fields more than dt_limit from the state epoch): the
 //     candidates are re-propagated with the Radau kernel, then a light-time
 //     step and containment (fov_like.rs:90-108) finish them.
+#include <hipcub/hipcub.hpp>
+
 #include "kh_device.h"
 #include "kh_internal.h"
 
@@ -109,7 +111,7 @@ __global__ void __launch_bounds__(256) fov_two_body_kernel(FovTwoBody p) {
     if (!inside) return;
     const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
     if (slot < (unsigned long long)p.hit_cap) {
-      p.hit_field[slot] = f;
+      p.hit_field[slot] = p.field_base + f;
       p.hit_obj[slot] = i;
       p.hit_state[0][slot] = jd_obs;
       p.hit_state[1][slot] = p2.x;
@@ -160,7 +162,7 @@ __global__ void fov_nb_finish_kernel(FovNBody p) {
   if (!rect_contains(F.nrm, rel, dist)) return;
   const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
   if (slot < (unsigned long long)p.hit_cap) {
-    p.hit_field[slot] = f;
+    p.hit_field[slot] = p.field_base + f;
     p.hit_obj[slot] = p.nb_obj[q];
     p.hit_state[0][slot] = jd_obs;
     p.hit_state[1][slot] = p2.x;
@@ -170,6 +172,82 @@ __global__ void fov_nb_finish_kernel(FovNBody p) {
     p.hit_state[5][slot] = v2.y;
     p.hit_state[6][slot] = v2.z;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Hit ordering: fields in epoch order, then objects by index, as kete returns
+// them (the rayon collect over sorted fields, checks.rs:87-104)
+// ---------------------------------------------------------------------------
+__global__ void fov_hit_keys_kernel(int64_t n, int32_t obj_bits, const int32_t* __restrict__ f,
+                                    const int64_t* __restrict__ o, uint64_t* __restrict__ key,
+                                    uint32_t* __restrict__ val) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= n) return;
+  key[h] = ((uint64_t)(uint32_t)f[h] << obj_bits) | (uint64_t)o[h];
+  val[h] = (uint32_t)h;
+}
+
+__global__ void fov_hit_gather_kernel(int64_t n, const uint32_t* __restrict__ perm,
+                                      const int32_t* __restrict__ f,
+                                      const int64_t* __restrict__ o,
+                                      const int64_t* __restrict__ order, const double* s0,
+                                      const double* s1, const double* s2, const double* s3,
+                                      const double* s4, const double* s5, const double* s6,
+                                      int64_t* __restrict__ of, int64_t* __restrict__ oo,
+                                      double* d0, double* d1, double* d2, double* d3,
+                                      double* d4, double* d5, double* d6) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t h = perm[k];
+  of[k] = order[f[h]];
+  oo[k] = o[h];
+  d0[k] = s0[h];
+  d1[k] = s1[h];
+  d2[k] = s2[h];
+  d3[k] = s3[h];
+  d4[k] = s4[h];
+  d5[k] = s5[h];
+  d6[k] = s6[h];
+}
+
+size_t fov_sort_temp_bytes(int64_t n_hits) {
+  size_t cub = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, cub, (const uint64_t*)nullptr,
+                                           (uint64_t*)nullptr, (const uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)n_hits, 0, 64);
+  const size_t a = ((size_t)n_hits * 8 + 255) & ~(size_t)255;
+  const size_t b = ((size_t)n_hits * 4 + 255) & ~(size_t)255;
+  return 2 * a + 2 * b + cub + 256;
+}
+
+int32_t launch_fov_sort_hits(int64_t n_hits, int32_t obj_bits, int32_t field_bits,
+                             const int32_t* hit_field, const int64_t* hit_obj,
+                             double* const hit_state[7], const int64_t* order, void* temp,
+                             size_t temp_bytes, int64_t* out_field, int64_t* out_obj,
+                             double* const out_state[7], hipStream_t s) {
+  if (n_hits <= 0) return 0;
+  if (n_hits >= ((int64_t)1 << 31) || obj_bits + field_bits > 64) return -1;
+  const size_t a = ((size_t)n_hits * 8 + 255) & ~(size_t)255;
+  const size_t b = ((size_t)n_hits * 4 + 255) & ~(size_t)255;
+  char* q = (char*)temp;
+  uint64_t* k_in = (uint64_t*)q;
+  uint64_t* k_out = (uint64_t*)(q + a);
+  uint32_t* v_in = (uint32_t*)(q + 2 * a);
+  uint32_t* v_out = (uint32_t*)(q + 2 * a + b);
+  void* cub = q + 2 * a + 2 * b;
+  size_t cub_bytes = temp_bytes - (2 * a + 2 * b);
+  const unsigned g = (unsigned)((n_hits + 255) / 256);
+  hipLaunchKernelGGL(fov_hit_keys_kernel, dim3(g), dim3(256), 0, s, n_hits, obj_bits, hit_field,
+                     hit_obj, k_in, v_in);
+  if (hipcub::DeviceRadixSort::SortPairs(cub, cub_bytes, k_in, k_out, v_in, v_out, (int)n_hits, 0,
+                                         obj_bits + field_bits, s) != hipSuccess)
+    return -1;
+  hipLaunchKernelGGL(fov_hit_gather_kernel, dim3(g), dim3(256), 0, s, n_hits, v_out, hit_field,
+                     hit_obj, order, hit_state[0], hit_state[1], hit_state[2], hit_state[3],
+                     hit_state[4], hit_state[5], hit_state[6], out_field, out_obj, out_state[0],
+                     out_state[1], out_state[2], out_state[3], out_state[4], out_state[5],
+                     out_state[6]);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // ra_dec_with_rates (simult_states.rs:86-120, to_lat_lon frames/definitions.rs:

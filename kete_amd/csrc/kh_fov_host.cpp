@@ -161,7 +161,6 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
   int64_t cand_cap = std::max<int64_t>(1 << 20, 8 * n);
   int64_t nb_cap = std::max<int64_t>(1 << 16, n / 4);
   const int64_t hit_cap = std::max<int64_t>(cap, 1);
-  std::vector<std::pair<int64_t, int64_t>> chunk_hits;  // (first sorted field, hits so far)
   for (int attempt = 0; attempt < 3; ++attempt) {
     const size_t bn = al(n * 8), bs = al(n * 4);
     const size_t bw = al((size_t)win_epochs * n * 8);
@@ -331,6 +330,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       tb.jd = sjd;
       for (int k = 0; k < 6; ++k) tb.x[k] = sx[k];
       tb.fields = d_fields + f0;
+      tb.field_base = (int32_t)f0;
       tb.dt_limit = dt_limit;
       tb.cand_field = cand_field;
       tb.cand_kind = cand_kind;
@@ -359,6 +359,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
         nb.jd = sjd;
         for (int k = 0; k < 6; ++k) nb.x[k] = sx[k];
         nb.fields = d_fields + f0;
+        nb.field_base = (int32_t)f0;
         nb.nb_field = nb_field;
         nb.nb_obj = nb_obj;
         nb.g_jd0 = g_jd0;
@@ -381,49 +382,46 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
         if (r) return r;
         if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");
       }
-      // global field index of this chunk's hits is resolved on the host
-      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      chunk_hits.push_back({f0, (int64_t)counts[1]});
     }
-    if (overflow) {
-      chunk_hits.clear();
-      continue;
-    }
+    if (overflow) continue;
     unsigned long long total = 0;
     HIPCHK(hipMemcpyAsync(&total, d_nhit, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     *n_hits = (int64_t)total;
     if ((int64_t)total > cap) return kh_fail(KH_E_ARG, "hit capacity too small: need " +
                                                            std::to_string(total));
-    // hits in (sorted field, object index) order, the order kete returns
-    std::vector<int32_t> hf(total);
-    std::vector<int64_t> ho(total);
-    std::vector<double> hs(7 * total);
+    // hits in (sorted field, object index) order, the order kete returns:
+    // a device radix sort on packed keys, then one gather into the outputs
     if (total) {
-      HIPCHK(hipMemcpy(hf.data(), h_field, total * 4, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(ho.data(), h_obj, total * 8, hipMemcpyDeviceToHost));
+      auto bits = [](int64_t v) {
+        int32_t b = 1;
+        while (b < 63 && ((int64_t)1 << b) <= v) ++b;
+        return b;
+      };
+      const int32_t obj_bits = bits(n - 1), field_bits = bits(n_fields - 1);
+      const int64_t m = (int64_t)total;
+      const size_t tmp = fov_sort_temp_bytes(m);
+      const size_t b8 = al(m * 8);
+      r = kh_grow(&c->hbuf, &c->hbuf_size, al(n_fields * 8) + tmp + 9 * b8);
+      if (r) return r;
+      char* q2 = (char*)c->hbuf;
+      int64_t* d_order = (int64_t*)q2;
+      q2 += al(n_fields * 8);
+      void* d_tmp = q2;
+      q2 += tmp;
+      int64_t* o_field = (int64_t*)q2;
+      int64_t* o_obj = (int64_t*)(q2 + b8);
+      double* o_state[7];
+      for (int k = 0; k < 7; ++k) o_state[k] = (double*)(q2 + (2 + k) * b8);
+      HIPCHK(hipMemcpyAsync(d_order, order.data(), n_fields * 8, hipMemcpyHostToDevice, s));
+      if (launch_fov_sort_hits(m, obj_bits, field_bits, h_field, h_obj, h_state, d_order, d_tmp,
+                               tmp, o_field, o_obj, o_state, s))
+        return kh_fail(KH_E_HIP, "hit ordering failed");
+      HIPCHK(hipMemcpyAsync(hit_field, o_field, m * 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(hit_obj, o_obj, m * 8, hipMemcpyDeviceToHost, s));
       for (int k = 0; k < 7; ++k)
-        HIPCHK(hipMemcpy(hs.data() + k * total, h_state[k], total * 8, hipMemcpyDeviceToHost));
-    }
-    // Hits arrive grouped by chunk (chunks in field order); within a chunk,
-    // order them by (chunk-local field, object index) on packed keys.
-    std::vector<std::pair<uint64_t, uint32_t>> keys;
-    size_t k_out = 0, h0 = 0;
-    for (size_t ci = 0; ci < chunk_hits.size(); ++ci) {
-      const size_t h1 = (size_t)chunk_hits[ci].second;
-      keys.resize(h1 - h0);
-      for (size_t h = h0; h < h1; ++h)
-        keys[h - h0] = {((uint64_t)hf[h] << 40) | (uint64_t)ho[h], (uint32_t)(h - h0)};
-      std::sort(keys.begin(), keys.end());
-      for (const auto& kv : keys) {
-        const size_t h = h0 + kv.second;
-        hit_field[k_out] = order[chunk_hits[ci].first + hf[h]];
-        hit_obj[k_out] = ho[h];
-        for (int q = 0; q < 7; ++q) hit_state[q][k_out] = hs[q * total + h];
-        ++k_out;
-      }
-      h0 = h1;
+        HIPCHK(hipMemcpyAsync(hit_state[k], o_state[k], m * 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
     }
     return KH_OK;
   }

@@ -152,6 +152,7 @@ struct FovTwoBody {
   const int64_t* cand_obj;
   unsigned long long* n_hit;
   int64_t hit_cap;
+  int32_t field_base;  // sorted index of fields[0]; hits record field_base + f
   int32_t* hit_field;
   int64_t* hit_obj;
   double* hit_state[7];
@@ -175,6 +176,7 @@ struct FovNBody {
   const int32_t* g_status;
   unsigned long long* n_hit;
   int64_t hit_cap;
+  int32_t field_base;
   int32_t* hit_field;
   int64_t* hit_obj;
   double* hit_state[7];
@@ -195,6 +197,17 @@ int32_t launch_fov_two_body(const FovTwoBody& p, hipStream_t s);
 int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s);
 int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s);
 int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s);
+
+// Hits in kete's order -- sorted field index, then object index -- by a
+// device radix sort on packed keys, gathered into out_* with the original
+// field index order[sorted index].  temp/temp_bytes: scratch of at least
+// fov_sort_temp_bytes(n_hits) bytes.
+size_t fov_sort_temp_bytes(int64_t n_hits);
+int32_t launch_fov_sort_hits(int64_t n_hits, int32_t obj_bits, int32_t field_bits,
+                             const int32_t* hit_field, const int64_t* hit_obj,
+                             double* const hit_state[7], const int64_t* order, void* temp,
+                             size_t temp_bytes, int64_t* out_field, int64_t* out_obj,
+                             double* const out_state[7], hipStream_t s);
 
 // Observables and partial derivatives (kh_grad.hip).
 struct GradLaunch {

@@ -127,7 +127,9 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
             continue
         _lib.check(rc)
         k = nh.value
-        return FovHits(hf[:k].copy(), ho[:k].copy(), hs[:, :k].copy())
+        # views of the first k entries (the untouched rest of the capacity
+        # is never paged in)
+        return FovHits(hf[:k], ho[:k], hs[:, :k])
 
 
 def ra_dec_vmag(ctx, state, obs, h_mag, g_param=0.15, frame=FRAME_ECLIPTIC):
