@@ -87,6 +87,15 @@ struct NBodyArgs {
   int32_t n_ast;  // asteroids after the planets in the force list (0 or 5)
 };
 
+// Last index of the run of epochs from e that moves away from t in one
+// direction (multi-epoch propagation).
+__device__ __forceinline__ int32_t multi_run_end(const NBodyArgs& a, int32_t e, double t) {
+  const double dir = a.epochs[e] - t;
+  int32_t k = e;
+  while (k + 1 < a.n_epochs && (a.epochs[k + 1] - a.epochs[k]) * dir > 0.0) ++k;
+  return k;
+}
+
 // The slot range of this launch (uniform: read once per lane from the
 // device-chosen split when there is one).
 __device__ __forceinline__ void slot_range(NBodyArgs& a) {
@@ -390,7 +399,10 @@ struct LdsTables {
 // ILP: the latency variant for the slowest objects -- one wavefront per block
 // and per SIMD, the whole register file, the force evaluation arranged for
 // the shortest dependency chain (spk_accel_ilp), no LDS window.
-// MULTI: the multi-epoch propagation (plain launch only).
+// MULTI: the multi-epoch propagation (plain launch only).  The epochs are
+// taken in runs that go one way in time from the current time; each run is
+// one integration to its last epoch, and the states at the epochs inside it
+// come from the accepted steps' interpolants.
 // AST: the force list is MASSES (the planets and five asteroids) instead of
 // PLANETS; a separate instantiation, so the planets-only kernels carry no
 // asteroid code.
@@ -440,10 +452,10 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
   int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
   obj = fetch<PERSISTENT>(a, obj, t, tf, x, v);
-  int32_t ej = 0;                // MULTI: the epoch being integrated to
+  int32_t ej = 0;                // MULTI: the next epoch to store
   double h_want = 0.0;           // MULTI: the step proposed before shortening
   bool shortened = false;        // MULTI: the step was shortened to land on tf
-  if (MULTI) tf = a.epochs[0];
+  if (MULTI) tf = a.epochs[multi_run_end(a, 0, t)];
 
   while (obj < a.n) {
     const bool sub = phase == PH_SUB;
@@ -552,6 +564,42 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         xs[d] = conv ? nx : xs[d];
         vs[d] = conv ? nv : vs[d];
       }
+      if (MULTI && conv) {
+        // epochs strictly inside the accepted step: the step's own
+        // interpolant (the predictor of radau.rs:252-278 at s = (e - t)/h)
+        while (ej < a.n_epochs) {
+          const double e = a.epochs[ej];
+          const double se = (e - t) / h;
+          if (!(se > 0.0 && se < 1.0) || fabs(e - (t + h)) < 1e-12) break;
+          double ox[3], ov[3];
+          double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+          double sp = 1.0;
+#pragma unroll
+          for (int k = 0; k < 7; ++k) {
+            sp *= se;
+            const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              bw[d] += B[k][d] * wk;
+              bu[d] += B[k][d] * uk;
+            }
+          }
+          const double xv[3] = {x.x, x.y, x.z}, vv[3] = {v.x, v.y, v.z};
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            ox[d] = xv[d] + se * vv[d] * h + (se * se) * (h * h) * (as[d] / 2.0 + bw[d]);
+            ov[d] = vv[d] + se * h * (as[d] + bu[d]);
+          }
+          const int64_t q = (int64_t)ej * a.n + obj;
+          a.mout[0][q] = ox[0];
+          a.mout[1][q] = ox[1];
+          a.mout[2][q] = ox[2];
+          a.mout[3][q] = ov[0];
+          a.mout[4][q] = ov[1];
+          a.mout[5][q] = ov[2];
+          ++ej;
+        }
+      }
       x = Vec3{xs[0], xs[1], xs[2]};
       v = Vec3{vs[0], vs[1], vs[2]};
     }
@@ -609,9 +657,10 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         begin = true;
       }
     }
-    if (MULTI && done == ST_OK) {
-      // store the state at this epoch and carry on to the next one
-      for (;;) {
+    if (MULTI && (done == ST_OK || (conv && done < 0))) {
+      // the epochs at the current time: after an accepted step (one that
+      // ends on an epoch) and at the end of a run
+      while (ej < a.n_epochs && fabs(a.epochs[ej] - t) < 1e-12) {
         const int64_t q = (int64_t)ej * a.n + obj;
         a.mout[0][q] = x.x;
         a.mout[1][q] = x.y;
@@ -619,12 +668,14 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         a.mout[3][q] = v.x;
         a.mout[4][q] = v.y;
         a.mout[5][q] = v.z;
-        if (ej + 1 >= a.n_epochs || !(fabs(a.epochs[ej + 1] - t) < 1e-12)) break;
         ++ej;
       }
-      if (ej + 1 < a.n_epochs) {
-        ++ej;
-        tf = a.epochs[ej];
+    }
+    if (MULTI && done == ST_OK) {
+      // end of a run of epochs in one direction: integrate on to the end of
+      // the next run
+      if (ej < a.n_epochs) {
+        tf = a.epochs[multi_run_end(a, ej, t)];
         // the controller's proposal: before the last step was shortened to
         // land on the epoch, else after it
         const double hc = phase == PH_INIT ? 0.1 : (shortened ? h_want : h);
@@ -659,7 +710,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       Vec3 nx, nv;
       obj = fetch<PERSISTENT>(a, nxt, nt, ntf, nx, nv);
       t = nt;
-      tf = MULTI ? a.epochs[0] : ntf;
+      tf = MULTI ? a.epochs[multi_run_end(a, 0, nt)] : ntf;
       x = nx;
       v = nv;
       phase = PH_INIT;
