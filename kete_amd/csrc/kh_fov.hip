@@ -46,43 +46,86 @@ __device__ __forceinline__ bool rect_contains(const double* __restrict__ nrm, Ve
 
 constexpr int MAX_GROUP_FIELDS = 256;
 
+// Candidates collect in a per-block LDS list and reach the global list with
+// one atomic per block: a single global counter bumped per wavefront and
+// field serialised ~1e5 atomics per launch (5x the kernel's arithmetic).
+constexpr int PF_BLOCK_CAND = 1024;
+
 __global__ void __launch_bounds__(256) fov_prefilter_kernel(FovPrefilter p) {
-  __shared__ FieldDev sf[MAX_GROUP_FIELDS];
+  // the group's fields, sized to the group (dynamic LDS)
+  extern __shared__ FieldDev sf[];
+  __shared__ int32_t c_field[PF_BLOCK_CAND], c_kind[PF_BLOCK_CAND];
+  __shared__ int64_t c_obj[PF_BLOCK_CAND];
+  __shared__ uint32_t c_n;
+  __shared__ unsigned long long c_base;
   for (int k = threadIdx.x; k < p.nf; k += blockDim.x) sf[k] = p.fields[k];
+  if (threadIdx.x == 0) c_n = 0;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
-  const double jd = p.jd[i];
-  const Vec3 x{p.x[0][i], p.x[1][i], p.x[2][i]};
-  const Vec3 v{p.x[3][i], p.x[4][i], p.x[5][i]};
-  // objects whose re-propagation failed were dropped by kete (filter_map);
-  // their NaN states can never be seen, so they are skipped here.
-  if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
-        isfinite(v.z) && isfinite(jd)))
-    return;
-  for (int f = 0; f < p.nf; ++f) {
-    const FieldDev& F = sf[f];
+  bool active = i < p.n;
+  double jd = 0.0;
+  Vec3 x{0, 0, 0}, v{0, 0, 0};
+  if (active) {
+    jd = p.jd[i];
+    x = Vec3{p.x[0][i], p.x[1][i], p.x[2][i]};
+    v = Vec3{p.x[3][i], p.x[4][i], p.x[5][i]};
+    // objects whose re-propagation failed were dropped by kete (filter_map);
+    // their NaN states can never be seen, so they are skipped here.
+    active = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) &&
+             isfinite(v.y) && isfinite(v.z) && isfinite(jd);
+  }
+  for (int f = 0; active && f < p.nf; ++f) {
+    // the whole field at once, and no branches below
+    const FieldDev F = sf[f];
     const Vec3 dv{v.x - F.obs[3], v.y - F.obs[4], v.z - F.obs[5]};
     const double max_dist = norm(dv) * p.dt_limit * 2.0;
-    int32_t kind;
-    if (fabs(jd - F.jd) < p.dt_limit) {
-      // check_linear (fov_like.rs:43-66)
-      const Vec3 rel{x.x - F.obs[0], x.y - F.obs[1], x.z - F.obs[2]};
-      const double dt = F.jd - jd - norm(rel) * C_AU_PER_DAY_INV;
-      const Vec3 nr{(x.x + v.x * dt) - F.obs[0], (x.y + v.y * dt) - F.obs[1],
-                    (x.z + v.z * dt) - F.obs[2]};
-      double dist;
-      const bool inside = rect_contains(F.nrm, nr, dist);
-      if (!inside && dist > max_dist) continue;
-      kind = 0;
-    } else {
-      kind = 1;  // two-body prefilter, then n-body
+    // check_linear (fov_like.rs:43-66) and SphericalPolygon::contains
+    // (patches.rs:214-234), evaluated for every field; only fields within
+    // dt_limit of the states' epoch use it (kind 0), the others go to the
+    // two-body check unconditionally (kind 1)
+    const Vec3 rel{x.x - F.obs[0], x.y - F.obs[1], x.z - F.obs[2]};
+    const double dt = F.jd - jd - norm(rel) * C_AU_PER_DAY_INV;
+    const Vec3 nr{(x.x + v.x * dt) - F.obs[0], (x.y + v.y * dt) - F.obs[1],
+                  (x.z + v.z * dt) - F.obs[2]};
+    bool any_nan = false;
+    double closest = -__builtin_inf();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double d = nr.x * F.nrm[3 * e] + nr.y * F.nrm[3 * e + 1] + nr.z * F.nrm[3 * e + 2];
+      any_nan = any_nan || isnan(d);
+      closest = (signbit(d) && fabs(d) > closest) ? fabs(d) : closest;
     }
-    const unsigned long long slot = atomicAdd(p.n_cand, 1ull);
+    // NaN: Outside(NaN), never farther than max_dist; an edge with a
+    // negative side: Outside(min(closest, |nr|)); otherwise Inside
+    const bool outside_far = !any_nan && isfinite(closest) && fmin(closest, norm(nr)) > max_dist;
+    const bool near = fabs(jd - F.jd) < p.dt_limit;
+    if (near && outside_far) continue;
+    const int32_t kind = near ? 0 : 1;
+    const uint32_t k = atomicAdd(&c_n, 1u);
+    if (k < (uint32_t)PF_BLOCK_CAND) {
+      c_field[k] = f;
+      c_kind[k] = kind;
+      c_obj[k] = i;
+    } else {
+      // the block's list is full: straight to the global list
+      const unsigned long long slot = atomicAdd(p.n_cand, 1ull);
+      if (slot < (unsigned long long)p.cap) {
+        p.cand_field[slot] = f;
+        p.cand_kind[slot] = kind;
+        p.cand_obj[slot] = i;
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t nb = c_n < (uint32_t)PF_BLOCK_CAND ? c_n : (uint32_t)PF_BLOCK_CAND;
+  if (threadIdx.x == 0) c_base = nb ? atomicAdd(p.n_cand, (unsigned long long)nb) : 0ull;
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+    const unsigned long long slot = c_base + k;
     if (slot < (unsigned long long)p.cap) {
-      p.cand_field[slot] = f;
-      p.cand_kind[slot] = kind;
-      p.cand_obj[slot] = i;
+      p.cand_field[slot] = c_field[k];
+      p.cand_kind[slot] = c_kind[k];
+      p.cand_obj[slot] = c_obj[k];
     }
   }
 }
@@ -326,8 +369,8 @@ __global__ void radec_vmag_kernel(RadecArgs p) {
 int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s) {
   if (p.n <= 0 || p.nf <= 0) return 0;
   if (p.nf > MAX_GROUP_FIELDS) return -1;
-  hipLaunchKernelGGL(fov_prefilter_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s,
-                     p);
+  hipLaunchKernelGGL(fov_prefilter_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256),
+                     (size_t)p.nf * sizeof(FieldDev), s, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
