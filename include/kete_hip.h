@@ -211,6 +211,22 @@ int32_t kh_fov_state_check(kh_ctx *ctx, int64_t n, double jd0, const double *con
                            int64_t cap, int64_t *n_hits, int64_t *hit_field, int64_t *hit_obj,
                            double *const hit_state[7]);
 
+/* SPK objects in rectangular fields with light time.  Replaces
+ * fov_spk_checks_py (src/kete/rust/fovs/checks.rs:123-142) and the per-field
+ * FovLike::check_spks (src/kete_core/src/fov/fov_like.rs:196-226): for every
+ * field (sorted by epoch, stably) and object id, the object's state from the
+ * uploaded ephemeris (Type 21 segments or the planet tree) at the field
+ * epoch, Sun-centred in the field frame (ecliptic, 17); ids with no state
+ * there are skipped; then the two-body light-time step and the field test.
+ * Fields as for kh_fov_state_check.  Outputs in kete's order (field epoch
+ * order, then id order): the original field index, the index into obj_ids,
+ * and the state [7][cap] (jd, pos, vel).  *n_hits receives the count; a
+ * count above cap returns KH_E_ARG with *n_hits set to the size needed. */
+int32_t kh_fov_spk_check(kh_ctx *ctx, int64_t n_ids, const int64_t *obj_ids, int32_t frame,
+                         int64_t n_fields, const double *field_jd, const double *const obs[6],
+                         const double *normals, int64_t cap, int64_t *n_hits,
+                         int64_t *hit_field, int64_t *hit_obj, double *const hit_state[7]);
+
 /* RA, Dec, dRA/dt*cos(Dec), dDec/dt (radians, radians/day) and V magnitude
  * of n detections.  Replaces SimultaneousStates::ra_dec_with_rates
  * (src/kete_core/src/simult_states.rs:86-120) and HGParams::apparent_mag

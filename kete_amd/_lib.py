@@ -27,7 +27,7 @@ EXPORTS = [
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
-    "kh_autotune_clear",
+    "kh_autotune_clear", "kh_fov_spk_check",
 ]
 
 
@@ -135,6 +135,9 @@ def load():
     L.kh_fov_state_check.argtypes = [V, I64, D, PP, I32, I64, P, PP, P, D, I32, I64,
                                      C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
     L.kh_fov_state_check.restype = I32
+    L.kh_fov_spk_check.argtypes = [V, I64, C.POINTER(I64), I32, I64, P, PP, P, I64,
+                                   C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
+    L.kh_fov_spk_check.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32
     rp = [V, I64, P, PP, I32, C.POINTER(KhNonGrav), I32, P, PP, I32, I32, D, PP, PP, PP, PP,

@@ -218,6 +218,67 @@ __global__ void fov_nb_finish_kernel(FovNBody p) {
 }
 
 // ---------------------------------------------------------------------------
+// fov_spk_check: SPK objects in fields (FovLike::check_spks,
+// fov/fov_like.rs:196-226): the object's Sun-centred state at the field epoch
+// from the ephemeris (try_get_state with the observer's center and frame),
+// then check_two_body -- a zero-length two-body step, the light-time step,
+// containment.  Objects without a state at the epoch are skipped.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) fov_spk_kernel(const SpkDev* __restrict__ spk,
+                                                      const double* __restrict__ coef,
+                                                      FovSpk p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= p.n_obj * p.n_fields) return;
+  const int64_t fi = q / p.n_obj, k = q - fi * p.n_obj;
+  const SpkObjDesc d = p.obj[k];
+  if (d.kind == 0) return;
+  const FieldDev& F = p.fields[fi];
+  Vec3 pos, vel;
+  int32_t st;
+  bool sun_centred = false;
+  if (d.kind == 1) st = t21_state<true>(spk, d.t21_begin, d.t21_end, F.jd, pos, vel, sun_centred);
+  else st = chain_state(spk, coef, d.body, F.jd, pos, vel);
+  if (st == ST_OK && !sun_centred) {
+    Vec3 sp, sv;
+    st = chain_state(spk, coef, p.sun, F.jd, sp, sv);
+    pos = Vec3{pos.x - sp.x, pos.y - sp.y, pos.z - sp.z};
+    vel = Vec3{vel.x - sv.x, vel.y - sv.y, vel.z - sv.z};
+  }
+  if (st != ST_OK) return;
+  // the fields are ecliptic (kete converts FOVs to it, collection.rs:23-36)
+  pos = equ_to_ecl(pos);
+  vel = equ_to_ecl(vel);
+  Vec3 p1, v1, p2, v2;
+  if (analytic_2_body(0.0, pos, vel, p1, v1) != ST_OK) return;
+  const Vec3 d1{p1.x - F.obs[0], p1.y - F.obs[1], p1.z - F.obs[2]};
+  const double jd_obs = F.jd + (-norm(d1) * C_AU_PER_DAY_INV);
+  if (analytic_2_body(jd_obs - F.jd, p1, v1, p2, v2) != ST_OK) return;
+  const Vec3 rel{p2.x - F.obs[0], p2.y - F.obs[1], p2.z - F.obs[2]};
+  double dist;
+  if (!rect_contains(F.nrm, rel, dist)) return;
+  const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
+  if (slot < (unsigned long long)p.hit_cap) {
+    p.hit_field[slot] = (int32_t)fi;
+    p.hit_obj[slot] = k;
+    p.hit_state[0][slot] = jd_obs;
+    p.hit_state[1][slot] = p2.x;
+    p.hit_state[2][slot] = p2.y;
+    p.hit_state[3][slot] = p2.z;
+    p.hit_state[4][slot] = v2.x;
+    p.hit_state[5][slot] = v2.y;
+    p.hit_state[6][slot] = v2.z;
+  }
+}
+
+int32_t launch_fov_spk(const SpkDev* spk, const double* coef, const FovSpk& p, hipStream_t s) {
+  const int64_t pairs = p.n_obj * p.n_fields;
+  if (pairs <= 0) return 0;
+  hipLaunchKernelGGL(fov_spk_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s, spk,
+                     coef, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
 // Hit ordering: fields in epoch order, then objects by index, as kete returns
 // them (the rayon collect over sorted fields, checks.rs:87-104)
 // ---------------------------------------------------------------------------

@@ -428,6 +428,123 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
   return kh_fail(KH_E_HIP, "candidate lists overflowed twice");
 }
 
+int32_t kh_fov_spk_check(kh_ctx* c, int64_t n_ids, const int64_t* obj_ids, int32_t frame,
+                         int64_t n_fields, const double* field_jd, const double* const obs[6],
+                         const double* normals, int64_t cap, int64_t* n_hits, int64_t* hit_field,
+                         int64_t* hit_obj, double* const hit_state[7]) {
+  if (!c || !n_hits) return kh_fail(KH_E_ARG, "null argument");
+  *n_hits = 0;
+  if (n_ids < 0 || n_fields < 0 || cap < 0) return kh_fail(KH_E_ARG, "negative size");
+  if (frame != KH_FRAME_ECLIPTIC)
+    return kh_fail(KH_E_ARG, "fields must be ecliptic (kete converts FOVs to it)");
+  if (!c->has_spk) return kh_fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  if (n_ids == 0 || n_fields == 0) return KH_OK;
+  if (!obj_ids || !field_jd || !obs || !normals ||
+      (cap > 0 && (!hit_field || !hit_obj || !hit_state)))
+    return kh_fail(KH_E_ARG, "null array");
+  if (n_ids * n_fields >= ((int64_t)1 << 40)) return kh_fail(KH_E_ARG, "too many pairs");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // fields in epoch order (into_sorted_vec_fov: a stable sort by jd)
+  std::vector<int64_t> order(n_fields);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return field_jd[a] < field_jd[b]; });
+  std::vector<FieldDev> fields(n_fields);
+  for (int64_t k = 0; k < n_fields; ++k) {
+    const int64_t f = order[k];
+    FieldDev& F = fields[k];
+    F.jd = field_jd[f];
+    for (int q = 0; q < 6; ++q) F.obs[q] = obs[q][f];
+    for (int q = 0; q < 12; ++q) F.nrm[q] = normals[12 * f + q];
+    F.pad = 0.0;
+  }
+  // where each object's state comes from (SpkCollection::try_get_raw_state
+  // scans an object's segments, spk.rs:65-82): Type 21 segments, else the
+  // planet tree; objects in neither are never visible (filter_map)
+  std::vector<SpkObjDesc> desc(n_ids);
+  for (int64_t k = 0; k < n_ids; ++k) {
+    SpkObjDesc& d = desc[k];
+    std::memset(&d, 0, sizeof(d));
+    for (size_t q = 0; q < c->t21_obj.size(); ++q)
+      if (c->t21_obj[q] == obj_ids[k]) {
+        if (d.kind == 0) d.t21_begin = (int32_t)q;
+        d.kind = 1;
+        d.t21_end = (int32_t)q + 1;
+      }
+    if (d.kind == 0 && kh_build_chain(c, obj_ids[k], d.body)) d.kind = 2;
+  }
+  Chain sun{};
+  if (!kh_build_chain(c, 10, sun)) return kh_fail(KH_E_NO_SPK, "no path from the Sun to 0");
+  const int64_t hit_cap = std::max<int64_t>(cap, 1);
+  const size_t bd = al(n_ids * sizeof(SpkObjDesc)), bf = al(n_fields * sizeof(FieldDev));
+  const size_t need = bd + bf + al(8) + al(hit_cap * 4) + al(hit_cap * 8) * 8;
+  int32_t r = kh_grow(&c->fbuf, &c->fbuf_size, need);
+  if (r) return r;
+  char* q = (char*)c->fbuf;
+  SpkObjDesc* d_desc = (SpkObjDesc*)q;
+  FieldDev* d_fields = (FieldDev*)(q + bd);
+  unsigned long long* d_n = (unsigned long long*)(q + bd + bf);
+  int32_t* h_field = (int32_t*)(q + bd + bf + al(8));
+  char* q2 = q + bd + bf + al(8) + al(hit_cap * 4);
+  int64_t* h_obj = (int64_t*)q2;
+  double* h_state[7];
+  for (int k = 0; k < 7; ++k) h_state[k] = (double*)(q2 + (1 + k) * al(hit_cap * 8));
+  HIPCHK(hipMemcpyAsync(d_desc, desc.data(), n_ids * sizeof(SpkObjDesc), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_fields, fields.data(), n_fields * sizeof(FieldDev),
+                        hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(zero_counts, dim3(1), dim3(64), 0, s, d_n);
+  FovSpk p{};
+  p.n_obj = n_ids;
+  p.n_fields = n_fields;
+  p.obj = d_desc;
+  p.sun = sun;
+  p.fields = d_fields;
+  p.n_hit = d_n;
+  p.hit_cap = hit_cap;
+  p.hit_field = h_field;
+  p.hit_obj = h_obj;
+  for (int k = 0; k < 7; ++k) p.hit_state[k] = h_state[k];
+  if (launch_fov_spk(c->d_spk, c->d_coef, p, s)) return kh_fail(KH_E_HIP, "fov_spk launch failed");
+  unsigned long long total = 0;
+  HIPCHK(hipMemcpyAsync(&total, d_n, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *n_hits = (int64_t)total;
+  if ((int64_t)total > cap)
+    return kh_fail(KH_E_ARG, "hit capacity too small: need " + std::to_string(total));
+  if (total) {
+    auto bits = [](int64_t v) {
+      int32_t b = 1;
+      while (b < 63 && ((int64_t)1 << b) <= v) ++b;
+      return b;
+    };
+    const int64_t m = (int64_t)total;
+    const size_t tmp = fov_sort_temp_bytes(m);
+    const size_t b8 = al(m * 8);
+    r = kh_grow(&c->hbuf, &c->hbuf_size, al(n_fields * 8) + tmp + 9 * b8);
+    if (r) return r;
+    char* h = (char*)c->hbuf;
+    int64_t* d_order = (int64_t*)h;
+    h += al(n_fields * 8);
+    void* d_tmp = h;
+    h += tmp;
+    int64_t* o_field = (int64_t*)h;
+    int64_t* o_obj = (int64_t*)(h + b8);
+    double* o_state[7];
+    for (int k = 0; k < 7; ++k) o_state[k] = (double*)(h + (2 + k) * b8);
+    HIPCHK(hipMemcpyAsync(d_order, order.data(), n_fields * 8, hipMemcpyHostToDevice, s));
+    if (launch_fov_sort_hits(m, bits(n_ids - 1), bits(n_fields - 1), h_field, h_obj, h_state,
+                             d_order, d_tmp, tmp, o_field, o_obj, o_state, s))
+      return kh_fail(KH_E_HIP, "hit ordering failed");
+    HIPCHK(hipMemcpyAsync(hit_field, o_field, m * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hit_obj, o_obj, m * 8, hipMemcpyDeviceToHost, s));
+    for (int k = 0; k < 7; ++k)
+      HIPCHK(hipMemcpyAsync(hit_state[k], o_state[k], m * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  return KH_OK;
+}
+
 int32_t kh_radec_vmag(kh_ctx* c, int64_t n, const double* const state[6],
                       const double* const obs[6], int32_t frame, const double* h_mag,
                       double g_param, double* const out[5]) {

@@ -198,6 +198,28 @@ int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s);
 int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s);
 int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s);
 
+// Where an SPK object's state comes from (fov_spk_check): kind 0 not in
+// the ephemeris, 1 its Type 21 segments t21[t21_begin, t21_end), 2 the
+// planet-tree chain `body` to the barycentre.
+struct SpkObjDesc {
+  int32_t kind, t21_begin, t21_end, pad;
+  Chain body;
+};
+
+struct FovSpk {
+  int64_t n_obj;           // objects per field (pairs: n_fields x n_obj, objects fastest)
+  int64_t n_fields;
+  const SpkObjDesc* obj;
+  Chain sun;               // the Sun's chain (states are Sun-centred)
+  const FieldDev* fields;  // sorted by epoch
+  unsigned long long* n_hit;
+  int64_t hit_cap;
+  int32_t* hit_field;      // sorted field index
+  int64_t* hit_obj;        // object index
+  double* hit_state[7];
+};
+int32_t launch_fov_spk(const SpkDev* spk, const double* coef, const FovSpk& p, hipStream_t s);
+
 // Hits in kete's order -- sorted field index, then object index -- by a
 // device radix sort on packed keys, gathered into out_* with the original
 // field index order[sorted index].  temp/temp_bytes: scratch of at least

@@ -132,6 +132,50 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
         return FovHits(hf[:k], ho[:k], hs[:, :k])
 
 
+def fov_spk_check(obj_ids, fovs, ctx=None, fields=None, cap=None) -> FovHits:
+    """kete ``fov_spk_check`` (``src/kete/rust/fovs/checks.rs:123-142``):
+    which objects of the loaded ephemeris (NAIF ids, e.g. the Type 21 small
+    bodies) are visible in which fields, with light time.  ``obj`` of the
+    result indexes ``obj_ids``."""
+    from . import get_context
+    ctx = ctx or get_context()
+    L = ctx.L
+    ids = np.ascontiguousarray(obj_ids, dtype=np.int64)
+    if fields is None:
+        nf = len(fovs)
+        fjd = np.array([f.jd for f in fovs], dtype=np.float64)
+        obs = np.empty((6, nf))
+        nrm = np.empty((nf, 12))
+        for k, f in enumerate(fovs):
+            n_, p_, v_ = f.ecliptic()
+            nrm[k] = n_.reshape(12)
+            obs[:3, k] = p_
+            obs[3:, k] = v_
+    else:
+        fjd, obs, nrm = fields
+        fjd = np.ascontiguousarray(fjd, dtype=np.float64)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float64).reshape(-1, 12)
+        nf = len(fjd)
+    cap = cap or max(1024, min(len(ids) * max(nf, 1), 1 << 22))
+    while True:
+        hf = np.empty(cap, dtype=np.int64)
+        ho = np.empty(cap, dtype=np.int64)
+        hs = np.empty((7, cap))
+        nh = C.c_int64(0)
+        rc = L.kh_fov_spk_check(ctx.h, len(ids), _lib.iptr(ids, C.c_int64), FRAME_ECLIPTIC, nf,
+                                _lib.dptr(fjd), _lib.ptr_array([obs[k] for k in range(6)]),
+                                _lib.dptr(nrm.reshape(-1)), cap, C.byref(nh),
+                                _lib.iptr(hf, C.c_int64), _lib.iptr(ho, C.c_int64),
+                                _lib.ptr_array([hs[k] for k in range(7)]))
+        if rc == -1 and nh.value > cap:
+            cap = nh.value
+            continue
+        _lib.check(rc)
+        k = nh.value
+        return FovHits(hf[:k], ho[:k], hs[:, :k])
+
+
 def ra_dec_vmag(ctx, state, obs, h_mag, g_param=0.15, frame=FRAME_ECLIPTIC):
     """RA, Dec, RA rate x cos(Dec), Dec rate (rad, rad/day) and V magnitude per
     detection -> array [5, n]."""

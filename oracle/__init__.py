@@ -92,6 +92,9 @@ def lib():
         L.ko_fov_state_check_ext.restype = I64
         L.ko_fov_state_check_ext.argtypes = [I64, D, PP, I32, I64, P, P, P, D, C.c_int, I64,
                                              C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+        L.ko_fov_spk_check.restype = I64
+        L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
+                                       C.POINTER(I64), C.POINTER(I64), P]
         L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
         L.ko_hg_apparent_mag.restype = D
         L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
@@ -289,6 +292,26 @@ def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
                                       cap, hf.ctypes.data_as(C.POINTER(C.c_int64)),
                                       ho.ctypes.data_as(C.POINTER(C.c_int64)), _d(hs),
                                       n_threads or os.cpu_count())
+    if nh < 0:
+        raise RuntimeError("hit capacity exceeded")
+    return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()
+
+
+def fov_spk_check(ids, field_jd, obs_xyz, normals, frame=FRAME_ECLIPTIC, cap=None):
+    """fov_spk_check over fields in epoch order -> (field, id index, state[n, 7])."""
+    ids = np.ascontiguousarray(ids, dtype=np.int64)
+    field_jd = np.ascontiguousarray(field_jd, dtype=np.float64)
+    nf = len(field_jd)
+    obs_xyz = np.ascontiguousarray(obs_xyz, dtype=np.float64).reshape(nf, 6)
+    nrm = np.ascontiguousarray(normals, dtype=np.float64).reshape(nf, 12)
+    cap = cap or max(1024, len(ids) * nf)
+    hf = np.empty(cap, dtype=np.int64)
+    ho = np.empty(cap, dtype=np.int64)
+    hs = np.empty((cap, 7))
+    nh = lib().ko_fov_spk_check(len(ids), ids.ctypes.data_as(C.POINTER(C.c_int64)), frame, nf,
+                                _d(field_jd), _d(obs_xyz), _d(nrm), cap,
+                                hf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                ho.ctypes.data_as(C.POINTER(C.c_int64)), _d(hs))
     if nh < 0:
         raise RuntimeError("hit capacity exceeded")
     return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()

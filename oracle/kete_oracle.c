@@ -1565,6 +1565,38 @@ out:
     return nh;
 }
 
+/* fov_spk_checks_py (src/kete/rust/fovs/checks.rs:123-142) over fields given
+ * in epoch order, each by FovLike::check_spks (fov/fov_like.rs:196-226): the
+ * state of every id from the loaded SPK at the field epoch (the observer's
+ * center, the Sun, and the field's frame), ids without one skipped; then
+ * check_two_body; visible ones in id order. */
+int64_t ko_fov_spk_check(int64_t n_ids, const int64_t *ids, int32_t frame, int64_t n_fields,
+                         const double *field_jd, const double *obs_xyz, const double *normals,
+                         int64_t cap, int64_t *hit_field, int64_t *hit_obj, double *hit_state) {
+    int64_t nh = 0;
+    for (int64_t f = 0; f < n_fields; ++f) {
+        fov_t F = {normals + 12 * f, field_jd[f],
+                   {obs_xyz[6 * f], obs_xyz[6 * f + 1], obs_xyz[6 * f + 2]},
+                   {obs_xyz[6 * f + 3], obs_xyz[6 * f + 4], obs_xyz[6 * f + 5]}, frame, 0};
+        for (int64_t k = 0; k < n_ids; ++k) {
+            ko_state s = {F.jd, {0, 0, 0}, {0, 0, 0}, frame, 10};
+            if (ko_spk_get_state(ids[k], F.jd, 10, frame, s.pos, s.vel) != KO_OK) continue;
+            int inside = 0;
+            double dist;
+            ko_state out;
+            if (check_two_body(&F, &s, &inside, &dist, &out) != KO_OK || !inside) continue;
+            if (nh >= cap) return -1;
+            hit_field[nh] = f;
+            hit_obj[nh] = k;
+            double *hs = hit_state + 7 * nh;
+            hs[0] = out.jd;
+            for (int q = 0; q < 3; ++q) { hs[1 + q] = out.pos[q]; hs[4 + q] = out.vel[q]; }
+            nh++;
+        }
+    }
+    return nh;
+}
+
 int64_t ko_fov_state_check(int64_t n, double jd0, const double *const *xyz,
                            int32_t frame, int64_t n_fields,
                            const double *field_jd, const double *obs_xyz,

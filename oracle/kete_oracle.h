@@ -181,6 +181,14 @@ int64_t ko_fov_state_check_ext(int64_t n, double jd0, const double *const *xyz,
                                int include_asteroids, int64_t cap, int64_t *hit_field,
                                int64_t *hit_obj, double *hit_state, int n_threads);
 
+/* fov_spk_checks_py (src/kete/rust/fovs/checks.rs:123-142) with
+ * FovLike::check_spks (fov/fov_like.rs:196-226) per field; fields in epoch
+ * order; hits (field, id index, state [cap][7]) by field then id order.
+ * Returns the count, or -1 if cap is too small. */
+int64_t ko_fov_spk_check(int64_t n_ids, const int64_t *ids, int32_t frame, int64_t n_fields,
+                         const double *field_jd, const double *obs_xyz, const double *normals,
+                         int64_t cap, int64_t *hit_field, int64_t *hit_obj, double *hit_state);
+
 /* ---- post-detection epilogue --------------------------------------------- */
 /* ra_dec_with_rates for one state (simult_states.rs:86-120), equatorial. */
 void ko_ra_dec_with_rates(const double pos[3], const double vel[3],
