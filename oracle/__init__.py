@@ -92,6 +92,8 @@ def lib():
         L.ko_fov_state_check_ext.restype = I64
         L.ko_fov_state_check_ext.argtypes = [I64, D, PP, I32, I64, P, P, P, D, C.c_int, I64,
                                              C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+        L.ko_propagate_n_body_long.argtypes = [I64, D, PP, I32, D, P, C.POINTER(NonGrav), I64,
+                                               PP, P, C.c_int]
         L.ko_fov_spk_check.restype = I64
         L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
                                        C.POINTER(I64), C.POINTER(I64), P]
@@ -295,6 +297,24 @@ def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
     if nh < 0:
         raise RuntimeError("hit capacity exceeded")
     return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()
+
+
+def propagate_n_body_long(jd0, xyz, jd_final, frame=FRAME_ECLIPTIC, planet_xyz=None,
+                          nongravs=None, batch_size=10, n_threads=None):
+    """-> (status, out[6, n] Sun-centred equatorial, planets[9, 6] of the last batch)"""
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    out = np.empty((6, n))
+    planets = np.empty((9, 6))
+    pl = None
+    if planet_xyz is not None:
+        pl_arr = np.ascontiguousarray(planet_xyz, dtype=np.float64).reshape(9, 6)
+        pl = _d(pl_arr)
+    ng = (NonGrav * n)(*nongravs) if nongravs is not None else None
+    st = lib().ko_propagate_n_body_long(n, jd0, _ptrs(list(xyz)), frame, jd_final, pl, ng,
+                                        batch_size, _ptrs([out[k] for k in range(6)]),
+                                        _d(planets), n_threads or os.cpu_count())
+    return st, out, planets
 
 
 def fov_spk_check(ids, field_jd, obs_xyz, normals, frame=FRAME_ECLIPTIC, cap=None):

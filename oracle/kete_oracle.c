@@ -1280,6 +1280,273 @@ int ko_batch_n_body(int64_t n, const double *jd0, const double *const *xyz,
 }
 
 /* ------------------------------------------------------------------------ */
+/* propagate_n_body_long (src/kete/rust/propagation.rs:202-246):            */
+/* propagate_n_body_vec (propagation/mod.rs:142-237) with vec_accel          */
+/* (acceleration.rs:197-249) over SIMPLE_PLANETS (gravity.rs:137-154), one   */
+/* RadauIntegrator over the whole batch state (radau.rs:119-346, any dim).  */
+/* ------------------------------------------------------------------------ */
+#define N_SIMPLE 9
+static const grav_params SIMPLE_PLANETS[N_SIMPLE] = {
+    {10, GMS, 0.004654758765894654},
+    {1, 1.66012082548908e-07 * GMS, 1.63139354095098e-05},
+    {2, 2.44783828779694e-06 * GMS, 4.04537843465442e-05},
+    /* EM_BARY (gravity.rs:137-143): Earth + Moon mass, Earth-Moon distance */
+    {3, (3.00348961546514e-06 + 3.69430335010988e-08) * GMS, 385000.0 / AU_KM},
+    {4, 3.22715608291416e-07 * GMS, 2.27021279387769e-05},
+    {5, 0.0009547919099414246 * GMS, 0.00047789450254521576},
+    {6, 0.00028588567002459455 * GMS, 0.0004028666966848747},
+    {7, 4.36624961322212e-05 * GMS, 0.0001708513622580592},
+    {8, 5.15138377265457e-05 * GMS, 0.0001655371154958558},
+};
+
+typedef int (*vec_force_fn)(double t, const double *x, const double *v, int exact,
+                            double *a, void *ctx);
+
+typedef struct {
+    int64_t dim;
+    vec_force_fn f;
+    void *ctx;
+    double t;
+    double *x, *v, *a;
+    double *B;   /* [7][dim] */
+    double *G;   /* [7][dim] */
+    double *xs, *vs, *ev, *bprev, *gk;
+} radau_vec_t;
+
+/* RadauIntegrator::step (radau.rs:225-346) for a state of any dimension:
+ * convergence and the step ratio take maxima over all components. */
+static int radau_vec_step(radau_vec_t *R, double h, double *next_h) {
+    const int64_t D = R->dim;
+    memset(R->G, 0, sizeof(double) * 7 * D);
+    memcpy(R->ev, R->a, sizeof(double) * D);
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        for (int64_t d = 0; d < D; ++d) R->bprev[d] = R->B[6 * D + d];
+        for (int j = 1; j < 8; ++j) {
+            double s = H_SP[j];
+            for (int64_t d = 0; d < D; ++d) {
+                double bw = 0.0, bu = 0.0;
+                for (int k = 0; k < 7; ++k) bw += R->B[k * D + d] * WPOW[j][k];
+                for (int k = 0; k < 7; ++k) bu += R->B[k * D + d] * UPOW[j][k];
+                R->xs[d] = R->x[d] + s * R->v[d] * h + (s * s) * (h * h) * (R->a[d] / 2.0 + bw);
+                R->vs[d] = R->v[d] + s * h * (R->a[d] + bu);
+            }
+            int rc = R->f(R->t + s * h, R->xs, R->vs, 0, R->ev, R->ctx);
+            if (rc) return rc;
+            for (int64_t d = 0; d < D; ++d) {
+                double g = (R->ev[d] - R->a[d]) / s;
+                for (int z = 1; z < j; ++z) g = (g - R->G[(z - 1) * D + d]) / (s - H_SP[z]);
+                R->G[(j - 1) * D + d] = g;
+            }
+        }
+        for (int c = 0; c < 7; ++c)
+            for (int64_t d = 0; d < D; ++d) {
+                double acc = R->G[d] * C_MAT[0][c];
+                for (int k = 1; k < 7; ++k) acc += R->G[k * D + d] * C_MAT[k][c];
+                R->B[c * D + d] = acc;
+            }
+        double rmax = -INFINITY, fmax_ = -INFINITY, bmax = -INFINITY;
+        for (int64_t d = 0; d < D; ++d) {
+            double fm = fabs(R->ev[d]) + 1e-6;
+            double r = fabs(R->B[6 * D + d] - R->bprev[d]) / fm;
+            rmax = fmax(rmax, r);
+            fmax_ = fmax(fmax_, fm);
+            bmax = fmax(bmax, fabs(R->B[6 * D + d]));
+        }
+        if (rmax < 1e-14) {
+            for (int64_t d = 0; d < D; ++d) {
+                double bw = 0.0, bu = 0.0;
+                for (int k = 0; k < 7; ++k) bw += R->B[k * D + d] * W_VEC[k];
+                for (int k = 0; k < 7; ++k) bu += R->B[k * D + d] * U_VEC[k];
+                R->x[d] += R->v[d] * h + (h * h) * (R->a[d] * 0.5 + bw);
+                R->v[d] += h * (R->a[d] + bu);
+            }
+            R->t += h;
+            int rc = R->f(R->t, R->x, R->v, 1, R->a, R->ctx);
+            if (rc) return rc;
+            double ratio7 = pow(RADAU_EPSILON * fmax_ / bmax, 1.0 / 7.0);
+            if (ratio7 < MIN_RATIO) ratio7 = MIN_RATIO;
+            else if (ratio7 > 1.0 / MIN_RATIO) ratio7 = 1.0 / MIN_RATIO;
+            *next_h = h * ratio7;
+            return KO_OK;
+        }
+    }
+    return KO_CONVERGENCE;
+}
+
+/* RadauIntegrator::new + integrate (radau.rs:119-218), any dimension. */
+static int radau_vec_integrate(vec_force_fn f, void *ctx, int64_t D, double *x, double *v,
+                               double t0, double tf) {
+    pthread_once(&radau_once, radau_tables);
+    radau_vec_t R;
+    memset(&R, 0, sizeof(R));
+    R.dim = D; R.f = f; R.ctx = ctx; R.t = t0;
+    double *mem = calloc((size_t)D * 21, sizeof(double));
+    if (!mem) return KO_VALUE_ERROR;
+    R.x = x; R.v = v;
+    R.a = mem; R.B = mem + D; R.G = mem + 8 * D; R.xs = mem + 15 * D; R.vs = mem + 16 * D;
+    R.ev = mem + 17 * D; R.bprev = mem + 18 * D; R.gk = mem + 19 * D;
+    int rc = f(t0, R.x, R.v, 1, R.a, ctx);
+    if (rc) { free(mem); return rc; }
+    if (fabs(tf - t0) < 1e-10) { free(mem); return KO_OK; }
+    double h = copysign(0.1, tf - R.t);
+    int fails = 0;
+    for (;;) {
+        if (fabs(R.t - tf) <= fabs(h)) h = tf - R.t;
+        double nh;
+        rc = radau_vec_step(&R, h, &nh);
+        if (rc == KO_OK) {
+            h = nh;
+            if (fabs(R.t - tf) < 1e-12) { free(mem); return KO_OK; }
+            fails = 0;
+        } else if (rc == KO_IMPACT || rc == KO_DAF_LIMITS) {
+            free(mem);
+            return rc;
+        } else {
+            fails++;
+            h *= 0.7;
+            if (fails > 10) { free(mem); return KO_CONVERGENCE; }
+        }
+        if (fabs(h) < MIN_STEP) h = copysign(MIN_STEP, h);
+    }
+}
+
+typedef struct {
+    int64_t n_obj;            /* massive + particles */
+    const ko_nongrav *ng;     /* per particle (kind 0: none) */
+    int64_t impact_id;
+    double impact_jd;
+} vec_ctx;
+
+/* vec_accel (acceleration.rs:197-249).  kete applies a particle's non-grav
+ * model only for idx > n_massive, so the first particle of a batch never
+ * gets its model; restated as is. */
+static int vec_force(double t, const double *x, const double *v, int exact, double *a,
+                     void *vctx) {
+    vec_ctx *c = vctx;
+    for (int64_t i = 0; i < 3 * c->n_obj; ++i) a[i] = 0.0;
+    for (int64_t idx = 0; idx < c->n_obj; ++idx) {
+        for (int idy = 0; idy < N_SIMPLE; ++idy) {
+            if (idx == idy) continue;
+            const grav_params *gp = &SIMPLE_PLANETS[idy];
+            double w[3] = {0.0, 0.0, 0.0}, rp[3], rv[3];
+            for (int k = 0; k < 3; ++k) {
+                rp[k] = x[3 * idx + k] - x[3 * idy + k];
+                rv[k] = v[3 * idx + k] - v[3 * idy + k];
+            }
+            if (exact && norm3(rp) <= gp->radius) {
+                c->impact_id = gp->naif_id;
+                c->impact_jd = t;
+                return KO_IMPACT;
+            }
+            grav_add_acceleration(gp, w, rp, rv);
+            if (gp->naif_id == 10 && idx > N_SIMPLE) {
+                const ko_nongrav *ng = c->ng ? &c->ng[idx - N_SIMPLE] : NULL;
+                if (ng && ng->kind) nongrav_add_acceleration(ng, w, rp, rv);
+            }
+            for (int k = 0; k < 3; ++k) a[3 * idx + k] += w[k];
+        }
+    }
+    return KO_OK;
+}
+
+/* propagate_n_body_vec (propagation/mod.rs:142-237) for one batch: m
+ * particles at jd0 (Sun-centred equatorial), the 9 SIMPLE_PLANETS states
+ * planets[9][6] (Sun-centred equatorial at jd0); out: particles then planets,
+ * Sun-centred equatorial at jd_final. */
+static int long_batch(int64_t m, double jd0, const double *px, const ko_nongrav *ng,
+                      const double *planets, double jd_final, double *out_part,
+                      double *out_planets) {
+    const int64_t n = N_SIMPLE + m, D = 3 * n;
+    double *x = malloc(sizeof(double) * 2 * D);
+    if (!x) return KO_VALUE_ERROR;
+    double *v = x + D;
+    for (int b = 0; b < N_SIMPLE; ++b)
+        for (int k = 0; k < 3; ++k) { x[3 * b + k] = planets[6 * b + k]; v[3 * b + k] = planets[6 * b + 3 + k]; }
+    for (int64_t i = 0; i < m; ++i)
+        for (int k = 0; k < 3; ++k) {
+            x[3 * (N_SIMPLE + i) + k] = px[6 * i + k];
+            v[3 * (N_SIMPLE + i) + k] = px[6 * i + 3 + k];
+        }
+    vec_ctx c = {n, ng, 0, 0.0};
+    int rc = radau_vec_integrate(vec_force, &c, D, x, v, jd0, jd_final);
+    if (rc == KO_OK) {
+        /* states relative to the Sun's final state */
+        for (int64_t i = 0; i < n; ++i) {
+            double *o = i < N_SIMPLE ? out_planets + 6 * i : out_part + 6 * (i - N_SIMPLE);
+            for (int k = 0; k < 3; ++k) {
+                o[k] = x[3 * i + k] - x[k];
+                o[3 + k] = v[3 * i + k] - v[k];
+            }
+        }
+    }
+    free(x);
+    return rc;
+}
+
+typedef struct {
+    int64_t n, batch;
+    double jd0, jd_final;
+    const double *px;        /* [n][6] equatorial Sun-centred */
+    const ko_nongrav *ng;    /* [n] or NULL */
+    const double *planets;   /* [9][6] */
+    double *out;             /* [n][6] */
+    double *planets_out;     /* [n_batches][9][6] */
+    int32_t *status;         /* [n_batches] */
+} long_ctx;
+
+static void long_range(int64_t lo, int64_t hi, void *vctx) {
+    long_ctx *c = vctx;
+    for (int64_t b = lo; b < hi; ++b) {
+        const int64_t i0 = b * c->batch;
+        const int64_t m = i0 + c->batch <= c->n ? c->batch : c->n - i0;
+        c->status[b] = long_batch(m, c->jd0, c->px + 6 * i0, c->ng ? c->ng + i0 : NULL,
+                                  c->planets, c->jd_final, c->out + 6 * i0,
+                                  c->planets_out + 54 * b);
+    }
+}
+
+int ko_propagate_n_body_long(int64_t n, double jd0, const double *const *xyz, int32_t frame,
+                             double jd_final, const double *planet_xyz, const ko_nongrav *ng,
+                             int64_t batch_size, double *const *out, double *planets_out,
+                             int n_threads) {
+    if (n <= 0) return KO_VALUE_ERROR;        /* "State vector is empty" */
+    if (batch_size <= 0) return KO_VALUE_ERROR;
+    double planets[N_SIMPLE * 6];
+    if (planet_xyz) {
+        memcpy(planets, planet_xyz, sizeof(planets));
+    } else {
+        for (int b = 0; b < N_SIMPLE; ++b)
+            if (ko_spk_get_state(SIMPLE_PLANETS[b].naif_id, jd0, 10, KO_FRAME_EQUATORIAL,
+                                 planets + 6 * b, planets + 6 * b + 3) != KO_OK)
+                return KO_DAF_LIMITS;
+    }
+    double *px = malloc(sizeof(double) * 6 * n);
+    double *po = malloc(sizeof(double) * 6 * n);
+    const int64_t nb = (n + batch_size - 1) / batch_size;
+    double *pl = malloc(sizeof(double) * 54 * nb);
+    int32_t *st = malloc(sizeof(int32_t) * nb);
+    for (int64_t i = 0; i < n; ++i) {
+        ko_state s = {jd0, {xyz[0][i], xyz[1][i], xyz[2][i]}, {xyz[3][i], xyz[4][i], xyz[5][i]},
+                      frame, 10};
+        int rc = ko_change_frame(&s, KO_FRAME_EQUATORIAL);
+        if (rc) { free(px); free(po); free(pl); free(st); return rc; }
+        memcpy(px + 6 * i, s.pos, 24);
+        memcpy(px + 6 * i + 3, s.vel, 24);
+    }
+    long_ctx c = {n, batch_size, jd0, jd_final, px, ng, planets, po, pl, st};
+    parallel_for(nb, 1, n_threads, long_range, &c);
+    int rc = KO_OK;
+    for (int64_t b = 0; b < nb && rc == KO_OK; ++b) rc = st[b];
+    if (rc == KO_OK) {
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 6; ++k) out[k][i] = po[6 * i + k];
+        if (planets_out) memcpy(planets_out, pl + 54 * (nb - 1), sizeof(double) * 54);
+    }
+    free(px); free(po); free(pl); free(st);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FOV: fov/patches.rs, fov/fov_like.rs, fov/generic.rs                      */
 /* ------------------------------------------------------------------------ */
 /* rotate_around (frames/definitions.rs:356-363): Rodrigues rotation matrix

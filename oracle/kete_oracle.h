@@ -147,6 +147,21 @@ int ko_batch_n_body_ext(int64_t n, const double *jd0, const double *const *xyz,
                         double *const *out, int32_t *status, int64_t *impact_id,
                         double *impact_jd, ko_counters *cnt, int n_threads);
 
+/* propagate_n_body_long (src/kete/rust/propagation.rs:202-246): the states
+ * (all at jd0, Sun-centred, in `frame`) in consecutive batches of
+ * batch_size, each integrated together with kete's SIMPLE_PLANETS (Sun,
+ * Mercury, Venus, Earth-Moon barycentre, Mars..Neptune; gravity.rs:137-154)
+ * as one Radau state (propagate_n_body_vec, propagation/mod.rs:142-237;
+ * vec_accel, acceleration.rs:197-249).  planet_xyz: [9][6] Sun-centred
+ * equatorial at jd0, or NULL to read them from the loaded SPK.  out: [6][n]
+ * Sun-centred EQUATORIAL at jd_final (kete returns that frame); planets_out
+ * ([9][6], nullable): the last batch's planets.  Returns the first failing
+ * batch's status (kete fails the whole call). */
+int ko_propagate_n_body_long(int64_t n, double jd0, const double *const *xyz, int32_t frame,
+                             double jd_final, const double *planet_xyz, const ko_nongrav *ng,
+                             int64_t batch_size, double *const *out, double *planets_out,
+                             int n_threads);
+
 /* ---- FOV ----------------------------------------------------------------- */
 /* OnSkyRectangle::new (fov/patches.rs:122-155): normals[4][3]. */
 void ko_rect_normals(const double pointing[3], double rotation, double lon_w,
