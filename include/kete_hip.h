@@ -278,6 +278,28 @@ int32_t kh_radec_partials_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                               double *const dstate[6], int32_t *status, kh_counters *counters,
                               int32_t sync);
 
+/* propagate_n_body_long: particles integrated together with kete's
+ * SIMPLE_PLANETS (Sun, Mercury, Venus, Earth-Moon barycentre, Mars .. Neptune)
+ * as one N-body system, no ephemeris during the integration.  Replaces
+ * propagation_n_body_py (src/kete/rust/propagation.rs:202-246): the states
+ * in consecutive batches of batch_size, each batch with its own copy of the
+ * planets integrated by propagate_n_body_vec
+ * (src/kete_core/src/propagation/mod.rs:142-237) with vec_accel
+ * (src/kete_core/src/propagation/acceleration.rs:197-249).  States: n
+ * objects at jd0, center 10 (Sun), `frame`; planet_xyz: [9][6] Sun-centred
+ * equatorial at jd0, or null for the ephemeris' states.  out: [6][n]
+ * Sun-centred EQUATORIAL at jd_final, as kete returns them; planets_out
+ * ([9][6], nullable): the last batch's planets.  *status: kete's error for
+ * the call (0 ok; 2 ValueError for an empty batch or a center other than 10;
+ * 1 Convergence or 6 Impact of the first failing batch; 3 DAFLimits when the
+ * ephemeris lacks a planet), out undefined unless 0.  batch_size 1 .. 55
+ * (kete's default is 10). */
+int32_t kh_propagate_n_body_long(kh_ctx *ctx, int64_t n, double jd0, const double *const xyz[6],
+                                 int64_t center, int32_t frame, double jd_final,
+                                 const double *planet_xyz, const kh_nongrav *ng,
+                                 int32_t batch_size, double *const out[6], double *planets_out,
+                                 int32_t *status);
+
 /* Launch shape picked by kh_autotune_n_body. */
 typedef struct {
     int32_t mode;       /* 0-3 as kh_propagate_n_body's mode */

@@ -359,6 +359,61 @@ def propagate_two_body(states, jd, observer_pos=None):
     return res if soa else res.to_states()
 
 
+def propagate_n_body_long(states, jd_final, planet_states=None, non_gravs=None, batch_size=10):
+    """kete ``propagate_n_body_long`` (``src/kete/rust/propagation.rs:202-246``):
+    the states (one epoch, Sun-centred) integrated in batches of
+    ``batch_size`` together with the Sun, Mercury, Venus, the Earth-Moon
+    barycentre and Mars to Neptune, with no ephemeris during the arc.
+    Returns ``(states, planets)`` at ``jd_final``, Sun-centred in the
+    equatorial frame as kete returns them; ``planets`` are the last batch's.
+    ``planet_states``: nine Sun-centred states at the input epoch (any
+    frame), else the loaded ephemeris' states.  Errors raise ValueError, as
+    kete does for the whole call."""
+    batch, soa = _as_batch(states)
+    n = len(batch)
+    if n == 0:
+        raise ValueError("State vector is empty, propagation cannot continue")
+    jd0 = float(batch.jd[0])
+    if np.any(batch.jd != jd0):
+        raise ValueError("All input states must have the same JD")
+    if non_gravs is not None and len(non_gravs) != n:
+        raise ValueError("Number of non-grav models doesnt match the number of provided objects.")
+    ctx = get_context()
+    pl = None
+    if planet_states is not None:
+        pb, _ = _as_batch(planet_states)
+        if len(pb) != 9:
+            raise ValueError("Input planet states must contain the correct number of states.")
+        if pb.jd[0] != jd0:
+            raise ValueError("Planet states JD must match JD of input state.")
+        pl = np.empty((9, 6))
+        for k in range(9):
+            p, v = pb.xyz[:3, k], pb.xyz[3:, k]
+            if pb.frame == FRAME_ECLIPTIC:
+                c, sn = np.cos(0.40909280422232897), np.sin(0.40909280422232897)
+                p = np.array([p[0], c * p[1] - sn * p[2], sn * p[1] + c * p[2]])
+                v = np.array([v[0], c * v[1] - sn * v[2], sn * v[1] + c * v[2]])
+            pl[k, :3], pl[k, 3:] = p, v
+    ng = None
+    if non_gravs is not None:
+        ng_arr = non_gravs if isinstance(non_gravs, np.ndarray) else nongrav_array(non_gravs, n)
+        ng = ng_arr.ctypes.data_as(C.POINTER(_lib.KhNonGrav))
+    out = np.empty((6, n))
+    planets = np.empty((9, 6))
+    st = C.c_int32(0)
+    _lib.check(ctx.L.kh_propagate_n_body_long(
+        ctx.h, n, jd0, _lib.ptr_array(list(batch.xyz)), batch.center_id, batch.frame,
+        float(jd_final), _lib.dptr(pl.reshape(-1)) if pl is not None else None, ng,
+        int(batch_size), _lib.ptr_array([out[k] for k in range(6)]),
+        _lib.dptr(planets.reshape(-1)), C.byref(st)))
+    if st.value != 0:
+        raise ValueError(f"propagate_n_body_long: {STATUS_NAMES.get(st.value, st.value)}")
+    res = StateArray(np.full(n, float(jd_final)), out, FRAME_EQUATORIAL, 10, batch.desigs)
+    pls = StateArray(np.full(9, float(jd_final)), np.ascontiguousarray(planets.T),
+                     FRAME_EQUATORIAL, 10)
+    return (res, pls) if soa else (res.to_states(), pls.to_states())
+
+
 def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppress_errors=True,
                      suppress_impact_errors=False):
     """kete ``propagate_n_body`` (``src/kete/rust/propagation.rs:70-151``):

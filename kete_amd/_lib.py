@@ -27,7 +27,7 @@ EXPORTS = [
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
-    "kh_autotune_clear", "kh_fov_spk_check",
+    "kh_autotune_clear", "kh_fov_spk_check", "kh_propagate_n_body_long",
 ]
 
 
@@ -138,6 +138,9 @@ def load():
     L.kh_fov_spk_check.argtypes = [V, I64, C.POINTER(I64), I32, I64, P, PP, P, I64,
                                    C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
     L.kh_fov_spk_check.restype = I32
+    L.kh_propagate_n_body_long.argtypes = [V, I64, D, PP, I64, I32, D, P, C.POINTER(KhNonGrav),
+                                           I32, PP, P, C.POINTER(I32)]
+    L.kh_propagate_n_body_long.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32
     rp = [V, I64, P, PP, I32, C.POINTER(KhNonGrav), I32, P, PP, I32, I32, D, PP, PP, PP, PP,
@@ -174,10 +177,14 @@ def iptr(a: np.ndarray, ct):
 
 
 def ptr_array(ptrs):
-    """Array of double* from numpy arrays or raw device addresses (ints)."""
+    """Array of double* from numpy arrays or raw device addresses (ints).
+    The numpy arrays are kept alive with the pointer array (a pointer stored
+    into a ctypes array does not hold its array, so a temporary passed here
+    would otherwise be freed before the call reads it)."""
     arr = (C.POINTER(C.c_double) * len(ptrs))()
     for k, p in enumerate(ptrs):
         arr[k] = C.cast(C.c_void_p(p), C.POINTER(C.c_double)) if isinstance(p, int) else dptr(p)
+    arr._keep = [p for p in ptrs if not isinstance(p, int)]
     return arr
 
 

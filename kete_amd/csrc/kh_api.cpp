@@ -843,6 +843,115 @@ int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
 }
 
 // ---------------------------------------------------------------------------
+// propagate_n_body_long
+// ---------------------------------------------------------------------------
+int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double* const xyz[6],
+                                 int64_t center, int32_t frame, double jd_final,
+                                 const double* planet_xyz, const kh_nongrav* ng,
+                                 int32_t batch_size, double* const out[6], double* planets_out,
+                                 int32_t* status) {
+  if (!c || !status) return fail(KH_E_ARG, "null argument");
+  *status = ST_OK;
+  if (batch_size < 1 || batch_size > 55)
+    return fail(KH_E_ARG, "batch_size must be 1 .. 55 (a batch and its nine planets share one "
+                          "wavefront)");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  // propagate_n_body_vec's own checks (propagation/mod.rs:148-160,190-208)
+  if (n <= 0) {
+    *status = ST_VALUE_ERROR;  // "State vector is empty"
+    return KH_OK;
+  }
+  if (!xyz || !out) return fail(KH_E_ARG, "null array");
+  for (int k = 0; k < 6; ++k)
+    if (!xyz[k] || !out[k]) return fail(KH_E_ARG, "null state column");
+  if (center != 10) {
+    *status = ST_VALUE_ERROR;  // "Center of all states must be 10 (the Sun)."
+    return KH_OK;
+  }
+  if (ng)
+    for (int64_t i = 0; i < n; ++i)
+      if (ng[i].kind == 1 && ng[i].dt != 0.0)
+        return fail(KH_E_ARG, "JplComet non-grav with a time delay (dt != 0) is not supported");
+  HIPCHK(hipSetDevice(c->device));
+  // SIMPLE_PLANETS at jd0, Sun-centred equatorial: given, or from the
+  // ephemeris (try_get_state(id, jd_init, 10, Equatorial))
+  double planets[54];
+  if (planet_xyz) {
+    std::memcpy(planets, planet_xyz, sizeof(planets));
+  } else {
+    if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+    static const int64_t ids[9] = {10, 1, 2, 3, 4, 5, 6, 7, 8};
+    for (int b = 0; b < 9; ++b) {
+      double col[6];
+      double* cols[6] = {&col[0], &col[1], &col[2], &col[3], &col[4], &col[5]};
+      int32_t st = 0;
+      int32_t r = kh_spk_states(c, ids[b], 10, 1, 1, &jd0, cols, &st);
+      if (r) return r;
+      if (st != ST_OK) {
+        *status = st;
+        return KH_OK;
+      }
+      for (int k = 0; k < 6; ++k) planets[6 * b + k] = col[k];
+    }
+  }
+  const int64_t nb = (n + batch_size - 1) / batch_size;
+  const size_t bn = align256(n * 8), bp = align256(54 * 8), bpo = align256(nb * 54 * 8),
+               bs = align256(nb * 4), bng = align256(n * sizeof(kh_nongrav));
+  char* base;
+  int32_t r = staging(c, 12 * bn + bp + bpo + bs + (ng ? bng : 0), &base);
+  if (r) return r;
+  char* q = base;
+  const double* d_x[6];
+  double* d_o[6];
+  for (int k = 0; k < 6; ++k, q += bn) d_x[k] = (double*)q;
+  for (int k = 0; k < 6; ++k, q += bn) d_o[k] = (double*)q;
+  double* d_pl = (double*)q;
+  q += bp;
+  double* d_plo = (double*)q;
+  q += bpo;
+  int32_t* d_st = (int32_t*)q;
+  q += bs;
+  kh_nongrav* d_ng = ng ? (kh_nongrav*)q : nullptr;
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync((void*)d_x[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_pl, planets, sizeof(planets), hipMemcpyHostToDevice, c->stream));
+  if (ng) HIPCHK(hipMemcpyAsync(d_ng, ng, n * sizeof(kh_nongrav), hipMemcpyHostToDevice, c->stream));
+  LongLaunch p{};
+  p.n = n;
+  p.batch = batch_size;
+  p.frame = frame;
+  p.jd0 = jd0;
+  p.jd_final = jd_final;
+  for (int k = 0; k < 6; ++k) {
+    p.xyz[k] = d_x[k];
+    p.out[k] = d_o[k];
+  }
+  p.planets = d_pl;
+  p.ng = d_ng;
+  p.planets_out = d_plo;
+  p.status = d_st;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (launch_nbody_long(p, c->stream)) return fail(KH_E_HIP, "n-body long launch failed");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  std::vector<int32_t> st(nb);
+  HIPCHK(hipMemcpyAsync(st.data(), d_st, nb * 4, hipMemcpyDeviceToHost, c->stream));
+  for (int k = 0; k < 6; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], d_o[k], n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (planets_out)
+    HIPCHK(hipMemcpyAsync(planets_out, d_plo + (nb - 1) * 54, 54 * 8, hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // kete collects the batches into one Result: the first failure fails the call
+  for (int64_t b = 0; b < nb; ++b)
+    if (st[b] != ST_OK) {
+      *status = st[b];
+      break;
+    }
+  return KH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Autotuner: launch shape of the N-body path measured on the caller's batch
 // ---------------------------------------------------------------------------
 int32_t kh_autotune_n_body(kh_ctx* c, int64_t n, const double* jd0, const double* const xyz[6],

@@ -80,6 +80,24 @@ inline size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 +
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket
 // the integrator kernel alone.
+// propagate_n_body_long: batches of `batch` particles, each integrated with
+// the nine SIMPLE_PLANETS as one Radau state (one lane per body, a group of
+// 32 or 64 lanes per batch).
+struct LongLaunch {
+  int64_t n;          // particles
+  int32_t batch;      // particles per batch (<= 55)
+  int32_t frame;      // input frame (1 or 17)
+  double jd0, jd_final;
+  const double* xyz[6];     // particles at jd0, Sun-centred, in `frame`
+  const double* planets;    // [9][6] Sun-centred equatorial at jd0
+  const void* ng;           // [n] kh_nongrav or null
+  double* out[6];           // particles at jd_final, Sun-centred equatorial
+  double* planets_out;      // [n_batches][9][6]
+  int32_t* status;          // [n_batches]
+  int32_t* counters;        // [n_batches][4] or null
+};
+int32_t launch_nbody_long(const LongLaunch& p, hipStream_t stream);
+
 int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
                      hipStream_t stream, int32_t* used_block, int32_t* used_grid,
                      int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1);

@@ -1196,6 +1196,314 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------------------
+// propagate_n_body_long (src/kete/rust/propagation.rs:202-246): each batch of
+// particles is integrated with kete's SIMPLE_PLANETS as one Radau state
+// (propagate_n_body_vec, propagation/mod.rs:142-237), the bodies pulling on
+// each other and on the particles (vec_accel, acceleration.rs:197-249).
+// One lane per body of the batch, in kete's order (the nine planets, then the
+// particles); a batch takes an aligned group of 32 (<= 23 particles) or 64
+// lanes, and every integrator decision is the group's: convergence is the
+// largest |dB6|/(|a|+1e-6) over all its components, the step ratio uses the
+// largest |a|+1e-6 and |B6| (radau.rs:313-342), and an impact anywhere ends
+// the batch.  The massive bodies' positions at each evaluation point reach
+// the other lanes by shuffles.
+// ---------------------------------------------------------------------------
+// SIMPLE_PLANETS (gravity.rs:150-154, EM_BARY gravity.rs:137-143):
+// id, mass, radius, J2 and the terms of GravParams::add_acceleration.
+__constant__ BodyDev kSimplePlanets[9] = {
+    {-1, 0, 10, 1, GMS, 0.004654758765894654, SUN_J2, 0.0},
+    {-1, 0, 1, 0, 1.66012082548908e-07 * GMS, 1.63139354095098e-05, 0.0, 0.0},
+    {-1, 0, 2, 0, 2.44783828779694e-06 * GMS, 4.04537843465442e-05, 0.0, 0.0},
+    {-1, 0, 3, 0, (3.00348961546514e-06 + 3.69430335010988e-08) * GMS, 385000.0 / AU_KM, 0.0,
+     0.0},
+    {-1, 0, 4, 0, 3.22715608291416e-07 * GMS, 2.27021279387769e-05, 0.0, 0.0},
+    {-1, 0, 5, 1, 0.0009547919099414246 * GMS, 0.00047789450254521576, JUPITER_J2, 0.0},
+    {-1, 0, 6, 0, 0.00028588567002459455 * GMS, 0.0004028666966848747, 0.0, 0.0},
+    {-1, 0, 7, 0, 4.36624961322212e-05 * GMS, 0.0001708513622580592, 0.0, 0.0},
+    {-1, 0, 8, 0, 5.15138377265457e-05 * GMS, 0.0001655371154958558, 0.0, 0.0},
+};
+
+template <int PAD>
+__device__ __forceinline__ double group_maxd(double v) {
+#pragma unroll
+  for (int o = PAD / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int PAD>
+__device__ __forceinline__ int group_maxi(int v) {
+#pragma unroll
+  for (int o = PAD / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int PAD>
+__global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
+  __shared__ LdsTables lds;
+  for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
+    const int r = k / 8, cc = k % 8;
+    lds.rdiff[r][cc] = RT.rdiff[r][cc];
+    if (r == 0) {
+      lds.h[cc] = RT.h[cc];
+      lds.inv_h[cc] = RT.inv_h[cc];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int li = lane % PAD;            // index of the body in its batch
+  const int base = lane - li;           // first lane of the group
+  const int64_t gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PAD;
+  const int64_t n_batches = (a.n + a.batch - 1) / a.batch;
+  const int64_t p0 = gid * a.batch;     // first particle of the batch
+  const int m = gid < n_batches ? (int)min((int64_t)a.batch, a.n - p0) : 0;
+  // this lane carries a body of a batch (groups past the last batch, m == 0,
+  // carry none: their planet lanes must not load or store)
+  const bool body = m > 0 && li < 9 + m;
+  const int64_t pi = p0 + (li - 9);     // particle index (li >= 9)
+  Vec3 x{0, 0, 0}, v{0, 0, 0}, a0{0, 0, 0};
+  if (body && li < 9) {
+    const double* ps = a.planets + 6 * li;
+    x = Vec3{ps[0], ps[1], ps[2]};
+    v = Vec3{ps[3], ps[4], ps[5]};
+  } else if (body) {
+    x = Vec3{a.xyz[0][pi], a.xyz[1][pi], a.xyz[2][pi]};
+    v = Vec3{a.xyz[3][pi], a.xyz[4][pi], a.xyz[5][pi]};
+    if (a.frame == 17) {  // State::try_change_frame_mut to Equatorial
+      x = ecl_to_equ(x);
+      v = ecl_to_equ(v);
+    }
+  }
+  // kete's vec_accel applies a particle's non-grav model for idx > n_massive
+  const NonGravDev* ngp = nullptr;
+  if (body && li > 9 && a.ng) {
+    const NonGravDev* g = reinterpret_cast<const NonGravDev*>(a.ng) + pi;
+    if (g->kind != 0) ngp = g;
+  }
+  double B[7][3], G[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
+  double t = a.jd0, h = 0.0, h_next = 0.0;
+  const double tf = a.jd_final;
+  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
+  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
+  bool live = m > 0;  // the group's batch is still integrating
+  int32_t status = ST_OK;
+  while (__any(live)) {
+    const bool sub = phase == PH_SUB;
+    const double s = lds.h[j];
+    double xe[3], ve[3];
+    {
+      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double sp = 1.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        sp *= s;
+        const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bw[d] += B[k][d] * wk;
+          bu[d] += B[k][d] * uk;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
+        const double pv = vs[d] + s * h * (as[d] + bu[d]);
+        xe[d] = sub ? px : xs[d];
+        ve[d] = sub ? pv : vs[d];
+      }
+    }
+    // ---- vec_accel at the evaluation point ----
+    const bool exact = !sub;
+    Vec3 acc{0.0, 0.0, 0.0};
+    int32_t st = ST_OK;
+    n_force += live ? 1 : 0;
+#pragma unroll 1
+    for (int b = 0; b < 9; ++b) {
+      const Vec3 bp{__shfl(xe[0], base + b, 64), __shfl(xe[1], base + b, 64),
+                    __shfl(xe[2], base + b, 64)};
+      const Vec3 bv{__shfl(ve[0], base + b, 64), __shfl(ve[1], base + b, 64),
+                    __shfl(ve[2], base + b, 64)};
+      if (b == li || !body) continue;
+      Vec3 w{0.0, 0.0, 0.0};
+      const int32_t sb = add_body_at(kSimplePlanets[b], bp, bv, Vec3{xe[0], xe[1], xe[2]},
+                                     Vec3{ve[0], ve[1], ve[2]}, exact, ngp, w);
+      if (sb && st == ST_OK) st = sb;
+      acc = Vec3{acc.x + w.x, acc.y + w.y, acc.z + w.z};
+    }
+    const int32_t gst = group_maxi<PAD>(live ? st : 0);
+    const double ev[3] = {acc.x, acc.y, acc.z};
+    // ---- G column j (radau.rs:292-308) ----
+    {
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double gk[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = lds.rdiff[j][z];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const bool put = sub && q == j - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
+      }
+    }
+    // ---- end of sweep: B = G*C, the group's convergence test ----
+    const bool es = sub && j == 7 && gst == ST_OK;
+    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double acc2 = G[c][d];
+#pragma unroll
+        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
+        B[c][d] = es ? acc2 : B[c][d];
+      }
+    }
+    double rmax = 0.0, fmx = 0.0, bmx = 0.0;
+    if (body) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double fm = fabs(ev[d]) + 1e-6;
+        rmax = fmax(rmax, fabs(B[6][d] - bp6[d]) / fm);
+        fmx = fmax(fmx, fm);
+        bmx = fmax(bmx, fabs(B[6][d]));
+      }
+    }
+    rmax = group_maxd<PAD>(rmax);
+    fmx = group_maxd<PAD>(fmx);
+    bmx = group_maxd<PAD>(bmx);
+    const bool conv = es && rmax < 1e-14;
+    // ---- accepted step (radau.rs:320-338) ----
+    {
+      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double bw = 0.0, bu = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          bw += B[k][d] * RTC.w[k];
+          bu += B[k][d] * RTC.u[k];
+        }
+        const double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
+        const double nv = vs[d] + h * (as[d] + bu);
+        xs[d] = conv ? nx : xs[d];
+        vs[d] = conv ? nv : vs[d];
+      }
+      x = Vec3{xs[0], xs[1], xs[2]};
+      v = Vec3{vs[0], vs[1], vs[2]};
+    }
+    if (conv) {
+      double ratio = root7(1e-6 * fmx / bmx);
+      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
+      h_next = h * ratio;
+    }
+    t = conv ? t + h : t;
+    const bool take_a = gst == ST_OK && !sub;
+    a0 = Vec3{take_a ? acc.x : a0.x, take_a ? acc.y : a0.y, take_a ? acc.z : a0.z};
+    // ---- control (radau.rs:160-218), the same for the whole group ----
+    int32_t done = -1;
+    bool begin = false;
+    if (gst != ST_OK) {
+      done = gst;
+    } else if (phase == PH_INIT) {
+      if (fabs(tf - t) < 1e-10) {
+        done = ST_OK;
+      } else {
+        h = copysign(0.1, tf - t);
+        begin = true;
+      }
+    } else if (phase == PH_EXACT) {
+      h = h_next;
+      if (fabs(t - tf) < 1e-12) {
+        done = ST_OK;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    } else if (j < 7) {
+      j = j + 1;
+    } else if (conv) {
+      n_acc += live ? 1 : 0;
+      phase = PH_EXACT;
+    } else if (sweep + 1 < 10) {
+      sweep++;
+      j = 1;
+      n_sweep += live ? 1 : 0;
+    } else {
+      n_fail += live ? 1 : 0;
+      fails++;
+      h *= 0.7;
+      if (fails > 10) {
+        done = ST_CONVERGENCE;
+      } else {
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    }
+    if (begin) {
+      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      sweep = 0;
+      j = 1;
+      n_sweep += live ? 1 : 0;
+      phase = PH_SUB;
+    }
+    if (live && done >= 0) {
+      live = false;
+      status = done;
+    }
+  }
+  // states relative to the Sun's final state (propagation/mod.rs:224-233)
+  const Vec3 sx{__shfl(x.x, base, 64), __shfl(x.y, base, 64), __shfl(x.z, base, 64)};
+  const Vec3 sv{__shfl(v.x, base, 64), __shfl(v.y, base, 64), __shfl(v.z, base, 64)};
+  if (!body) return;
+  const bool ok = status == ST_OK;
+  const double nan = __builtin_nan("");
+  const Vec3 rx = ok ? Vec3{x.x - sx.x, x.y - sx.y, x.z - sx.z} : Vec3{nan, nan, nan};
+  const Vec3 rv = ok ? Vec3{v.x - sv.x, v.y - sv.y, v.z - sv.z} : Vec3{nan, nan, nan};
+  if (li < 9) {
+    double* po = a.planets_out + (gid * 9 + li) * 6;
+    po[0] = rx.x; po[1] = rx.y; po[2] = rx.z;
+    po[3] = rv.x; po[4] = rv.y; po[5] = rv.z;
+  } else {
+    a.out[0][pi] = rx.x;
+    a.out[1][pi] = rx.y;
+    a.out[2][pi] = rx.z;
+    a.out[3][pi] = rv.x;
+    a.out[4][pi] = rv.y;
+    a.out[5][pi] = rv.z;
+  }
+  if (li == 0) {
+    a.status[gid] = status;
+    if (a.counters) {
+      int32_t* c = a.counters + 4 * gid;
+      c[0] = n_force; c[1] = n_sweep; c[2] = n_acc; c[3] = n_fail;
+    }
+  }
+}
+
+int32_t launch_nbody_long(const LongLaunch& p, hipStream_t stream) {
+  if (p.n <= 0 || p.batch <= 0 || p.batch > 55) return -1;
+  const int64_t nb = (p.n + p.batch - 1) / p.batch;
+  const int pad = p.batch + 9 <= 32 ? 32 : 64;
+  const int64_t lanes = nb * pad;
+  const unsigned grid = (unsigned)((lanes + 255) / 256);
+  if (pad == 32)
+    hipLaunchKernelGGL(radau_long_kernel<32>, dim3(grid), dim3(256), 0, stream, p);
+  else
+    hipLaunchKernelGGL(radau_long_kernel<64>, dim3(grid), dim3(256), 0, stream, p);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int32_t upload_radau_tables(const RadauTables& t) {
   return hipMemcpyToSymbol(HIP_SYMBOL(RT), &t, sizeof(RadauTables)) == hipSuccess ? 0 : -1;
 }

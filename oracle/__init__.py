@@ -48,6 +48,7 @@ def _ptrs(cols):
     arr = (C.POINTER(C.c_double) * len(cols))()
     for i, c in enumerate(cols):
         arr[i] = _d(c)
+    arr._keep = list(cols)  # the pointers do not hold their arrays
     return arr
 
 
