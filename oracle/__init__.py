@@ -98,6 +98,11 @@ def lib():
         L.ko_fov_spk_check.restype = I64
         L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
                                        C.POINTER(I64), C.POINTER(I64), P]
+        L.ko_comet_elements_peri.restype = None
+        L.ko_comet_elements_peri.argtypes = [D, P, P, P, P]
+        L.ko_moid_batch.restype = None
+        L.ko_moid_batch.argtypes = [I64, P, PP, I32, P, PP, P, P, C.POINTER(I32),
+                                    C.POINTER(I32), C.c_int]
         L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
         L.ko_hg_apparent_mag.restype = D
         L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
@@ -336,6 +341,32 @@ def fov_spk_check(ids, field_jd, obs_xyz, normals, frame=FRAME_ECLIPTIC, cap=Non
     if nh < 0:
         raise RuntimeError("hit capacity exceeded")
     return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()
+
+
+def comet_elements_peri(epoch, pos, vel):
+    """-> (peri_time, orbital_period) of CometElements::from_state."""
+    pt, per = C.c_double(), C.c_double()
+    p, v = _v3(pos), _v3(vel)
+    lib().ko_comet_elements_peri(epoch, _d(p), _d(v), C.byref(pt), C.byref(per))
+    return pt.value, per.value
+
+
+def moid_batch(jd_a, xyz_a, frame=FRAME_ECLIPTIC, jd_b=None, xyz_b=None, n_threads=None):
+    """kete moid per object -> (moid[n], status[n], grid_min[n], iters[n]); with
+    xyz_b None the second orbit is the Earth's at jd_a (moid_py's default)."""
+    xyz_a = _soa(xyz_a)
+    n = xyz_a.shape[1]
+    jd_a = np.ascontiguousarray(np.broadcast_to(np.asarray(jd_a, dtype=np.float64), (n,)))
+    if jd_b is not None:
+        jd_b = np.ascontiguousarray(np.broadcast_to(np.asarray(jd_b, dtype=np.float64), (n,)))
+    xb = _ptrs(list(_soa(xyz_b))) if xyz_b is not None else None
+    out, gmin = np.empty(n), np.empty(n)
+    st, it = np.empty(n, dtype=np.int32), np.empty(n, dtype=np.int32)
+    lib().ko_moid_batch(n, _d(jd_a), _ptrs(list(xyz_a)), frame,
+                        _d(jd_b) if jd_b is not None else None, xb, _d(out), _d(gmin),
+                        st.ctypes.data_as(C.POINTER(C.c_int32)),
+                        it.ctypes.data_as(C.POINTER(C.c_int32)), n_threads or os.cpu_count())
+    return out, st, gmin, it
 
 
 def ra_dec_with_rates(pos, vel, obs_pos, obs_vel):

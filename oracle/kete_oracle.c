@@ -1930,3 +1930,232 @@ double ko_hg_apparent_mag(double h, double g, const double sun2obj[3],
     double corr = log10(ko_hg_phase_curve_correction(g, phase));
     return h + 5.0 * log10(obj_r * o2o_r) - 2.5 * corr;
 }
+
+/* ------------------------------------------------------------------------ */
+/* MOID (propagation/kepler.rs:325-404; moid_py src/kete/rust/propagation.rs:27-39) */
+/* ------------------------------------------------------------------------ */
+/* nalgebra Vector::angle: acos of the cosine clamped to [-1, 1], 0 when
+ * either vector is zero. */
+static double vec_angle(const double *a, const double *b) {
+    double prod = dot3(a, b), n1 = norm3(a), n2 = norm3(b);
+    if (n1 == 0.0 || n2 == 0.0) return 0.0;
+    double c = prod / (n1 * n2);
+    return c > 1.0 ? 0.0 : (c < -1.0 ? PI_ : acos(c));
+}
+
+/* CometElements::from_pos_vel (elements.rs:62-163), the two quantities moid
+ * uses: the perihelion time and orbital_period (elements.rs:278-293). */
+static void moid_elements(double epoch, const double *pos, const double *vel,
+                          double *peri_time, double *period) {
+    double vs[3] = {vel[0] / GMS_SQRT, vel[1] / GMS_SQRT, vel[2] / GMS_SQRT};
+    double v_mag2 = dot3(vs, vs);
+    double p_mag = norm3(pos);
+    double vp_mag = dot3(pos, vs);
+    double ecc_vec[3], ang[3];
+    for (int k = 0; k < 3; ++k) ecc_vec[k] = (v_mag2 - 1.0 / p_mag) * pos[k] - vp_mag * vs[k];
+    cross3(pos, vs, ang);
+    double lon[3] = {-ang[1], ang[0], 0.0};
+    double ecc = norm3(ecc_vec), ang_mag = norm3(ang), lon_mag = norm3(lon);
+    double peri_dist = (ang_mag * ang_mag) / (1.0 + ecc);
+    if (lon_mag < 1e-8) { lon[0] = 1.0; lon[1] = 0.0; lon[2] = 0.0; }
+    double pt;
+    if (fabs(ecc - 1.0) < PARABOLIC_ECC_LIMIT) {
+        double ta = vec_angle(ecc_vec, pos);
+        if (signbit(vp_mag)) ta = -ta;
+        double d = tan(ta / 2.0);
+        double dt = (sqrt(2.0) * pow(peri_dist, 1.5) / GMS_SQRT) * (d + (d * d * d) / 3.0);
+        pt = epoch - dt;
+    } else if (ecc < 1e-6) {
+        double a = 1.0 / (2.0 / p_mag - v_mag2);
+        double mm = pow(fabs(a), -1.5) * GMS_SQRT;
+        double ta = vec_angle(lon, pos);
+        double c[3];
+        cross3(ang, lon, c);
+        if (signbit(dot3(c, pos))) ta = -ta;
+        pt = epoch - ta / mm;
+    } else {
+        double a = 1.0 / (2.0 / p_mag - v_mag2);
+        double mm = pow(fabs(a), -1.5) * GMS_SQRT;
+        double x_bar = (ang_mag * ang_mag - p_mag) / ecc;
+        double y_bar = vp_mag / ecc * ang_mag;
+        double b = a * sqrt(fabs(1.0 - ecc * ecc));
+        double s_e = y_bar / b;
+        double m;
+        if (ecc < 1.0) {
+            double c_e = x_bar / a + ecc;
+            m = atan2(s_e, c_e) - ecc * s_e;
+        } else {
+            m = -ecc * s_e - asinh(-s_e);
+        }
+        pt = epoch - m / mm;
+    }
+    *peri_time = pt;
+    /* semi_major: NaN for parabolic; period infinite for a <= 1e-8 */
+    double sm = fabs(ecc - 1.0) <= PARABOLIC_ECC_LIMIT ? NAN : peri_dist / (1.0 - ecc);
+    *period = sm <= 1e-8 ? INFINITY : TAU * pow(sm, 1.5) / GMS_SQRT;
+}
+
+void ko_comet_elements_peri(double epoch, const double pos[3], const double vel[3],
+                            double *peri_time, double *period) {
+    moid_elements(epoch, pos, vel, peri_time, period);
+}
+
+/* MoidCost::cost (kepler.rs:333-344): distance between the two states moved
+ * by param = [dt_a, dt_b] days. */
+static int moid_cost(const ko_state *a, const ko_state *b, const double prm[2], double *out) {
+    ko_state sa = *a, sb = *b;
+    int rc = ko_propagate_two_body(&sa, a->jd + prm[0]);
+    if (!rc) rc = ko_propagate_two_body(&sb, b->jd + prm[1]);
+    if (rc) return rc;
+    double d[3] = {sa.pos[0] - sb.pos[0], sa.pos[1] - sb.pos[1], sa.pos[2] - sb.pos[2]};
+    *out = norm3(d);
+    return KO_OK;
+}
+
+/* Nelder-Mead of the argmin crate (argmin ^0.10.0, kete_core/Cargo.toml:13;
+ * the crate is not in the kete source tree): alpha 1, gamma 2, rho 0.5,
+ * sigma 0.5; vertices kept sorted by cost (stable); per iteration the
+ * reflection of the worst vertex through the centroid of the others,
+ * accepted when it lies in [best, second-worst); an expansion when it beats
+ * the best (the better of the two kept); otherwise an inside contraction
+ * toward the worst, kept only if it beats the worst (argmin reaches its
+ * shrink step only for NaN costs, which here are an error).  Before every
+ * iteration the executor stops when the sample standard deviation of the
+ * vertex costs is below f64::EPSILON, then when 1000 iterations are done.
+ * The result is the best vertex's cost. */
+typedef struct { double p[2]; double c; } nm_vertex;
+
+static void nm_sort(nm_vertex v[3]) {
+    for (int i = 1; i < 3; ++i) {
+        nm_vertex x = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j].c > x.c) { v[j + 1] = v[j]; --j; }
+        v[j + 1] = x;
+    }
+}
+
+static int moid_nelder_mead(const ko_state *a, const ko_state *b, double *best, int32_t *iters) {
+    nm_vertex v[3] = {{{-15.0, -15.0}, 0.0}, {{15.0, -15.0}, 0.0}, {{0.0, 15.0}, 0.0}};
+    for (int k = 0; k < 3; ++k) {
+        int rc = moid_cost(a, b, v[k].p, &v[k].c);
+        if (rc) return rc;
+        if (isnan(v[k].c)) return KO_CONVERGENCE;
+    }
+    nm_sort(v);
+    int it = 0;
+    for (;;) {
+        double c0 = ((0.0 + v[0].c) + v[1].c) + v[2].c;
+        c0 = c0 / 3.0;
+        double ss = 0.0;
+        for (int k = 0; k < 3; ++k) ss += (v[k].c - c0) * (v[k].c - c0);
+        double s = sqrt(1.0 / (3.0 - 1.0) * ss);
+        if (s < 2.220446049250313e-16) break;
+        if (it >= 1000) break;
+        double x0[2], xr[2], cr;
+        for (int d = 0; d < 2; ++d) x0[d] = (v[0].p[d] + v[1].p[d]) * (1.0 / 2.0);
+        for (int d = 0; d < 2; ++d) xr[d] = x0[d] + (x0[d] - v[2].p[d]) * 1.0;
+        int rc = moid_cost(a, b, xr, &cr);
+        if (rc) return rc;
+        if (isnan(cr)) return KO_CONVERGENCE;
+        if (cr < v[1].c && cr >= v[0].c) {
+            v[2].p[0] = xr[0]; v[2].p[1] = xr[1]; v[2].c = cr;
+        } else if (cr < v[0].c) {
+            double xe[2], ce;
+            for (int d = 0; d < 2; ++d) xe[d] = x0[d] + (xr[d] - x0[d]) * 2.0;
+            rc = moid_cost(a, b, xe, &ce);
+            if (rc) return rc;
+            if (isnan(ce)) return KO_CONVERGENCE;
+            if (ce < cr) { v[2].p[0] = xe[0]; v[2].p[1] = xe[1]; v[2].c = ce; }
+            else { v[2].p[0] = xr[0]; v[2].p[1] = xr[1]; v[2].c = cr; }
+        } else {
+            double xc[2], cc;
+            for (int d = 0; d < 2; ++d) xc[d] = x0[d] + (v[2].p[d] - x0[d]) * 0.5;
+            rc = moid_cost(a, b, xc, &cc);
+            if (rc) return rc;
+            if (isnan(cc)) return KO_CONVERGENCE;
+            if (cc < v[2].c) { v[2].p[0] = xc[0]; v[2].p[1] = xc[1]; v[2].c = cc; }
+        }
+        nm_sort(v);
+        ++it;
+    }
+    *best = v[0].c;
+    if (iters) *iters = it;
+    return KO_OK;
+}
+
+#define MOID_STEPS 50
+
+int ko_moid(const ko_state *a_in, const ko_state *b_in, double *moid, double *grid_min,
+            int32_t *iters) {
+    ko_state a = *a_in, b = *b_in;
+    int rc = ko_change_frame(&a, KO_FRAME_ECLIPTIC);
+    if (!rc) rc = ko_change_frame(&b, KO_FRAME_ECLIPTIC);
+    if (rc) return rc;
+    double pt_a, per_a, pt_b, per_b;
+    moid_elements(a.jd, a.pos, a.vel, &pt_a, &per_a);
+    rc = ko_propagate_two_body(&a, pt_a);
+    if (rc) return rc;
+    moid_elements(b.jd, b.pos, b.vel, &pt_b, &per_b);
+    rc = ko_propagate_two_body(&b, pt_b);
+    if (rc) return rc;
+    double step_a = isfinite(per_a) ? per_a / MOID_STEPS : 300.0 / MOID_STEPS;
+    double step_b = isfinite(per_b) ? per_b / MOID_STEPS : 300.0 / MOID_STEPS;
+    static __thread ko_state ga[2 * MOID_STEPS], gb[2 * MOID_STEPS];
+    for (int idx = -MOID_STEPS; idx < MOID_STEPS; ++idx) {
+        ga[idx + MOID_STEPS] = a;
+        rc = ko_propagate_two_body(&ga[idx + MOID_STEPS], a.jd + idx * step_a);
+        if (rc) return rc;
+        gb[idx + MOID_STEPS] = b;
+        rc = ko_propagate_two_body(&gb[idx + MOID_STEPS], b.jd + idx * step_b);
+        if (rc) return rc;
+    }
+    double best = INFINITY;
+    ko_state ba = a, bb = b;
+    for (int i = 0; i < 2 * MOID_STEPS; ++i)
+        for (int j = 0; j < 2 * MOID_STEPS; ++j) {
+            double d[3] = {ga[i].pos[0] - gb[j].pos[0], ga[i].pos[1] - gb[j].pos[1],
+                           ga[i].pos[2] - gb[j].pos[2]};
+            double r = norm3(d);
+            if (r < best) { best = r; ba = ga[i]; bb = gb[j]; }
+        }
+    if (grid_min) *grid_min = best;
+    return moid_nelder_mead(&ba, &bb, moid, iters);
+}
+
+typedef struct {
+    const double *jd_a, *jd_b;
+    const double *const *xa, *const *xb;
+    int32_t frame;
+    double *out, *grid_min;
+    int32_t *status, *iters;
+} moid_ctx;
+
+static void moid_range(int64_t lo, int64_t hi, void *vctx) {
+    moid_ctx *c = vctx;
+    for (int64_t i = lo; i < hi; ++i) {
+        ko_state a = {c->jd_a[i], {c->xa[0][i], c->xa[1][i], c->xa[2][i]},
+                      {c->xa[3][i], c->xa[4][i], c->xa[5][i]}, c->frame, 10};
+        ko_state b = {c->jd_b ? c->jd_b[i] : c->jd_a[i], {0, 0, 0}, {0, 0, 0}, c->frame, 10};
+        int rc = KO_OK;
+        if (c->xb) {
+            for (int k = 0; k < 3; ++k) { b.pos[k] = c->xb[k][i]; b.vel[k] = c->xb[3 + k][i]; }
+        } else {
+            /* moid_py's default: the Earth at state_a's epoch, Sun-centred */
+            rc = ko_spk_get_state(399, a.jd, 10, c->frame, b.pos, b.vel);
+        }
+        double m = NAN, g = NAN;
+        int32_t it = 0;
+        if (!rc) rc = ko_moid(&a, &b, &m, &g, &it);
+        c->status[i] = rc;
+        c->out[i] = rc ? NAN : m;
+        if (c->grid_min) c->grid_min[i] = g;
+        if (c->iters) c->iters[i] = it;
+    }
+}
+
+void ko_moid_batch(int64_t n, const double *jd_a, const double *const *xyz_a, int32_t frame,
+                   const double *jd_b, const double *const *xyz_b, double *out,
+                   double *grid_min, int32_t *status, int32_t *iters, int n_threads) {
+    moid_ctx c = {jd_a, jd_b, xyz_a, xyz_b, frame, out, grid_min, status, iters};
+    parallel_for(n, 16, n_threads, moid_range, &c);
+}

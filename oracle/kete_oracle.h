@@ -204,6 +204,23 @@ int64_t ko_fov_spk_check(int64_t n_ids, const int64_t *ids, int32_t frame, int64
                          const double *field_jd, const double *obs_xyz, const double *normals,
                          int64_t cap, int64_t *hit_field, int64_t *hit_obj, double *hit_state);
 
+/* ---- MOID (propagation/kepler.rs:325-404) --------------------------------- */
+/* kete's moid of two states: both to ecliptic, each moved to its perihelion,
+ * a 100 x 100 grid of two-body positions (steps of period/50, or 6 days for
+ * open orbits), then Nelder-Mead (argmin 0.10) from the closest grid pair.
+ * grid_min: the grid's smallest distance; iters: Nelder-Mead iterations. */
+/* CometElements::from_state's perihelion time and orbital_period (elements.rs:50-163,287-293). */
+void ko_comet_elements_peri(double epoch, const double pos[3], const double vel[3],
+                            double *peri_time, double *period);
+int ko_moid(const ko_state *a, const ko_state *b, double *moid, double *grid_min,
+            int32_t *iters);
+/* A batch: a[i] = (jd_a[i], xyz_a[.][i]) and b[i] = (jd_b[i] or jd_a[i],
+ * xyz_b[.][i]) or, with xyz_b NULL, the Earth at jd_a[i] from the loaded
+ * SPK (moid_py, src/kete/rust/propagation.rs:27-39).  Sun-centred, frame. */
+void ko_moid_batch(int64_t n, const double *jd_a, const double *const *xyz_a, int32_t frame,
+                   const double *jd_b, const double *const *xyz_b, double *out,
+                   double *grid_min, int32_t *status, int32_t *iters, int n_threads);
+
 /* ---- post-detection epilogue --------------------------------------------- */
 /* ra_dec_with_rates for one state (simult_states.rs:86-120), equatorial. */
 void ko_ra_dec_with_rates(const double pos[3], const double vel[3],
