@@ -300,6 +300,27 @@ int32_t kh_propagate_n_body_long(kh_ctx *ctx, int64_t n, double jd0, const doubl
                                  int32_t batch_size, double *const out[6], double *planets_out,
                                  int32_t *status);
 
+/* MOID (minimum orbit intersection distance, AU) of n orbit pairs.  Replaces
+ * moid (src/kete_core/src/propagation/kepler.rs:325-404) called per pair, and
+ * moid_py (src/kete/rust/propagation.rs:27-39) for its default second state:
+ * pair i is state a (jd_a[i], xyz_a[.][i]) and state b (jd_b[i], or jd_a[i]
+ * when jd_b is null; xyz_b[.][i]), both Sun-centred in `frame`; with xyz_b
+ * null, b is the Earth (399) at jd_a[i] from the uploaded ephemeris.
+ * moid[i]: the result (NaN on error); status[i]: 0, or 1 Convergence when a
+ * two-body propagation fails (kete's moid returns that error; a failure
+ * inside the Nelder-Mead search, where kete panics, is reported the same
+ * way), 3 DAFLimits when the Earth is not covered.  Host pointers. */
+int32_t kh_moid(kh_ctx *ctx, int64_t n, const double *jd_a, const double *const xyz_a[6],
+                int32_t frame, const double *jd_b, const double *const xyz_b[6], double *moid,
+                int32_t *status);
+/* kh_moid on device pointers; grid_min[n] (the closest of the 100 x 100
+ * grid positions) and iters[n] (Nelder-Mead iterations) are optional.
+ * sync = 0 returns with the work queued on the library's stream. */
+int32_t kh_moid_dev(kh_ctx *ctx, int64_t n, const double *jd_a, const double *const xyz_a[6],
+                    int32_t frame, const double *jd_b, const double *const xyz_b[6],
+                    double *moid, int32_t *status, double *grid_min, int32_t *iters,
+                    int32_t sync);
+
 /* Launch shape picked by kh_autotune_n_body. */
 typedef struct {
     int32_t mode;       /* 0-3 as kh_propagate_n_body's mode */

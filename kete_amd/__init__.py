@@ -21,7 +21,8 @@ from .spk import FRAME_ECLIPTIC, FRAME_EQUATORIAL, Segment, Type2Table, read_spk
 
 __all__ = [
     "State", "StateArray", "NonGravModel", "Context", "propagate_n_body", "propagate_two_body",
-    "spk_state", "load_spk", "get_context", "Frames", "KeteHipError",
+    "spk_state", "load_spk", "get_context", "Frames", "KeteHipError", "propagate_n_body_long",
+    "moid",
 ]
 
 KeteHipError = _lib.KeteHipError
@@ -412,6 +413,60 @@ def propagate_n_body_long(states, jd_final, planet_states=None, non_gravs=None, 
     pls = StateArray(np.full(9, float(jd_final)), np.ascontiguousarray(planets.T),
                      FRAME_EQUATORIAL, 10)
     return (res, pls) if soa else (res.to_states(), pls.to_states())
+
+
+_OBLIQUITY = 0.40909280422232897
+
+
+def _rotate_frame(xyz, frame_from, frame_to):
+    """Rotate [6, n] states between the equatorial (1) and ecliptic (17) frames."""
+    if frame_from == frame_to:
+        return xyz
+    c, sn = np.cos(_OBLIQUITY), np.sin(_OBLIQUITY)
+    sn = sn if frame_from == FRAME_ECLIPTIC else -sn  # ecliptic -> equatorial, or back
+    out = np.empty_like(xyz)
+    for o in (0, 3):
+        out[o] = xyz[o]
+        out[o + 1] = c * xyz[o + 1] - sn * xyz[o + 2]
+        out[o + 2] = sn * xyz[o + 1] + c * xyz[o + 2]
+    return out
+
+
+def moid(state_a, state_b=None):
+    """kete ``moid`` (``src/kete/rust/propagation.rs:27-39``,
+    ``propagation/kepler.rs:325-404``): the minimum orbit intersection
+    distance (AU) between the orbit of ``state_a`` and that of ``state_b``,
+    by default the Earth's at ``state_a``'s epoch.  ``state_a`` may be one
+    State (a float is returned, errors raise ValueError as kete's do) or a
+    batch (State list or StateArray: an array with NaN where a pair fails);
+    ``state_b`` one State for every pair, or a batch of the same length."""
+    batch, soa = _as_batch(state_a)
+    single = isinstance(state_a, State)
+    n = len(batch)
+    ctx = get_context()
+    jd_b = xb = None
+    if state_b is not None:
+        bb, _ = _as_batch(state_b)
+        if len(bb) == 1 and n != 1:
+            bb = StateArray(np.full(n, bb.jd[0]), np.repeat(bb.xyz, n, axis=1), bb.frame,
+                            bb.center_id)
+        if len(bb) != n:
+            raise ValueError("state_b must be one state or one per state_a")
+        jd_b = np.ascontiguousarray(bb.jd)
+        xb = np.ascontiguousarray(_rotate_frame(bb.xyz, bb.frame, batch.frame))
+    out = np.empty(n)
+    st = np.empty(n, dtype=np.int32)
+    if n:
+        _lib.check(ctx.L.kh_moid(
+            ctx.h, n, _lib.dptr(batch.jd), _lib.ptr_array(list(batch.xyz)), batch.frame,
+            _lib.dptr(jd_b) if jd_b is not None else None,
+            _lib.ptr_array(list(xb)) if xb is not None else None, _lib.dptr(out),
+            _lib.iptr(st, C.c_int32)))
+    if single:
+        if st[0] != 0:
+            raise ValueError(f"moid: {STATUS_NAMES.get(int(st[0]), int(st[0]))}")
+        return float(out[0])
+    return out
 
 
 def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppress_errors=True,

@@ -119,6 +119,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->obuf) (void)hipFree(c->obuf);
   if (c->fbuf) (void)hipFree(c->fbuf);
   if (c->hbuf) (void)hipFree(c->hbuf);
+  if (c->mbuf) (void)hipFree(c->mbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);
@@ -948,6 +949,109 @@ int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double*
       *status = st[b];
       break;
     }
+  return KH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MOID (propagation/kepler.rs:325-404; moid_py src/kete/rust/propagation.rs:27-39)
+// ---------------------------------------------------------------------------
+int32_t kh_moid_dev(kh_ctx* c, int64_t n, const double* jd_a, const double* const xyz_a[6],
+                    int32_t frame, const double* jd_b, const double* const xyz_b[6],
+                    double* moid, int32_t* status, double* grid_min, int32_t* iters,
+                    int32_t sync) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (n < 0) return fail(KH_E_ARG, "negative size");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  if (n == 0) return KH_OK;
+  if (!jd_a || !xyz_a || !moid || !status) return fail(KH_E_ARG, "null array");
+  for (int k = 0; k < 6; ++k)
+    if (!xyz_a[k] || (xyz_b && !xyz_b[k])) return fail(KH_E_ARG, "null state column");
+  Chain earth{}, cen{};
+  if (!xyz_b) {
+    // the Earth at each state_a epoch, Sun-centred in the states' frame
+    if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+    if (!kh_build_chain(c, 399, earth) || !kh_build_chain(c, 10, cen))
+      return fail(KH_E_NO_SPK, "no path from the Earth or the Sun to 0");
+  }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bd = align256(n * 8), bi = align256(n * 4);
+  int32_t r = kh_grow(&c->mbuf, &c->mbuf_size, 20 * bd + bi);
+  if (r) return r;
+  char* q = (char*)c->mbuf;
+  MoidLaunch p{};
+  p.n = n;
+  p.frame = frame;
+  p.max_iter = 1000;
+  p.jd_a = jd_a;
+  p.jd_b = jd_b;
+  for (int k = 0; k < 7; ++k, q += bd) p.best_a[k] = (double*)q;
+  for (int k = 0; k < 7; ++k, q += bd) p.best_b[k] = (double*)q;
+  for (int k = 0; k < 6; ++k) p.xa[k] = xyz_a[k];
+  if (xyz_b) {
+    for (int k = 0; k < 6; ++k) p.xb[k] = xyz_b[k];
+  } else {
+    double* eo[6];
+    for (int k = 0; k < 6; ++k, q += bd) {
+      eo[k] = (double*)q;
+      p.xb[k] = eo[k];
+    }
+    int32_t* est = (int32_t*)q;
+    p.st_b = est;
+    Chain none{};
+    if (launch_spk_states(c->d_spk, c->d_coef, earth, cen, 0, 0, none, 10, frame, n, jd_a, eo, est,
+                          c->stream))
+      return fail(KH_E_HIP, "spk_states launch failed");
+  }
+  p.moid = moid;
+  p.grid_min = grid_min;
+  p.status = status;
+  p.iters = iters;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (launch_moid(p, c->stream)) return fail(KH_E_HIP, "moid launch failed");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  if (sync) HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
+int32_t kh_moid(kh_ctx* c, int64_t n, const double* jd_a, const double* const xyz_a[6],
+                int32_t frame, const double* jd_b, const double* const xyz_b[6], double* moid,
+                int32_t* status) {
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (n <= 0) return n == 0 ? KH_OK : fail(KH_E_ARG, "negative size");
+  if (!jd_a || !xyz_a || !moid || !status) return fail(KH_E_ARG, "null array");
+  for (int k = 0; k < 6; ++k)
+    if (!xyz_a[k] || (xyz_b && !xyz_b[k])) return fail(KH_E_ARG, "null state column");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bd = align256(n * 8), bi = align256(n * 4);
+  char* base;
+  int32_t r = staging(c, 15 * bd + bi, &base);
+  if (r) return r;
+  char* q = base;
+  double* d_jda = (double*)q;
+  q += bd;
+  double* d_jdb = jd_b ? (double*)q : nullptr;
+  q += bd;
+  const double* d_xa[6];
+  const double* d_xb[6];
+  for (int k = 0; k < 6; ++k, q += bd) d_xa[k] = (const double*)q;
+  for (int k = 0; k < 6; ++k, q += bd) d_xb[k] = (const double*)q;
+  double* d_moid = (double*)q;
+  q += bd;
+  int32_t* d_st = (int32_t*)q;
+  HIPCHK(hipMemcpyAsync(d_jda, jd_a, n * 8, hipMemcpyHostToDevice, c->stream));
+  if (jd_b) HIPCHK(hipMemcpyAsync(d_jdb, jd_b, n * 8, hipMemcpyHostToDevice, c->stream));
+  for (int k = 0; k < 6; ++k) {
+    HIPCHK(hipMemcpyAsync((void*)d_xa[k], xyz_a[k], n * 8, hipMemcpyHostToDevice, c->stream));
+    if (xyz_b)
+      HIPCHK(hipMemcpyAsync((void*)d_xb[k], xyz_b[k], n * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  r = kh_moid_dev(c, n, d_jda, d_xa, frame, d_jdb, xyz_b ? d_xb : nullptr, d_moid, d_st, nullptr,
+                  nullptr, 0);
+  if (r) return r;
+  HIPCHK(hipMemcpyAsync(moid, d_moid, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(status, d_st, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return KH_OK;
 }
 

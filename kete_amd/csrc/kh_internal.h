@@ -98,6 +98,28 @@ struct LongLaunch {
 };
 int32_t launch_nbody_long(const LongLaunch& p, hipStream_t stream);
 
+// kete moid for n orbit pairs (kh_moid.hip): state a (jd_a, xa) and state b
+// (jd_b or jd_a, xb), Sun-centred in `frame`; st_b: the b states' lookup
+// status or null.  best_a/best_b [7][n] (jd, pos, vel) carry the grid's
+// closest pair from the first kernel to the Nelder-Mead one.
+struct MoidLaunch {
+  int64_t n;
+  int32_t frame;
+  int32_t max_iter;
+  const double* jd_a;
+  const double* xa[6];
+  const double* jd_b;
+  const double* xb[6];
+  const int32_t* st_b;
+  double* best_a[7];
+  double* best_b[7];
+  double* moid;
+  double* grid_min;  // or null
+  int32_t* status;
+  int32_t* iters;    // or null
+};
+int32_t launch_moid(const MoidLaunch& p, hipStream_t stream);
+
 int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p,
                      hipStream_t stream, int32_t* used_block, int32_t* used_grid,
                      int32_t* used_mode, hipEvent_t ev0, hipEvent_t ev1);
