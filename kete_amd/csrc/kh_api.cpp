@@ -975,7 +975,8 @@ int32_t kh_moid_dev(kh_ctx* c, int64_t n, const double* jd_a, const double* cons
   }
   HIPCHK(hipSetDevice(c->device));
   const size_t bd = align256(n * 8), bi = align256(n * 4);
-  int32_t r = kh_grow(&c->mbuf, &c->mbuf_size, 20 * bd + bi);
+  const size_t bsg = align256(sizeof(MoidSharedGrid));
+  int32_t r = kh_grow(&c->mbuf, &c->mbuf_size, 20 * bd + bi + bsg);
   if (r) return r;
   char* q = (char*)c->mbuf;
   MoidLaunch p{};
@@ -996,7 +997,9 @@ int32_t kh_moid_dev(kh_ctx* c, int64_t n, const double* jd_a, const double* cons
       p.xb[k] = eo[k];
     }
     int32_t* est = (int32_t*)q;
+    q += bi;
     p.st_b = est;
+    p.sg = (MoidSharedGrid*)q;
     Chain none{};
     if (launch_spk_states(c->d_spk, c->d_coef, earth, cen, 0, 0, none, 10, frame, n, jd_a, eo, est,
                           c->stream))

@@ -476,6 +476,17 @@ struct KepEll {
     double g3d = (1.0 - c) / beta;
     return r0 * c + rv0 * s + GMS * g3d;
   }
+  // f and d at one point, sharing sin(b x)
+  __device__ __forceinline__ void fd(double x, double& fv, double& dv) const {
+    double s, c;
+    sincos(b * x, &s, &c);
+    double g1 = s / b;
+    double sh = sin(b * x / 2.0);
+    double g2 = 2.0 * (sh * sh) / beta;
+    double g3 = (x - g1) / beta;
+    fv = r0 * g1 + rv0 * g2 + GMS * g3 - dt;
+    dv = r0 * c + rv0 * s + GMS * ((1.0 - c) / beta);
+  }
 };
 
 struct KepHyp {
@@ -491,20 +502,32 @@ struct KepHyp {
     double g3d = (1.0 - ch) / beta;
     return r0 * ch + rv0 * sh + GMS * g3d;
   }
+  // f and d at one point, sharing sinh(b x) and cosh(b x)
+  __device__ __forceinline__ void fd(double x, double& fv, double& dv) const {
+    double sh = sinh(b * x), ch = cosh(b * x);
+    double g1 = sh / b;
+    double g3 = (x - g1) / beta;
+    fv = r0 * g1 + rv0 * ((1.0 - ch) / beta) + GMS * g3 - dt;
+    dv = r0 * ch + rv0 * sh + GMS * ((1.0 - ch) / beta);
+  }
 };
 
 // newton_raphson (fitting/newton.rs:18-65); returns iterations used or -1.
+// The function and its derivative at the top of an iteration come from one
+// evaluation (fd) that shares the transcendentals; kete evaluates the
+// derivative after the convergence test, so an iteration that converges
+// computes it without using it.
 template <class F>
 __device__ __forceinline__ int newton(const F& fn, double start, double atol, double& out) {
   double x = start;
   if (fabs(fn.d(x)) < 1e-12) x += 0.1;
   for (int it = 0; it < 100; ++it) {
-    double f = fn.f(x);
+    double f, d;
+    fn.fd(x, f, d);
     if (fabs(f) < atol) {
       out = x;
       return it + 1;
     }
-    double d = fn.d(x);
     if (fabs(d) < 1e-12) return -1;
     if (!isfinite(d) || !isfinite(f)) return -1;
     x -= 0.5 * f / d;

@@ -102,6 +102,14 @@ int32_t launch_nbody_long(const LongLaunch& p, hipStream_t stream);
 // (jd_b or jd_a, xb), Sun-centred in `frame`; st_b: the b states' lookup
 // status or null.  best_a/best_b [7][n] (jd, pos, vel) carry the grid's
 // closest pair from the first kernel to the Nelder-Mead one.
+// The Earth's MOID grid at one epoch, shared by the pairs at that epoch.
+struct MoidSharedGrid {
+  double key;      // the epoch (jd_a of the pairs served)
+  int32_t st, pad;
+  double step;     // grid step (days)
+  double peri_jd;  // epoch of the perihelion state
+  double g[6][100];  // grid states x y z vx vy vz
+};
 struct MoidLaunch {
   int64_t n;
   int32_t frame;
@@ -117,6 +125,7 @@ struct MoidLaunch {
   double* grid_min;  // or null
   int32_t* status;
   int32_t* iters;    // or null
+  MoidSharedGrid* sg;  // the second orbit is the Earth at jd_a: its grid at jd_a[0]
 };
 int32_t launch_moid(const MoidLaunch& p, hipStream_t stream);
 

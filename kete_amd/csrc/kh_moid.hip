@@ -5,12 +5,14 @@
 // epochs a period/50 apart (6 days for open orbits), takes the closest of the
 // 100 x 100 position pairs, and runs Nelder-Mead in (dt_a, dt_b) from there.
 // Two kernels:
-//  - moid_grid_kernel: one wavefront per pair.  The 200 grid propagations are
-//    spread over the lanes, the positions go to LDS (x, y, z planes, so the
-//    lanes of a row read consecutive doubles), and the 10,000 distances are
-//    scanned in row-major order, each lane taking every 64th; the pair kete
-//    keeps (the first strict minimum) is the smallest (distance, index) of
-//    the lanes' candidates.
+//  - moid_grid_kernel: one wavefront per pair.  The perihelion states take
+//    one solve round (half the lanes each), the 200 grid propagations are
+//    spread over the lanes (100 when the second orbit is the Earth at the
+//    batch's first epoch: that grid is computed once), the states go to LDS,
+//    and the 10,000 distances are scanned with each lane owning one or two
+//    columns of the table; the pair kete keeps (the first strict minimum in
+//    row-major order) is the smallest (distance, index) of the lanes'
+//    candidates.
 //  - moid_nm_kernel: one lane per pair runs the Nelder-Mead iterations (two
 //    Kepler solves per cost).  An iteration that leaves the simplex unchanged
 //    (a failed contraction) repeats identically until the iteration limit, so
@@ -107,9 +109,57 @@ __device__ __forceinline__ void load_pair(const MoidLaunch& a, int64_t i, MoidSt
   }
 }
 
+// grid state k (idx = k - 50) of an orbit from its perihelion state
+__device__ __forceinline__ int32_t grid_state(const MoidState& peri, double step, int k,
+                                              MoidState& s) {
+  return two_body_to(peri, peri.jd + (double)(k - 50) * step, s);
+}
+
+// elements -> grid step (orbital_period / 50, or 300 / 50 days)
+__device__ __forceinline__ double grid_step(double period) {
+  return isfinite(period) ? period / 50.0 : 300.0 / 50.0;
+}
+
+// The Earth's grid at the first pair's epoch, shared by every pair whose
+// second orbit is the Earth at that epoch (moid_py's default, one epoch for
+// the batch): the same states each such pair would compute.
+__global__ void __launch_bounds__(64) moid_earth_grid_kernel(MoidLaunch a) {
+  MoidSharedGrid* g = a.sg;
+  const int lane = threadIdx.x;
+  MoidState sa, sb;
+  load_pair(a, 0, sa, sb);
+  int32_t st = a.st_b ? a.st_b[0] : ST_OK;
+  double pt, per;
+  comet_peri(sb.jd, sb.p, sb.v, pt, per);
+  MoidState peri = sb;
+  if (st == ST_OK) st = two_body_to(sb, pt, peri);
+  const double step = grid_step(per);
+  int32_t gst = ST_OK;
+  if (st == ST_OK) {
+    for (int k = lane; k < MOID_N; k += 64) {
+      MoidState s;
+      const int32_t r = grid_state(peri, step, k, s);
+      if (r && gst == ST_OK) gst = r;
+      g->g[0][k] = s.p.x;
+      g->g[1][k] = s.p.y;
+      g->g[2][k] = s.p.z;
+      g->g[3][k] = s.v.x;
+      g->g[4][k] = s.v.y;
+      g->g[5][k] = s.v.z;
+    }
+  }
+  gst = __any(gst != ST_OK) ? ST_CONVERGENCE : ST_OK;
+  if (lane == 0) {
+    g->key = a.jd_a[0];
+    g->st = st == ST_OK ? gst : st;
+    g->step = step;
+    g->peri_jd = peri.jd;
+  }
+}
+
 struct MoidLds {
-  double ax[MOID_N], ay[MOID_N], az[MOID_N];
-  double bx[MOID_N], by[MOID_N], bz[MOID_N];
+  double a[6][MOID_N];  // x y z vx vy vz of orbit a's grid states
+  double b[6][MOID_N];
 };
 
 __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
@@ -122,45 +172,85 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
   int32_t st = valid && a.st_b ? a.st_b[i] : ST_OK;
   MoidState pa{0.0, {1.0, 0.0, 0.0}, {0.0, 0.01, 0.0}}, pb = pa;
   if (valid) load_pair(a, i, pa, pb);
-  // each state to its perihelion; grid steps from the periods
+  const MoidSharedGrid* sg = a.sg;
+  const bool use_sg = sg != nullptr && valid && a.jd_a[i] == sg->key;
+  // both perihelion states in one solve round: lanes 0-31 orbit a, 32-63 b
   double pt_a, per_a, pt_b, per_b;
   comet_peri(pa.jd, pa.p, pa.v, pt_a, per_a);
-  if (st == ST_OK) st = two_body_to(pa, pt_a, pa);
   comet_peri(pb.jd, pb.p, pb.v, pt_b, per_b);
-  if (st == ST_OK) st = two_body_to(pb, pt_b, pb);
-  const double step_a = isfinite(per_a) ? per_a / 50.0 : 300.0 / 50.0;
-  const double step_b = isfinite(per_b) ? per_b / 50.0 : 300.0 / 50.0;
+  const bool half_b = lane >= 32;
+  MoidState peri = half_b ? pb : pa;
+  const int32_t pr = st == ST_OK ? two_body_to(peri, half_b ? pt_b : pt_a, peri) : ST_OK;
+  const MoidState qa{__shfl(peri.jd, 0, 64),
+                     Vec3{__shfl(peri.p.x, 0, 64), __shfl(peri.p.y, 0, 64),
+                          __shfl(peri.p.z, 0, 64)},
+                     Vec3{__shfl(peri.v.x, 0, 64), __shfl(peri.v.y, 0, 64),
+                          __shfl(peri.v.z, 0, 64)}};
+  const MoidState qb{__shfl(peri.jd, 32, 64),
+                     Vec3{__shfl(peri.p.x, 32, 64), __shfl(peri.p.y, 32, 64),
+                          __shfl(peri.p.z, 32, 64)},
+                     Vec3{__shfl(peri.v.x, 32, 64), __shfl(peri.v.y, 32, 64),
+                          __shfl(peri.v.z, 32, 64)}};
+  const int32_t st_pa = __shfl(pr, 0, 64), st_pb = __shfl(pr, 32, 64);
+  if (st == ST_OK) st = st_pa;
+  if (st == ST_OK) st = use_sg ? sg->st : st_pb;
+  const double step_a = grid_step(per_a);
+  const double step_b = use_sg ? sg->step : grid_step(per_b);
+  // grid states: orbit a's 100 (and b's unless shared), one per lane and round
   int32_t gst = ST_OK;
   if (st == ST_OK) {
-    for (int k = lane; k < MOID_N; k += 64) {
+    const int nj = use_sg ? MOID_N : 2 * MOID_N;
+    for (int j = lane; j < nj; j += 64) {
+      const bool ob = j >= MOID_N;
+      const int k = ob ? j - MOID_N : j;
       MoidState s;
-      int32_t r = two_body_to(pa, pa.jd + (double)(k - 50) * step_a, s);
-      L.ax[k] = s.p.x;
-      L.ay[k] = s.p.y;
-      L.az[k] = s.p.z;
+      const int32_t r = grid_state(ob ? qb : qa, ob ? step_b : step_a, k, s);
       if (r && gst == ST_OK) gst = r;
-      r = two_body_to(pb, pb.jd + (double)(k - 50) * step_b, s);
-      L.bx[k] = s.p.x;
-      L.by[k] = s.p.y;
-      L.bz[k] = s.p.z;
-      if (r && gst == ST_OK) gst = r;
+      double(*dst)[MOID_N] = ob ? L.b : L.a;
+      dst[0][k] = s.p.x;
+      dst[1][k] = s.p.y;
+      dst[2][k] = s.p.z;
+      dst[3][k] = s.v.x;
+      dst[4][k] = s.v.y;
+      dst[5][k] = s.v.z;
     }
+    if (use_sg)
+      for (int k = lane; k < MOID_N; k += 64) {
+        L.b[0][k] = sg->g[0][k];
+        L.b[1][k] = sg->g[1][k];
+        L.b[2][k] = sg->g[2][k];
+      }
   }
   __syncthreads();
   // kete returns the first grid propagation's error; all of them are
   // Kepler convergence failures
   gst = __any(gst != ST_OK) ? ST_CONVERGENCE : ST_OK;
   if (st == ST_OK) st = gst;
+  // closest grid pair: lane l owns columns l and l + 64 of the 100 x 100
+  // table (b's positions in registers) and walks the rows (a's positions,
+  // one LDS broadcast per coordinate), so each lane meets its entries in
+  // increasing row-major index; squared distances are compared
   double best = __builtin_inf();
   int kbest = 0x7fffffff;
   if (st == ST_OK) {
-    for (int k = lane; k < MOID_N * MOID_N; k += 64) {
-      const int r = k / MOID_N, c = k - r * MOID_N;
-      const double dx = L.ax[r] - L.bx[c], dy = L.ay[r] - L.by[c], dz = L.az[r] - L.bz[c];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      if (d < best) {
-        best = d;
-        kbest = k;
+    const int c0 = lane, c1 = lane + 64;
+    const bool two = c1 < MOID_N;
+    const double b0x = L.b[0][c0], b0y = L.b[1][c0], b0z = L.b[2][c0];
+    const double b1x = two ? L.b[0][c1] : 0.0, b1y = two ? L.b[1][c1] : 0.0,
+                 b1z = two ? L.b[2][c1] : 0.0;
+    for (int r = 0; r < MOID_N; ++r) {
+      const double ax = L.a[0][r], ay = L.a[1][r], az = L.a[2][r];
+      const double dx = ax - b0x, dy = ay - b0y, dz = az - b0z;
+      const double d0 = dx * dx + dy * dy + dz * dz;
+      if (d0 < best) {
+        best = d0;
+        kbest = r * MOID_N + c0;
+      }
+      const double ex = ax - b1x, ey = ay - b1y, ez = az - b1z;
+      const double d1 = ex * ex + ey * ey + ez * ez;
+      if (two && d1 < best) {
+        best = d1;
+        kbest = r * MOID_N + c1;
       }
     }
   }
@@ -173,15 +263,24 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
       kbest = ok;
     }
   }
+  best = sqrt(best);
   if (!valid || lane > 1) return;
-  // the pair kept (the perihelion states when no distance compared)
-  MoidState s = lane == 0 ? pa : pb;
+  // the pair kept: grid states from LDS (or the shared Earth grid), the
+  // perihelion states when no distance compared
+  MoidState s = lane == 0 ? qa : qb;
   if (st == ST_OK && kbest != 0x7fffffff) {
     const int r = kbest / MOID_N, c = kbest - r * MOID_N;
-    const int idx = lane == 0 ? r : c;
-    const double step = lane == 0 ? step_a : step_b;
-    const int32_t e = two_body_to(s, s.jd + (double)(idx - 50) * step, s);
-    if (e) st = e;  // recomputes a grid state, which succeeded above
+    if (lane == 0) {
+      s = MoidState{qa.jd + (double)(r - 50) * step_a, Vec3{L.a[0][r], L.a[1][r], L.a[2][r]},
+                    Vec3{L.a[3][r], L.a[4][r], L.a[5][r]}};
+    } else if (use_sg) {
+      s = MoidState{sg->peri_jd + (double)(c - 50) * step_b,
+                    Vec3{sg->g[0][c], sg->g[1][c], sg->g[2][c]},
+                    Vec3{sg->g[3][c], sg->g[4][c], sg->g[5][c]}};
+    } else {
+      s = MoidState{qb.jd + (double)(c - 50) * step_b, Vec3{L.b[0][c], L.b[1][c], L.b[2][c]},
+                    Vec3{L.b[3][c], L.b[4][c], L.b[5][c]}};
+    }
   }
   double* const* out = lane == 0 ? a.best_a : a.best_b;
   out[0][i] = s.jd;
@@ -334,6 +433,10 @@ __global__ void __launch_bounds__(256) moid_nm_kernel(MoidLaunch a) {
 
 int32_t launch_moid(const MoidLaunch& p, hipStream_t stream) {
   if (p.n <= 0) return 0;
+  if (p.sg) {
+    hipLaunchKernelGGL(moid_earth_grid_kernel, dim3(1), dim3(64), 0, stream, p);
+    if (hipGetLastError() != hipSuccess) return -1;
+  }
   hipLaunchKernelGGL(moid_grid_kernel, dim3((unsigned)((p.n + 3) / 4)), dim3(256), 0, stream, p);
   if (hipGetLastError() != hipSuccess) return -1;
   hipLaunchKernelGGL(moid_nm_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, stream,
