@@ -601,41 +601,38 @@ __device__ __forceinline__ int kepler_once(double time, Vec3 p, Vec3 v, Vec3& po
 }
 
 // analytic_2_body (kepler.rs:254-297).  kete recurses on failure into two
-// half steps (depth <= 10); this walks the same binary tree iteratively:
-// pending[d] counts half steps still to take at depth d.
+// half steps, at most 10 levels deep.  Walked iteratively here: `at` counts
+// the part of `time` done and `blk` the current step, both in units of
+// time / 512 (the step at depth 10).  After a step the next one is the
+// largest block aligned at `at` -- the pending sibling or ancestor half --
+// which is the order the recursion takes them in; a failure halves the
+// block, and a failure at depth 10 is kete's recursion-depth error.
 __device__ __forceinline__ int32_t analytic_2_body(double time, Vec3 pos, Vec3 vel, Vec3& po,
                                                    Vec3& vo, int* iters_out = nullptr) {
   int iters = 0;
-  int pending[11];
-  double dts[11];
-#pragma unroll
-  for (int k = 0; k < 11; ++k) pending[k] = 0;
-  int depth = 1;  // depth after kete's increment of the top-level call
-  pending[1] = 1;
-  dts[1] = time;
   Vec3 p = pos, v = vel;
-  while (depth > 0) {
-    if (pending[depth] == 0) {
-      --depth;
-      continue;
+  int at = 0, blk = 512;
+  while (at < 512) {
+    const double t = time * ((double)blk * (1.0 / 512.0));  // exact: a power of two
+    bool ok = true;
+    if (!(fabs(t) < 1e-10)) {
+      const double rv0 = dot(p, v);
+      if (isnan(rv0) || isinf(rv0)) return ST_CONVERGENCE;
+      Vec3 np, nv;
+      if (kepler_once(t, p, v, np, nv, iters) == 0) {
+        p = np;
+        v = nv;
+      } else {
+        ok = false;
+      }
     }
-    pending[depth]--;
-    double t = dts[depth];
-    if (fabs(t) < 1e-10) continue;
-    double rv0 = dot(p, v);
-    if (isnan(rv0) || isinf(rv0)) return ST_CONVERGENCE;
-    Vec3 np, nv;
-    if (kepler_once(t, p, v, np, nv, iters) == 0) {
-      p = np;
-      v = nv;
-      continue;
+    if (ok) {
+      at += blk;
+      blk = at & -at;
+    } else {
+      if (blk == 1) return ST_CONVERGENCE;
+      blk >>= 1;
     }
-    // failed: two half steps one level deeper (kete's depth check is
-    // `depth >= 10` on entry to the recursive call)
-    if (depth >= 10) return ST_CONVERGENCE;
-    ++depth;
-    pending[depth] = 2;
-    dts[depth] = 0.5 * t;
   }
   po = p;
   vo = v;
