@@ -16,7 +16,9 @@ Other workloads (--config): 1 two-body (1e5 states x 64 epochs, solves/s),
 3 uneven spans (2e6 objects, |span| log-uniform in [1, 3650] d; also timed
 through the persistent work queue), 4 field-of-view checks (1e6 states x 3000 fields, pairs/s),
 5 orbit-fit gradients (4096 NEO-like orbits x 60 observations, gradients/s,
-against finite differences of the plain propagators).
+against finite differences of the plain propagators), 6 MOID of 1e6 objects
+against the Earth (pairs/s), 7 propagate_n_body_long (1e5 objects in batches
+of 10 with the nine planets, one year, object-years/s).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--config C]
 """
@@ -49,6 +51,12 @@ F_TAN_FORCE = 560.0
 F_TAN_SWEEP = 1120.0
 F_TAN_ACCEPT = 107.0
 F_KEPLER0, F_KEPLER_IT = 97.0, 37.0   # universal Kepler: 97 + 37 per Newton iteration
+# moid per pair: the 100 x 100 grid scan (3 sub, 3 mul, 2 add per entry) and
+# the two element sets; the Kepler solves are counted by the kernels
+F_MOID_SCAN, F_MOID_ELEM = 8.0e4, 120.0
+# propagate_n_body_long per force evaluation of a batch: 24 flop per Newton
+# pair term, 76 more for the GR and J2 terms of the Sun and Jupiter
+F_LONG_PAIR, F_LONG_GRJ2 = 24.0, 76.0
 
 
 def log(msg):
@@ -561,6 +569,158 @@ def run_fov(args):
     return line
 
 
+def run_moid(args):
+    """kete moid of each object against the Earth (moid_py's default) for the
+    config-2 population: pairs/s."""
+    import torch
+
+    import kete_amd as ka
+    from kete_amd import _lib
+    from kete_amd import synthetic as syn
+    n = args.n or 1_000_000
+    xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=2)
+    ctx = ka.get_context()
+    dev = torch.device("cuda:0")
+    d_jd = torch.full((n,), syn.EPOCH_JD, dtype=torch.float64, device=dev)
+    d_xyz = torch.from_numpy(xyz).to(dev).contiguous()
+    d_moid = torch.empty(n, dtype=torch.float64, device=dev)
+    d_st = torch.empty(n, dtype=torch.int32, device=dev)
+    d_it = torch.empty(n, dtype=torch.int32, device=dev)
+    d_work = torch.zeros((2, n), dtype=torch.int32, device=dev)
+    xyz_p = _lib.ptr_array([d_xyz[k].data_ptr() for k in range(6)])
+
+    def step(work=False):
+        _lib.check(ctx.L.kh_moid_dev(
+            ctx.h, n, _lib.raw(d_jd.data_ptr()), xyz_p, ka.Frames.Ecliptic, None, None,
+            _lib.raw(d_moid.data_ptr()), _lib.raw(d_st.data_ptr(), C.c_int32), None,
+            _lib.raw(d_it.data_ptr(), C.c_int32) if work else None,
+            _lib.raw(d_work.data_ptr(), C.c_int32) if work else None, 0))
+
+    step(work=True)                      # work counters, untimed
+    torch.cuda.synchronize()
+    solves, newton = (int(d_work[k].sum().item()) for k in range(2))
+    steps = max(args.steps, 5)
+    ms, k_ms = time_steps(ctx, step, steps, args.warmup)
+    moid = d_moid.cpu().numpy()
+    st = d_st.cpu().numpy()
+    flop = F_KEPLER0 * solves + F_KEPLER_IT * newton + (F_MOID_SCAN + F_MOID_ELEM) * n
+    achieved = flop / (k_ms / 1000.0) / 1e12
+    line = {"metric": "moid pairs/s", "value": n / (ms / 1000.0), "unit": "pairs/s",
+            "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "moid_1e6_objects_vs_earth", "objects": n,
+                       "second_orbit": "earth_at_epoch"},
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "traffic": None, "kernel": "moid_grid_kernel+moid_nm_kernel",
+                         "kernel_ms": k_ms, "flop_per_launch": flop,
+                         "work": {"kepler_solves": solves, "newton_iterations": newton,
+                                  "nm_iterations_median": float(np.median(
+                                      d_it.cpu().numpy()))}},
+            "failed": int((st != 0).sum())}
+    log(f"[bench] moid {ms:.2f} ms/step, kernels {k_ms:.2f} ms, {n / (ms / 1000.0):.3e} pairs/s, "
+        f"{solves / n:.1f} solves and {newton / max(solves, 1):.2f} Newton iterations per solve, "
+        f"{achieved:.2f} TFLOP/s")
+    if not args.no_cpu:
+        import oracle
+        oracle.spk_load(syn.ephemeris_path())
+        m = min(n, int(args.cpu_seconds * 2000 * cpu_threads()))
+        idx = np.linspace(0, n - 1, m).astype(np.int64)
+        tc = time.perf_counter()
+        ref, rst, _, _ = oracle.moid_batch(syn.EPOCH_JD, np.ascontiguousarray(xyz[:, idx]),
+                                           n_threads=cpu_threads())
+        dtc = time.perf_counter() - tc
+        ok = (rst == 0) & (st[idx] == 0)
+        d = np.abs(moid[idx] - ref)[ok]
+        line["cpu_baseline"] = {"value": m / dtc, "unit": "pairs/s", "cores": cpu_threads(),
+                                "kind": "port",
+                                "sample": f"{m} evenly spaced objects, {dtc:.1f} s wall"}
+        line["parity"] = {"sample": m, "status_match": bool(np.array_equal(rst, st[idx])),
+                          "max_abs_diff_au": float(d.max()),
+                          "q99_abs_diff_au": float(np.quantile(d, 0.99))}
+        log(f"[bench] moid cpu {m / dtc:.3e} pairs/s; parity {line['parity']}")
+    return line
+
+
+def run_long(args):
+    """propagate_n_body_long: particles integrated in batches of kete's default
+    10 with the nine SIMPLE_PLANETS, one year: object-years/s."""
+    import torch
+
+    import kete_amd as ka
+    from kete_amd import _lib
+    from kete_amd import synthetic as syn
+    n = args.n or 100_000
+    batch = 10
+    span = 365.25
+    xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=2)
+    ctx = ka.get_context()
+    dev = torch.device("cuda:0")
+    d_xyz = torch.from_numpy(xyz).to(dev).contiguous()
+    d_out = torch.empty((6, n), dtype=torch.float64, device=dev)
+    nb = (n + batch - 1) // batch
+    d_st = torch.empty(nb, dtype=torch.int32, device=dev)
+    d_cnt = torch.zeros((nb, 4), dtype=torch.int32, device=dev)
+    xyz_p = _lib.ptr_array([d_xyz[k].data_ptr() for k in range(6)])
+    out_p = _lib.ptr_array([d_out[k].data_ptr() for k in range(6)])
+    jf = syn.EPOCH_JD + span
+
+    def step(counters=False):
+        _lib.check(ctx.L.kh_propagate_n_body_long_dev(
+            ctx.h, n, syn.EPOCH_JD, xyz_p, 10, ka.Frames.Ecliptic, jf, None, None, batch,
+            out_p, None, _lib.raw(d_st.data_ptr(), C.c_int32),
+            _lib.raw(d_cnt.data_ptr(), _lib.KhCounters) if counters else None, 0))
+
+    step(counters=True)
+    torch.cuda.synchronize()
+    cnt = d_cnt.cpu().numpy().astype(np.int64)
+    ms, k_ms = time_steps(ctx, step, max(args.steps, 3), args.warmup)
+    m_b = np.full(nb, batch, dtype=np.int64)
+    m_b[-1] = n - batch * (nb - 1)
+    bodies = 9 + m_b
+    pairs = 72 + 9 * m_b
+    gr_pairs = 2 * (8 + m_b)
+    flop = float(np.sum(cnt[:, 0] * (F_LONG_PAIR * pairs + F_LONG_GRJ2 * gr_pairs)
+                        + cnt[:, 1] * F_SWEEP * bodies + cnt[:, 2] * F_ACCEPT * bodies))
+    achieved = flop / (k_ms / 1000.0) / 1e12
+    st = d_st.cpu().numpy()
+    out = d_out.cpu().numpy()
+    value = n * span / 365.25 / (ms / 1000.0)
+    line = {"metric": "n-body long object-years/s", "value": value, "unit": "object-years/s",
+            "n_gpus": 1, "steps": max(args.steps, 3), "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "n_body_long_1e5_objects_1yr_batch10", "objects": n,
+                       "span_days": span, "batch_size": batch},
+            "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "traffic": None, "kernel": "radau_long_kernel", "kernel_ms": k_ms,
+                         "flop_per_launch": flop,
+                         "work": {"force_evals": int(cnt[:, 0].sum()),
+                                  "sweeps": int(cnt[:, 1].sum()),
+                                  "accepted_steps": int(cnt[:, 2].sum())}},
+            "failed_batches": int((st != 0).sum())}
+    log(f"[bench] long {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {value:.3e} obj-yr/s, "
+        f"{achieved:.2f} TFLOP/s")
+    if not args.no_cpu:
+        import oracle
+        oracle.spk_load(syn.ephemeris_path())
+        m = min(n, max(batch, int(args.cpu_seconds * 300 * cpu_threads()) // batch * batch))
+        tc = time.perf_counter()
+        rst, ref, _ = oracle.propagate_n_body_long(syn.EPOCH_JD, np.ascontiguousarray(xyz[:, :m]),
+                                                   jf, batch_size=batch,
+                                                   n_threads=cpu_threads())
+        dtc = time.perf_counter() - tc
+        line["cpu_baseline"] = {"value": m * span / 365.25 / dtc, "unit": "object-years/s",
+                                "cores": cpu_threads(), "kind": "port",
+                                "sample": f"the first {m} objects, {dtc:.1f} s wall"}
+        line["parity"] = {"sample": m, "status": int(rst),
+                          "max_pos_err_au": float(np.abs(out[:3, :m] - ref[:3]).max())}
+        log(f"[bench] long cpu {m * span / 365.25 / dtc:.3e} obj-yr/s; parity {line['parity']}")
+    return line
+
+
 def run_copy_calibration(args):
     """A plain device copy of known size (2^27 doubles read, 2^27 written) for
     scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -584,7 +744,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--fields", type=int, default=3000)
-    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4, 5])
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4, 5, 6, 7])
     ap.add_argument("--mode", type=int, default=-1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of the baseline sample")
@@ -604,6 +764,10 @@ def main():
         line = run_copy_calibration(args)
     elif args.config == 5:
         line = run_adjoint(args)
+    elif args.config == 6:
+        line = run_moid(args)
+    elif args.config == 7:
+        line = run_long(args)
     else:
         line = run_fov(args)
     print(json.dumps(line))

@@ -299,6 +299,19 @@ int32_t kh_propagate_n_body_long(kh_ctx *ctx, int64_t n, double jd0, const doubl
                                  const double *planet_xyz, const kh_nongrav *ng,
                                  int32_t batch_size, double *const out[6], double *planets_out,
                                  int32_t *status);
+/* kh_propagate_n_body_long on device pointers (xyz, ng, out; planet_xyz stays
+ * a host array or null): every batch's status in batch_status[nb] and, when
+ * not null, its planets at jd_final in batch_planets[nb][9][6] and its
+ * integrator work in batch_counters[nb]; nb = ceil(n / batch_size).  An empty
+ * batch or a center other than 10 is KH_E_ARG here; kete's call-level error
+ * is the first non-zero batch_status. */
+int32_t kh_propagate_n_body_long_dev(kh_ctx *ctx, int64_t n, double jd0,
+                                     const double *const xyz[6], int64_t center, int32_t frame,
+                                     double jd_final, const double *planet_xyz,
+                                     const kh_nongrav *ng, int32_t batch_size,
+                                     double *const out[6], double *batch_planets,
+                                     int32_t *batch_status, kh_counters *batch_counters,
+                                     int32_t sync);
 
 /* MOID (minimum orbit intersection distance, AU) of n orbit pairs.  Replaces
  * moid (src/kete_core/src/propagation/kepler.rs:325-404) called per pair, and
@@ -313,13 +326,14 @@ int32_t kh_propagate_n_body_long(kh_ctx *ctx, int64_t n, double jd0, const doubl
 int32_t kh_moid(kh_ctx *ctx, int64_t n, const double *jd_a, const double *const xyz_a[6],
                 int32_t frame, const double *jd_b, const double *const xyz_b[6], double *moid,
                 int32_t *status);
-/* kh_moid on device pointers; grid_min[n] (the closest of the 100 x 100
- * grid positions) and iters[n] (Nelder-Mead iterations) are optional.
+/* kh_moid on device pointers; optional outputs: grid_min[n] (the closest of
+ * the 100 x 100 grid positions), iters[n] (Nelder-Mead iterations, kete's
+ * count), work[2][n] (Kepler solves and their Newton iterations per pair).
  * sync = 0 returns with the work queued on the library's stream. */
 int32_t kh_moid_dev(kh_ctx *ctx, int64_t n, const double *jd_a, const double *const xyz_a[6],
                     int32_t frame, const double *jd_b, const double *const xyz_b[6],
                     double *moid, int32_t *status, double *grid_min, int32_t *iters,
-                    int32_t sync);
+                    int32_t *work, int32_t sync);
 
 /* Launch shape picked by kh_autotune_n_body. */
 typedef struct {

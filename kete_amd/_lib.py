@@ -27,7 +27,8 @@ EXPORTS = [
     "kh_propagate_two_body_dev", "kh_propagate_n_body", "kh_propagate_n_body_dev",
     "kh_last_launch", "kh_last_kernel_ms", "kh_fov_rect_normals", "kh_fov_state_check",
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
-    "kh_autotune_clear", "kh_fov_spk_check", "kh_propagate_n_body_long", "kh_moid", "kh_moid_dev",
+    "kh_autotune_clear", "kh_fov_spk_check", "kh_propagate_n_body_long",
+    "kh_propagate_n_body_long_dev", "kh_moid", "kh_moid_dev",
 ]
 
 
@@ -141,10 +142,14 @@ def load():
     L.kh_propagate_n_body_long.argtypes = [V, I64, D, PP, I64, I32, D, P, C.POINTER(KhNonGrav),
                                            I32, PP, P, C.POINTER(I32)]
     L.kh_propagate_n_body_long.restype = I32
+    L.kh_propagate_n_body_long_dev.argtypes = [V, I64, D, PP, I64, I32, D, P,
+                                               C.POINTER(KhNonGrav), I32, PP, P,
+                                               C.POINTER(I32), C.POINTER(KhCounters), I32]
+    L.kh_propagate_n_body_long_dev.restype = I32
     L.kh_moid.argtypes = [V, I64, P, PP, I32, P, PP, P, C.POINTER(I32)]
     L.kh_moid.restype = I32
     L.kh_moid_dev.argtypes = [V, I64, P, PP, I32, P, PP, P, C.POINTER(I32), P, C.POINTER(I32),
-                              I32]
+                              C.POINTER(I32), I32]
     L.kh_moid_dev.restype = I32
     L.kh_radec_vmag.argtypes = [V, I64, PP, PP, I32, P, D, PP]
     L.kh_radec_vmag.restype = I32

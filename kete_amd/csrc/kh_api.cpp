@@ -120,6 +120,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->fbuf) (void)hipFree(c->fbuf);
   if (c->hbuf) (void)hipFree(c->hbuf);
   if (c->mbuf) (void)hipFree(c->mbuf);
+  if (c->lbuf) (void)hipFree(c->lbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);
@@ -846,6 +847,102 @@ int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,
 // ---------------------------------------------------------------------------
 // propagate_n_body_long
 // ---------------------------------------------------------------------------
+// SIMPLE_PLANETS at jd0 (Sun-centred equatorial) into the device array
+// d_pl[9][6]: given by the caller (host), or read from the ephemeris
+// (try_get_state(id, jd_init, 10, Equatorial), propagation/mod.rs:163-175)
+// by the state kernel writing straight into d_pl.  *status: DAFLimits when a
+// planet is not covered.
+static int32_t long_planets(kh_ctx* c, double jd0, const double* planet_xyz, double* d_pl,
+                            int32_t* status) {
+  *status = ST_OK;
+  if (planet_xyz) {
+    HIPCHK(hipMemcpyAsync(d_pl, planet_xyz, 54 * 8, hipMemcpyHostToDevice, c->stream));
+    return KH_OK;
+  }
+  if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
+  static const int64_t ids[9] = {10, 1, 2, 3, 4, 5, 6, 7, 8};
+  Chain cen{};
+  if (!kh_build_chain(c, 10, cen)) return fail(KH_E_NO_SPK, "no path from the Sun to 0");
+  double* d_jd = d_pl + 54;
+  int32_t* d_pst = (int32_t*)(d_pl + 55);
+  HIPCHK(hipMemcpyAsync(d_jd, &jd0, 8, hipMemcpyHostToDevice, c->stream));
+  for (int b = 0; b < 9; ++b) {
+    Chain body{};
+    if (!kh_build_chain(c, ids[b], body)) return fail(KH_E_NO_SPK, "no path from a planet to 0");
+    double* o[6];
+    for (int k = 0; k < 6; ++k) o[k] = d_pl + 6 * b + k;
+    Chain none{};
+    if (launch_spk_states(c->d_spk, c->d_coef, body, cen, 0, 0, none, 10, 1, 1, d_jd, o,
+                          d_pst + b, c->stream))
+      return fail(KH_E_HIP, "spk_states launch failed");
+  }
+  int32_t pst[9];
+  HIPCHK(hipMemcpyAsync(pst, d_pst, sizeof(pst), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < 9; ++b)
+    if (pst[b] != ST_OK) {
+      *status = pst[b];
+      break;
+    }
+  return KH_OK;
+}
+
+int32_t kh_propagate_n_body_long_dev(kh_ctx* c, int64_t n, double jd0,
+                                     const double* const xyz[6], int64_t center, int32_t frame,
+                                     double jd_final, const double* planet_xyz,
+                                     const kh_nongrav* ng, int32_t batch_size,
+                                     double* const out[6], double* batch_planets,
+                                     int32_t* batch_status, kh_counters* batch_counters,
+                                     int32_t sync) {
+  if (!c || !batch_status) return fail(KH_E_ARG, "null argument");
+  if (batch_size < 1 || batch_size > 55)
+    return fail(KH_E_ARG, "batch_size must be 1 .. 55 (a batch and its nine planets share one "
+                          "wavefront)");
+  if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
+  if (n <= 0) return fail(KH_E_ARG, "empty batch");
+  if (center != 10) return fail(KH_E_ARG, "center must be 10 (the Sun)");
+  if (!xyz || !out) return fail(KH_E_ARG, "null array");
+  for (int k = 0; k < 6; ++k)
+    if (!xyz[k] || !out[k]) return fail(KH_E_ARG, "null state column");
+  HIPCHK(hipSetDevice(c->device));
+  // planets [54], their epoch, their lookup statuses [9]: 476 bytes
+  int32_t r = kh_grow(&c->lbuf, &c->lbuf_size, 512);
+  if (r) return r;
+  double* d_pl = (double*)c->lbuf;
+  int32_t pst = ST_OK;
+  r = long_planets(c, jd0, planet_xyz, d_pl, &pst);
+  if (r) return r;
+  const int64_t nb = (n + batch_size - 1) / batch_size;
+  if (pst != ST_OK) {
+    // every batch fails as kete's try_get_state does for the call
+    std::vector<int32_t> st(nb, pst);
+    HIPCHK(hipMemcpyAsync(batch_status, st.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return KH_OK;
+  }
+  LongLaunch p{};
+  p.n = n;
+  p.batch = batch_size;
+  p.frame = frame;
+  p.jd0 = jd0;
+  p.jd_final = jd_final;
+  for (int k = 0; k < 6; ++k) {
+    p.xyz[k] = xyz[k];
+    p.out[k] = out[k];
+  }
+  p.planets = d_pl;
+  p.ng = ng;
+  p.planets_out = batch_planets;
+  p.status = batch_status;
+  p.counters = (int32_t*)batch_counters;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (launch_nbody_long(p, c->stream)) return fail(KH_E_HIP, "n-body long launch failed");
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  if (sync) HIPCHK(hipStreamSynchronize(c->stream));
+  return KH_OK;
+}
+
 int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double* const xyz[6],
                                  int64_t center, int32_t frame, double jd_final,
                                  const double* planet_xyz, const kh_nongrav* ng,
@@ -874,40 +971,17 @@ int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double*
       if (ng[i].kind == 1 && ng[i].dt != 0.0)
         return fail(KH_E_ARG, "JplComet non-grav with a time delay (dt != 0) is not supported");
   HIPCHK(hipSetDevice(c->device));
-  // SIMPLE_PLANETS at jd0, Sun-centred equatorial: given, or from the
-  // ephemeris (try_get_state(id, jd_init, 10, Equatorial))
-  double planets[54];
-  if (planet_xyz) {
-    std::memcpy(planets, planet_xyz, sizeof(planets));
-  } else {
-    if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
-    static const int64_t ids[9] = {10, 1, 2, 3, 4, 5, 6, 7, 8};
-    for (int b = 0; b < 9; ++b) {
-      double col[6];
-      double* cols[6] = {&col[0], &col[1], &col[2], &col[3], &col[4], &col[5]};
-      int32_t st = 0;
-      int32_t r = kh_spk_states(c, ids[b], 10, 1, 1, &jd0, cols, &st);
-      if (r) return r;
-      if (st != ST_OK) {
-        *status = st;
-        return KH_OK;
-      }
-      for (int k = 0; k < 6; ++k) planets[6 * b + k] = col[k];
-    }
-  }
   const int64_t nb = (n + batch_size - 1) / batch_size;
-  const size_t bn = align256(n * 8), bp = align256(54 * 8), bpo = align256(nb * 54 * 8),
-               bs = align256(nb * 4), bng = align256(n * sizeof(kh_nongrav));
+  const size_t bn = align256(n * 8), bpo = align256(nb * 54 * 8), bs = align256(nb * 4),
+               bng = align256(n * sizeof(kh_nongrav));
   char* base;
-  int32_t r = staging(c, 12 * bn + bp + bpo + bs + (ng ? bng : 0), &base);
+  int32_t r = staging(c, 12 * bn + bpo + bs + (ng ? bng : 0), &base);
   if (r) return r;
   char* q = base;
   const double* d_x[6];
   double* d_o[6];
   for (int k = 0; k < 6; ++k, q += bn) d_x[k] = (double*)q;
   for (int k = 0; k < 6; ++k, q += bn) d_o[k] = (double*)q;
-  double* d_pl = (double*)q;
-  q += bp;
   double* d_plo = (double*)q;
   q += bpo;
   int32_t* d_st = (int32_t*)q;
@@ -915,26 +989,10 @@ int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double*
   kh_nongrav* d_ng = ng ? (kh_nongrav*)q : nullptr;
   for (int k = 0; k < 6; ++k)
     HIPCHK(hipMemcpyAsync((void*)d_x[k], xyz[k], n * 8, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_pl, planets, sizeof(planets), hipMemcpyHostToDevice, c->stream));
   if (ng) HIPCHK(hipMemcpyAsync(d_ng, ng, n * sizeof(kh_nongrav), hipMemcpyHostToDevice, c->stream));
-  LongLaunch p{};
-  p.n = n;
-  p.batch = batch_size;
-  p.frame = frame;
-  p.jd0 = jd0;
-  p.jd_final = jd_final;
-  for (int k = 0; k < 6; ++k) {
-    p.xyz[k] = d_x[k];
-    p.out[k] = d_o[k];
-  }
-  p.planets = d_pl;
-  p.ng = d_ng;
-  p.planets_out = d_plo;
-  p.status = d_st;
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  if (launch_nbody_long(p, c->stream)) return fail(KH_E_HIP, "n-body long launch failed");
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
-  c->timed = true;
+  r = kh_propagate_n_body_long_dev(c, n, jd0, d_x, center, frame, jd_final, planet_xyz, d_ng,
+                                   batch_size, d_o, d_plo, d_st, nullptr, 0);
+  if (r) return r;
   std::vector<int32_t> st(nb);
   HIPCHK(hipMemcpyAsync(st.data(), d_st, nb * 4, hipMemcpyDeviceToHost, c->stream));
   for (int k = 0; k < 6; ++k)
@@ -958,7 +1016,7 @@ int32_t kh_propagate_n_body_long(kh_ctx* c, int64_t n, double jd0, const double*
 int32_t kh_moid_dev(kh_ctx* c, int64_t n, const double* jd_a, const double* const xyz_a[6],
                     int32_t frame, const double* jd_b, const double* const xyz_b[6],
                     double* moid, int32_t* status, double* grid_min, int32_t* iters,
-                    int32_t sync) {
+                    int32_t* work, int32_t sync) {
   if (!c) return fail(KH_E_ARG, "null context");
   if (n < 0) return fail(KH_E_ARG, "negative size");
   if (frame != 1 && frame != 17) return fail(KH_E_ARG, "frame must be 1 or 17");
@@ -1009,6 +1067,7 @@ int32_t kh_moid_dev(kh_ctx* c, int64_t n, const double* jd_a, const double* cons
   p.grid_min = grid_min;
   p.status = status;
   p.iters = iters;
+  p.work = work;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (launch_moid(p, c->stream)) return fail(KH_E_HIP, "moid launch failed");
   HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -1050,7 +1109,7 @@ int32_t kh_moid(kh_ctx* c, int64_t n, const double* jd_a, const double* const xy
       HIPCHK(hipMemcpyAsync((void*)d_xb[k], xyz_b[k], n * 8, hipMemcpyHostToDevice, c->stream));
   }
   r = kh_moid_dev(c, n, d_jda, d_xa, frame, d_jdb, xyz_b ? d_xb : nullptr, d_moid, d_st, nullptr,
-                  nullptr, 0);
+                  nullptr, nullptr, 0);
   if (r) return r;
   HIPCHK(hipMemcpyAsync(moid, d_moid, n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(status, d_st, n * 4, hipMemcpyDeviceToHost, c->stream));

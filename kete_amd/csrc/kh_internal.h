@@ -92,7 +92,7 @@ struct LongLaunch {
   const double* planets;    // [9][6] Sun-centred equatorial at jd0
   const void* ng;           // [n] kh_nongrav or null
   double* out[6];           // particles at jd_final, Sun-centred equatorial
-  double* planets_out;      // [n_batches][9][6]
+  double* planets_out;      // [n_batches][9][6] or null
   int32_t* status;          // [n_batches]
   int32_t* counters;        // [n_batches][4] or null
 };
@@ -126,6 +126,7 @@ struct MoidLaunch {
   int32_t* status;
   int32_t* iters;    // or null
   MoidSharedGrid* sg;  // the second orbit is the Earth at jd_a: its grid at jd_a[0]
+  int32_t* work;       // [2][n] Kepler solves and Newton iterations per pair, or null
 };
 int32_t launch_moid(const MoidLaunch& p, hipStream_t stream);
 

@@ -88,9 +88,9 @@ struct MoidState {
 
 // propagate_two_body (kepler.rs:306-322) to jd_final
 __device__ __forceinline__ int32_t two_body_to(const MoidState& s, double jd_final,
-                                               MoidState& o) {
+                                               MoidState& o, int* newton = nullptr) {
   Vec3 po, vo;
-  const int32_t st = analytic_2_body(jd_final - s.jd, s.p, s.v, po, vo);
+  const int32_t st = analytic_2_body(jd_final - s.jd, s.p, s.v, po, vo, newton);
   o = MoidState{jd_final, po, vo};
   return st;
 }
@@ -111,8 +111,8 @@ __device__ __forceinline__ void load_pair(const MoidLaunch& a, int64_t i, MoidSt
 
 // grid state k (idx = k - 50) of an orbit from its perihelion state
 __device__ __forceinline__ int32_t grid_state(const MoidState& peri, double step, int k,
-                                              MoidState& s) {
-  return two_body_to(peri, peri.jd + (double)(k - 50) * step, s);
+                                              MoidState& s, int* newton = nullptr) {
+  return two_body_to(peri, peri.jd + (double)(k - 50) * step, s, newton);
 }
 
 // elements -> grid step (orbital_period / 50, or 300 / 50 days)
@@ -180,7 +180,9 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
   comet_peri(pb.jd, pb.p, pb.v, pt_b, per_b);
   const bool half_b = lane >= 32;
   MoidState peri = half_b ? pb : pa;
-  const int32_t pr = st == ST_OK ? two_body_to(peri, half_b ? pt_b : pt_a, peri) : ST_OK;
+  int it_peri = 0;
+  const int32_t pr =
+      st == ST_OK ? two_body_to(peri, half_b ? pt_b : pt_a, peri, &it_peri) : ST_OK;
   const MoidState qa{__shfl(peri.jd, 0, 64),
                      Vec3{__shfl(peri.p.x, 0, 64), __shfl(peri.p.y, 0, 64),
                           __shfl(peri.p.z, 0, 64)},
@@ -198,13 +200,17 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
   const double step_b = use_sg ? sg->step : grid_step(per_b);
   // grid states: orbit a's 100 (and b's unless shared), one per lane and round
   int32_t gst = ST_OK;
+  int n_solve = 0, n_newton = 0;  // work counters of the grid propagations
   if (st == ST_OK) {
     const int nj = use_sg ? MOID_N : 2 * MOID_N;
     for (int j = lane; j < nj; j += 64) {
       const bool ob = j >= MOID_N;
       const int k = ob ? j - MOID_N : j;
       MoidState s;
-      const int32_t r = grid_state(ob ? qb : qa, ob ? step_b : step_a, k, s);
+      int it = 0;
+      const int32_t r = grid_state(ob ? qb : qa, ob ? step_b : step_a, k, s, &it);
+      ++n_solve;
+      n_newton += it;
       if (r && gst == ST_OK) gst = r;
       double(*dst)[MOID_N] = ob ? L.b : L.a;
       dst[0][k] = s.p.x;
@@ -264,6 +270,17 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
     }
   }
   best = sqrt(best);
+  if (a.work) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      n_solve += __shfl_xor(n_solve, o, 64);
+      n_newton += __shfl_xor(n_newton, o, 64);
+    }
+    // the perihelion solves the pair needs (lane 0: a, lane 32: b)
+    const int it_a = __shfl(it_peri, 0, 64), it_b = __shfl(it_peri, 32, 64);
+    n_solve += use_sg ? 1 : 2;
+    n_newton += it_a + (use_sg ? 0 : it_b);
+  }
   if (!valid || lane > 1) return;
   // the pair kept: grid states from LDS (or the shared Earth grid), the
   // perihelion states when no distance compared
@@ -293,13 +310,17 @@ __global__ void __launch_bounds__(256) moid_grid_kernel(MoidLaunch a) {
   if (lane == 0) {
     a.status[i] = st;
     if (a.grid_min) a.grid_min[i] = st == ST_OK ? best : __builtin_nan("");
+    if (a.work) {
+      a.work[i] = n_solve;
+      a.work[a.n + i] = n_newton;
+    }
   }
 }
 
 // MoidCost::cost (kepler.rs:333-344); the two propagations share one solve
 // site (a loop of two trips) to keep the kernel's registers down
 __device__ __forceinline__ int32_t moid_cost(const MoidState& sa, const MoidState& sb,
-                                             double pa, double pb, double& c) {
+                                             double pa, double pb, double& c, int& newton) {
   Vec3 x[2];
   int32_t st = ST_OK;
 #pragma unroll 1
@@ -307,7 +328,9 @@ __device__ __forceinline__ int32_t moid_cost(const MoidState& sa, const MoidStat
     const MoidState& s = k == 0 ? sa : sb;
     const double dt = ((k == 0 ? pa : pb) + s.jd) - s.jd;
     Vec3 vo;
-    const int32_t r = analytic_2_body(dt, s.p, s.v, x[k], vo);
+    int it = 0;
+    const int32_t r = analytic_2_body(dt, s.p, s.v, x[k], vo, &it);
+    newton += it;
     if (r && st == ST_OK) st = r;
   }
   const Vec3 d{x[0].x - x[1].x, x[0].y - x[1].y, x[0].z - x[1].z};
@@ -353,11 +376,13 @@ __global__ void __launch_bounds__(256) moid_nm_kernel(MoidLaunch a) {
   double px = v0.x, py = v0.y;  // the point the next evaluation is at
   double x0 = 0.0, y0 = 0.0, xr = 0.0, yr = 0.0, cr = 0.0;
   int32_t phase = NM_INIT0, it = 0;
+  int n_solve = 0, n_newton = 0;
   bool done = st != ST_OK;
   while (__any(!done)) {
     if (done) continue;
     double c;
-    st = moid_cost(sa, sb, px, py, c);
+    st = moid_cost(sa, sb, px, py, c, n_newton);
+    n_solve += 2;
     if (st != ST_OK) {
       done = true;
       continue;
@@ -429,6 +454,10 @@ __global__ void __launch_bounds__(256) moid_nm_kernel(MoidLaunch a) {
   a.moid[i] = st == ST_OK ? v0.c : __builtin_nan("");
   a.status[i] = st;
   if (a.iters) a.iters[i] = it;
+  if (a.work) {
+    a.work[i] += n_solve;
+    a.work[a.n + i] += n_newton;
+  }
 }
 
 int32_t launch_moid(const MoidLaunch& p, hipStream_t stream) {

@@ -1471,9 +1471,11 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
   const Vec3 rx = ok ? Vec3{x.x - sx.x, x.y - sx.y, x.z - sx.z} : Vec3{nan, nan, nan};
   const Vec3 rv = ok ? Vec3{v.x - sv.x, v.y - sv.y, v.z - sv.z} : Vec3{nan, nan, nan};
   if (li < 9) {
-    double* po = a.planets_out + (gid * 9 + li) * 6;
-    po[0] = rx.x; po[1] = rx.y; po[2] = rx.z;
-    po[3] = rv.x; po[4] = rv.y; po[5] = rv.z;
+    if (a.planets_out) {
+      double* po = a.planets_out + (gid * 9 + li) * 6;
+      po[0] = rx.x; po[1] = rx.y; po[2] = rx.z;
+      po[3] = rv.x; po[4] = rv.y; po[5] = rv.z;
+    }
   } else {
     a.out[0][pi] = rx.x;
     a.out[1][pi] = rx.y;

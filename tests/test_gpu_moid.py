@@ -42,7 +42,7 @@ def run_dev(ctx, jd_a, xa, frame=ka.Frames.Ecliptic, jd_b=None, xb=None):
         frame, _lib.raw(d_jdb.data_ptr()) if d_jdb is not None else None,
         _lib.ptr_array([d_xb[k].data_ptr() for k in range(6)]) if d_xb is not None else None,
         _lib.raw(moid.data_ptr()), _lib.raw(st.data_ptr(), C.c_int32), _lib.raw(gmin.data_ptr()),
-        _lib.raw(it.data_ptr(), C.c_int32), 1))
+        _lib.raw(it.data_ptr(), C.c_int32), None, 1))
     return moid.cpu().numpy(), st.cpu().numpy(), gmin.cpu().numpy(), it.cpu().numpy()
 
 
