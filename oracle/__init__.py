@@ -35,6 +35,11 @@ class Counters(C.Structure):
                 ("n_acc", C.c_int64), ("n_fail", C.c_int64)]
 
 
+class KoState(C.Structure):
+    _fields_ = [("jd", C.c_double), ("pos", C.c_double * 3), ("vel", C.c_double * 3),
+                ("frame", C.c_int32), ("center", C.c_int64)]
+
+
 class NonGrav(C.Structure):
     _fields_ = [("kind", C.c_int32)] + [(k, C.c_double) for k in
                                         ("a1", "a2", "a3", "alpha", "r_0", "m", "n", "k", "dt", "beta")]
@@ -98,6 +103,9 @@ def lib():
         L.ko_fov_spk_check.restype = I64
         L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
                                        C.POINTER(I64), C.POINTER(I64), P]
+        L.ko_radau_one_body.argtypes = [I64, D, D, P, P, P, P]
+        L.ko_change_frame.argtypes = [C.POINTER(KoState), I32]
+        L.ko_change_center.argtypes = [C.POINTER(KoState), I64]
         L.ko_comet_elements_peri.restype = None
         L.ko_comet_elements_peri.argtypes = [D, P, P, P, P]
         L.ko_moid_batch.restype = None
@@ -341,6 +349,29 @@ def fov_spk_check(ids, field_jd, obs_xyz, normals, frame=FRAME_ECLIPTIC, cap=Non
     if nh < 0:
         raise RuntimeError("hit capacity exceeded")
     return hf[:nh].copy(), ho[:nh].copy(), hs[:nh].copy()
+
+
+def radau_one_body(naif_id, t0, t1, pos, vel):
+    """Test hook: kete's Radau with one PLANETS body at rest at the origin
+    (its GravParams::add_acceleration as the whole force), equatorial."""
+    p, v = _v3(pos), _v3(vel)
+    po, vo = np.zeros(3), np.zeros(3)
+    st = lib().ko_radau_one_body(naif_id, t0, t1, _d(p), _d(v), _d(po), _d(vo))
+    return st, po, vo
+
+
+def change_frame(jd, pos, vel, frame, center, new_frame):
+    """State::try_change_frame_mut -> (status, pos, vel)."""
+    s = KoState(jd, (C.c_double * 3)(*pos), (C.c_double * 3)(*vel), frame, center)
+    st = lib().ko_change_frame(C.byref(s), new_frame)
+    return st, np.array(s.pos[:]), np.array(s.vel[:])
+
+
+def change_center(jd, pos, vel, frame, center, new_center):
+    """State::try_change_center through the loaded SPK -> (status, pos, vel)."""
+    s = KoState(jd, (C.c_double * 3)(*pos), (C.c_double * 3)(*vel), frame, center)
+    st = lib().ko_change_center(C.byref(s), new_center)
+    return st, np.array(s.pos[:]), np.array(s.vel[:])
 
 
 def comet_elements_peri(epoch, pos, vel):

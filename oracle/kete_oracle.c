@@ -1074,6 +1074,27 @@ int ko_radau_central(double t0, double t1, const double pos[3],
     return radau_integrate(central_force, NULL, pos, vel, t0, t1, pos_out, vel_out, cnt);
 }
 
+/* Test hook (no kete counterpart): kete's integrator with one PLANETS body
+ * at rest at the origin, its GravParams::add_acceleration the whole force
+ * (Newton, and the GR and J2 terms of the Sun, Jupiter and the Earth), so
+ * the terms can be pinned by the orbital effects they cause. */
+static int one_body_force(double t, const double *x, const double *v, int exact,
+                          double *a, void *ctx) {
+    (void)t; (void)exact;
+    a[0] = a[1] = a[2] = 0.0;
+    grav_add_acceleration((const grav_params *)ctx, a, x, v);
+    return KO_OK;
+}
+
+int ko_radau_one_body(int64_t naif_id, double t0, double t1, const double pos[3],
+                      const double vel[3], double pos_out[3], double vel_out[3]) {
+    for (int b = 0; b < N_PLANETS; ++b)
+        if (PLANETS[b].naif_id == naif_id)
+            return radau_integrate(one_body_force, (void *)&PLANETS[b], pos, vel, t0, t1,
+                                   pos_out, vel_out, NULL);
+    return KO_VALUE_ERROR;
+}
+
 typedef struct {
     const ko_nongrav *ng;
     int include_asteroids;

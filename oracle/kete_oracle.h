@@ -115,6 +115,10 @@ int ko_radau_central(double t0, double t1, const double pos[3],
                      const double vel[3], double pos_out[3], double vel_out[3],
                      ko_counters *cnt);
 /* propagate_n_body_spk (propagation/mod.rs:70-118). */
+/* Test hook: Radau with one PLANETS body (naif_id) at rest at the origin,
+ * GravParams::add_acceleration as the force; equatorial states. */
+int ko_radau_one_body(int64_t naif_id, double t0, double t1, const double pos[3],
+                      const double vel[3], double pos_out[3], double vel_out[3]);
 int ko_propagate_n_body(ko_state *s, double jd_final, const ko_nongrav *ng,
                         ko_counters *cnt, int64_t *impact_id, double *impact_jd);
 /* ... with kete's include_extended flag (propagation/mod.rs:73,82-88). */
