@@ -131,3 +131,19 @@ def test_center_round_trip(oracle_spk):
     assert st == 0
     assert np.abs(p2 - p).max() < 10 * np.finfo(float).eps * 2
     assert np.abs(v2 - v).max() < 10 * np.finfo(float).eps * 0.02
+
+
+def test_dust_beta_one_moves_linearly(oracle_spk):
+    """kete's test_a_terms (src/tests/test_propagation.py:61-70): a dust grain
+    with beta = 1 feels no net solar gravity, so over 10 days it moves on
+    its initial velocity to 1e-5 AU (the planets and the Poynting-Robertson
+    term are all that is left); without the model it does not."""
+    from kete_amd import synthetic as syn
+    x0 = np.array([[10.0], [0.0], [0.0], [0.0], [0.01], [0.0]])
+    dust = [oracle.NonGrav(kind=2, beta=1.0)]
+    r = oracle.batch_n_body(syn.EPOCH_JD, x0, syn.EPOCH_JD + 10.0, nongravs=dust)
+    assert r["status"][0] == 0
+    lin = x0[:3, 0] + x0[3:, 0] * 10.0
+    assert np.allclose(r["out"][:3, 0], lin, atol=1e-5)
+    r0 = oracle.batch_n_body(syn.EPOCH_JD, x0, syn.EPOCH_JD + 10.0)
+    assert np.abs(r0["out"][:3, 0] - lin).max() > 1e-4
