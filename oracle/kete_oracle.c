@@ -1365,7 +1365,9 @@ static int radau_vec_step(radau_vec_t *R, double h, double *next_h) {
                 for (int k = 1; k < 7; ++k) acc += R->G[k * D + d] * C_MAT[k][c];
                 R->B[c * D + d] = acc;
             }
-        double rmax = -INFINITY, fmax_ = -INFINITY, bmax = -INFINITY;
+        /* nalgebra's max: a fold of f64::max from the first component, so
+         * NaN components are skipped and only an all-NaN vector gives NaN */
+        double rmax = NAN, fmax_ = NAN, bmax = NAN;
         for (int64_t d = 0; d < D; ++d) {
             double fm = fabs(R->ev[d]) + 1e-6;
             double r = fabs(R->B[6 * D + d] - R->bprev[d]) / fm;
