@@ -1202,12 +1202,14 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
 // (propagate_n_body_vec, propagation/mod.rs:142-237), the bodies pulling on
 // each other and on the particles (vec_accel, acceleration.rs:197-249).
 // One lane per body of the batch, in kete's order (the nine planets, then the
-// particles); a batch takes an aligned group of 32 (<= 23 particles) or 64
-// lanes, and every integrator decision is the group's: convergence is the
-// largest |dB6|/(|a|+1e-6) over all its components, the step ratio uses the
-// largest |a|+1e-6 and |B6| (radau.rs:313-342), and an impact anywhere ends
-// the batch.  The massive bodies' positions at each evaluation point reach
-// the other lanes by shuffles.
+// particles); a batch takes a group of G = 9 + batch lanes, packed
+// floor(64 / G) groups to a wavefront (three of 19 lanes at kete's default
+// batch of 10), or a whole wavefront when G > 32.  Every integrator decision
+// is the group's: convergence is the largest |dB6|/(|a|+1e-6) over all its
+// components, the step ratio uses the largest |a|+1e-6 and |B6|
+// (radau.rs:313-342), and an impact anywhere ends the batch.  The massive
+// bodies' positions at each evaluation point reach the other lanes by
+// shuffles.
 // ---------------------------------------------------------------------------
 // SIMPLE_PLANETS (gravity.rs:150-154, EM_BARY gravity.rs:137-143):
 // id, mass, radius, J2 and the terms of GravParams::add_acceleration.
@@ -1224,20 +1226,27 @@ __constant__ BodyDev kSimplePlanets[9] = {
     {-1, 0, 8, 0, 5.15138377265457e-05 * GMS, 0.0001655371154958558, 0.0, 0.0},
 };
 
-template <int PAD>
-__device__ __forceinline__ double group_maxd(double v) {
+// Maximum over a group of lanes [base, base + G) (G <= 64; li = lane - base):
+// an inclusive max-scan from the group's first lane (each step takes the
+// lane `o` below when it is in the group), read at the group's last lane.
+// fmax skips NaN as nalgebra's max does.
+__device__ __forceinline__ double group_maxd(double v, int li, int last) {
 #pragma unroll
-  for (int o = PAD / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(v, o, 64);
+    v = li >= o ? fmax(v, u) : v;
+  }
+  return __shfl(v, last, 64);
 }
-template <int PAD>
-__device__ __forceinline__ int group_maxi(int v) {
+__device__ __forceinline__ int group_maxi(int v, int li, int last) {
 #pragma unroll
-  for (int o = PAD / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    v = li >= o ? max(v, u) : v;
+  }
+  return __shfl(v, last, 64);
 }
 
-template <int PAD>
 __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
   __shared__ LdsTables lds;
   for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
@@ -1250,12 +1259,17 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int li = lane % PAD;            // index of the body in its batch
-  const int base = lane - li;           // first lane of the group
-  const int64_t gid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / PAD;
+  const int gl = 9 + a.batch;            // lanes per group
+  const int gpw = 64 / gl;               // groups per wavefront
+  const int slot = min(lane / gl, gpw);  // gpw: the wavefront's spare lanes
+  const int li = lane - slot * gl;       // index of the body in its batch
+  const int base = lane - li;            // first lane of the group
+  const int last = min(base + gl - 1, 63);
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t gid = wave * gpw + slot;
   const int64_t n_batches = (a.n + a.batch - 1) / a.batch;
   const int64_t p0 = gid * a.batch;     // first particle of the batch
-  const int m = gid < n_batches ? (int)min((int64_t)a.batch, a.n - p0) : 0;
+  const int m = (slot < gpw && gid < n_batches) ? (int)min((int64_t)a.batch, a.n - p0) : 0;
   // this lane carries a body of a batch (groups past the last batch, m == 0,
   // carry none: their planet lanes must not load or store)
   const bool body = m > 0 && li < 9 + m;
@@ -1333,7 +1347,7 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
       if (sb && st == ST_OK) st = sb;
       acc = Vec3{acc.x + w.x, acc.y + w.y, acc.z + w.z};
     }
-    const int32_t gst = group_maxi<PAD>(live ? st : 0);
+    const int32_t gst = group_maxi(live ? st : 0, li, last);
     const double ev[3] = {acc.x, acc.y, acc.z};
     // ---- G column j (radau.rs:292-308) ----
     {
@@ -1377,9 +1391,9 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
         bmx = fmax(bmx, fabs(B[6][d]));
       }
     }
-    rmax = group_maxd<PAD>(rmax);
-    fmx = group_maxd<PAD>(fmx);
-    bmx = group_maxd<PAD>(bmx);
+    rmax = group_maxd(rmax, li, last);
+    fmx = group_maxd(fmx, li, last);
+    bmx = group_maxd(bmx, li, last);
     const bool conv = es && rmax < 1e-14;
     // ---- accepted step (radau.rs:320-338) ----
     {
@@ -1496,13 +1510,10 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
 int32_t launch_nbody_long(const LongLaunch& p, hipStream_t stream) {
   if (p.n <= 0 || p.batch <= 0 || p.batch > 55) return -1;
   const int64_t nb = (p.n + p.batch - 1) / p.batch;
-  const int pad = p.batch + 9 <= 32 ? 32 : 64;
-  const int64_t lanes = nb * pad;
-  const unsigned grid = (unsigned)((lanes + 255) / 256);
-  if (pad == 32)
-    hipLaunchKernelGGL(radau_long_kernel<32>, dim3(grid), dim3(256), 0, stream, p);
-  else
-    hipLaunchKernelGGL(radau_long_kernel<64>, dim3(grid), dim3(256), 0, stream, p);
+  const int64_t gpw = 64 / (9 + p.batch);
+  const int64_t waves = (nb + gpw - 1) / gpw;
+  const unsigned grid = (unsigned)((waves + 3) / 4);
+  hipLaunchKernelGGL(radau_long_kernel, dim3(grid), dim3(256), 0, stream, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
