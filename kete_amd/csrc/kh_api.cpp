@@ -856,7 +856,9 @@ static int32_t long_planets(kh_ctx* c, double jd0, const double* planet_xyz, dou
                             int32_t* status) {
   *status = ST_OK;
   if (planet_xyz) {
+    // the caller's array may be gone before queued work reads it: wait
     HIPCHK(hipMemcpyAsync(d_pl, planet_xyz, 54 * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return KH_OK;
   }
   if (!c->has_spk) return fail(KH_E_NO_SPK, "no ephemeris uploaded");
