@@ -94,3 +94,24 @@ def test_moid_python_surface(ctx, oracle_spk, mix):
     rm, _, _, _ = oracle.moid_batch(JD0, mix[:, :1], jd_b=JD0, xyz_b=mix[:, 1:2])
     assert abs(one[0] - rm[0]) < 1e-9
     assert ka.moid(ka.StateArray(np.zeros(0), np.zeros((6, 0)))).shape == (0,)
+
+
+def test_moid_edges(ctx, oracle_spk, mix):
+    # ragged sizes for the four-pairs-per-block grid launch, a NaN state,
+    # a hyperbolic orbit (6-day grid steps) and a near-parabolic one
+    xa = mix[:, :7].copy()
+    xa[:, 2] = np.nan
+    r = np.array([0.9, 0.2, 0.05])
+    xa[:3, 4], xa[3:, 4] = r, np.array([0.0, 0.035, 0.004])        # e > 1
+    # perpendicular to r at the escape speed: e = 1 (kete's parabolic branch)
+    v_esc = np.sqrt(2 * 0.00029591220828411956 / np.linalg.norm(r))
+    perp = np.array([-0.2, 0.9, 0.0]) / np.hypot(0.2, 0.9)
+    xa[:3, 5], xa[3:, 5] = r, v_esc * perp
+    for n in (1, 5, 7):
+        m, s, g, it = run_dev(ctx, JD0, xa[:, :n])
+        rm, rs, rg, rit = oracle.moid_batch(JD0, xa[:, :n])
+        assert np.array_equal(s, rs), (n, s, rs)
+        ok = rs == 0
+        assert np.abs(m[ok] - rm[ok]).max() < 1e-6
+        assert np.all(np.isnan(m[~ok]))
+    assert s[2] != 0
