@@ -27,7 +27,9 @@ def _equ_to_ecl(v):
 @dataclass
 class RectangleFOV:
     """A rectangular field: pointing vector, rotation, widths (degrees, as
-    kete's constructor takes them), and the observer's Sun-centred state."""
+    kete's constructor takes them), and the observer's Sun-centred state.
+    A field read from a kete file carries its four edge normals instead
+    (``from_normals``)."""
     pointing: np.ndarray
     rotation: float
     observer_jd: float
@@ -36,8 +38,24 @@ class RectangleFOV:
     lon_width: float
     lat_width: float
     frame: int = FRAME_ECLIPTIC
+    observer_desig: object = None
+    observer_center: int = 10
+    edge_normals: np.ndarray = None
+
+    @classmethod
+    def from_normals(cls, normals, rotation, observer_jd, observer_pos, observer_vel,
+                     frame=FRAME_ECLIPTIC, observer_desig=None, observer_center=10):
+        """A field given by its four great-circle edge normals (kete's
+        SphericalPolygon<4>, ``fov/patches.rs:80-91``), rotation in degrees."""
+        return cls(None, float(rotation), float(observer_jd),
+                   np.asarray(observer_pos, dtype=np.float64),
+                   np.asarray(observer_vel, dtype=np.float64), float("nan"), float("nan"),
+                   frame, observer_desig, observer_center,
+                   np.asarray(normals, dtype=np.float64).reshape(4, 3))
 
     def normals(self) -> np.ndarray:
+        if self.edge_normals is not None:
+            return self.edge_normals
         L = _lib.load()
         p = np.ascontiguousarray(self.pointing, dtype=np.float64).reshape(3)
         out = np.zeros(12)
@@ -130,6 +148,28 @@ def fov_state_check(obj_state, fovs, dt_limit=3.0, include_asteroids=False, ctx=
         # views of the first k entries (the untouched rest of the capacity
         # is never paged in)
         return FovHits(hf[:k], ho[:k], hs[:, :k])
+
+
+def to_simultaneous_states(hits: FovHits, fovs, desigs=None):
+    """kete's result form: one SimultaneousStates per field with hits, in
+    field order, holding the visible objects' states (Sun-centred ecliptic,
+    at the light-time-corrected epoch) and the field
+    (``src/kete/rust/fovs/checks.rs:96-109``).  desigs[i] names object i."""
+    from . import State
+    from .io import SimultaneousStates
+    out = []
+    if len(hits) == 0:
+        return out
+    bounds = np.flatnonzero(np.diff(hits.field)) + 1
+    for lo, hi in zip(np.r_[0, bounds], np.r_[bounds, len(hits)]):
+        f = int(hits.field[lo])
+        states = [State(desigs[int(hits.obj[k])] if desigs is not None else None,
+                        float(hits.state[0, k]), hits.state[1:4, k].copy(),
+                        hits.state[4:7, k].copy(), FRAME_ECLIPTIC, 10) for k in range(lo, hi)]
+        fov = fovs[f] if fovs is not None else None
+        out.append(SimultaneousStates(states, fov.jd if fov is not None else states[0].jd, 10,
+                                      FRAME_ECLIPTIC, fov))
+    return out
 
 
 def fov_spk_check(obj_ids, fovs, ctx=None, fields=None, cap=None) -> FovHits:
