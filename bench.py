@@ -751,8 +751,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark measures one GPU; --gpus must be 1 "
+                                   "(the propagation shards by object across GPUs)"}))
         sys.exit(2)
     if args.config == 2:
         line = run_nbody(args)
