@@ -1338,8 +1338,12 @@ __global__ void __launch_bounds__(256) radau_long_kernel(LongLaunch a) {
     for (int b = 0; b < 9; ++b) {
       const Vec3 bp{__shfl(xe[0], base + b, 64), __shfl(xe[1], base + b, 64),
                     __shfl(xe[2], base + b, 64)};
-      const Vec3 bv{__shfl(ve[0], base + b, 64), __shfl(ve[1], base + b, 64),
-                    __shfl(ve[2], base + b, 64)};
+      // velocities enter only the GR term of the Sun and Jupiter (kind 1);
+      // the branch is uniform, so every lane takes part in the shuffles
+      Vec3 bv{0.0, 0.0, 0.0};
+      if (kSimplePlanets[b].kind == 1)
+        bv = Vec3{__shfl(ve[0], base + b, 64), __shfl(ve[1], base + b, 64),
+                  __shfl(ve[2], base + b, 64)};
       if (b == li || !body) continue;
       Vec3 w{0.0, 0.0, 0.0};
       const int32_t sb = add_body_at(kSimplePlanets[b], bp, bv, Vec3{xe[0], xe[1], xe[2]},
