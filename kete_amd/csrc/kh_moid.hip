@@ -4,15 +4,17 @@
 // Per pair kete moves both states to their perihelion, propagates each to 100
 // epochs a period/50 apart (6 days for open orbits), takes the closest of the
 // 100 x 100 position pairs, and runs Nelder-Mead in (dt_a, dt_b) from there.
-// Two kernels:
+// Kernels:
+//  - moid_earth_grid_kernel: the Earth's perihelion state and grid at the
+//    batch's first epoch, once, when the second orbit is the Earth.
 //  - moid_grid_kernel: one wavefront per pair.  The perihelion states take
 //    one solve round (half the lanes each), the 200 grid propagations are
 //    spread over the lanes (100 when the second orbit is the Earth at the
 //    batch's first epoch: that grid is computed once), the states go to LDS,
 //    and the 10,000 distances are scanned with each lane owning one or two
-//    columns of the table; the pair kete keeps (the first strict minimum in
-//    row-major order) is the smallest (distance, index) of the lanes'
-//    candidates.
+//    columns of the table; the pair kept (kete's first strict minimum in
+//    row-major order) is the smallest (squared distance, index) of the
+//    lanes' candidates.
 //  - moid_nm_kernel: one lane per pair runs the Nelder-Mead iterations (two
 //    Kepler solves per cost).  An iteration that leaves the simplex unchanged
 //    (a failed contraction) repeats identically until the iteration limit, so
