@@ -459,29 +459,77 @@ __device__ __forceinline__ int32_t chain_state(const SpkDev* __restrict__ spk,
 }
 
 // ---------------------------------------------------------------------------
+// sin and cos of one argument for the Kepler iteration: Cody-Waite reduction
+// by pi/2 (W. J. Cody and W. Waite, Software Manual for the Elementary
+// Functions, 1980) with pi/2 split into three doubles, the Taylor series of
+// sin (through r^17) and cos (through r^18) on |r| <= pi/4 (the first omitted
+// terms are below 1e-19), and the quadrant.  Arguments of 1e5 and beyond
+// (the elliptic iteration's b x stays within a few turns) go to the library.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void kep_sincos(double x, double& s, double& c) {
+  if (!(fabs(x) < 1.0e5)) {
+    sincos(x, &s, &c);
+    return;
+  }
+  const double k = rint(x * 0.6366197723675814);  // 2 / pi
+  double r = fma(-k, 1.5707963267948966, x);      // pi/2 = P1 + P2 + P3
+  r = fma(-k, 6.123233995736766e-17, r);
+  r = fma(-k, -1.4973849048591698e-33, r);
+  const double z = r * r;
+  // (-1)^n / (2n+1)!, n = 8 .. 1, and (-1)^n / (2n)!, n = 9 .. 1
+  double ps = 2.8114572543455206e-15;
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  const double sr = fma(r * z, ps, r);
+  double pc = -1.5619206968586225e-16;
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  pc = fma(pc, z, -0.5);
+  const double cr = fma(z, pc, 1.0);
+  const int q = (int)(long long)k & 3;
+  s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+  c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+__device__ __forceinline__ double kep_sin(double x) {
+  double s, c;
+  kep_sincos(x, s, c);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
 // Universal Kepler solver (kepler.rs:120-297, fitting/newton.rs:18-65)
 // ---------------------------------------------------------------------------
 struct KepEll {
   double r0, rv0, beta, b, dt;
   __device__ __forceinline__ double f(double x) const {
-    double g1 = sin(b * x) / b;
-    double sh = sin(b * x / 2.0);
+    double g1 = kep_sin(b * x) / b;
+    double sh = kep_sin(b * x / 2.0);
     double g2 = 2.0 * (sh * sh) / beta;
     double g3 = (x - g1) / beta;
     return r0 * g1 + rv0 * g2 + GMS * g3 - dt;
   }
   __device__ __forceinline__ double d(double x) const {
     double s, c;
-    sincos(b * x, &s, &c);
+    kep_sincos(b * x, s, c);
     double g3d = (1.0 - c) / beta;
     return r0 * c + rv0 * s + GMS * g3d;
   }
   // f and d at one point, sharing sin(b x)
   __device__ __forceinline__ void fd(double x, double& fv, double& dv) const {
     double s, c;
-    sincos(b * x, &s, &c);
+    kep_sincos(b * x, s, c);
     double g1 = s / b;
-    double sh = sin(b * x / 2.0);
+    double sh = kep_sin(b * x / 2.0);
     double g2 = 2.0 * (sh * sh) / beta;
     double g3 = (x - g1) / beta;
     fv = r0 * g1 + rv0 * g2 + GMS * g3 - dt;
@@ -567,14 +615,14 @@ __device__ __forceinline__ int solve_kepler_universal(double dt, double r0, doub
 __device__ __forceinline__ double g_1(double s, double beta) {
   if (fabs(beta) < PARABOLIC_BETA) return s;
   double bs = sqrt(fabs(beta));
-  if (beta >= 0.0) return sin(bs * s) / bs;
+  if (beta >= 0.0) return kep_sin(bs * s) / bs;
   return sinh(bs * s) / bs;
 }
 __device__ __forceinline__ double g_2(double s, double beta) {
   if (fabs(beta) < PARABOLIC_BETA) return s * s / 2.0;
   double bs = sqrt(fabs(beta)) * s;
   if (beta >= 0.0) {
-    double sh = sin(bs / 2.0);
+    double sh = kep_sin(bs / 2.0);
     return 2.0 * (sh * sh) / beta;
   }
   return (1.0 - cosh(bs)) / beta;
