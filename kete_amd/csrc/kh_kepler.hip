@@ -13,9 +13,9 @@
 
 namespace kh {
 
-// Two waves per SIMD: left to itself the compiler takes 256 VGPRs (one wave
-// per SIMD); at two it fits in 250 without spilling, and the second wave
-// hides the transcendental latency of the Newton iterations.
+// Two waves per SIMD: left to itself the compiler may take 256 VGPRs and
+// run one wave per SIMD; held to two it fits without spilling, and the second
+// wave hides the transcendental latency of the Newton iterations.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 two_body_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef, TwoBodyLaunch p) {
   const int64_t total = p.n * p.m;
