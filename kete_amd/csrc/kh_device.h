@@ -509,31 +509,40 @@ __device__ __forceinline__ double kep_sin(double x) {
 // ---------------------------------------------------------------------------
 // Universal Kepler solver (kepler.rs:120-297, fitting/newton.rs:18-65)
 // ---------------------------------------------------------------------------
+// Elliptic branch.  sin(b x) and 1 - cos(b x) come from one sin/cos pair at
+// the half angle: sin(b x) = 2 sh ch and 1 - cos(b x) = 2 sh^2, sh the
+// sin(b x / 2) kete's g_2 already takes (the identities change only the
+// rounding of the last bits).
 struct KepEll {
   double r0, rv0, beta, b, dt;
+  __device__ __forceinline__ void half(double x, double& s, double& omc) const {
+    double sh, ch;
+    kep_sincos(b * x / 2.0, sh, ch);
+    s = 2.0 * sh * ch;
+    omc = 2.0 * (sh * sh);
+  }
   __device__ __forceinline__ double f(double x) const {
-    double g1 = kep_sin(b * x) / b;
-    double sh = kep_sin(b * x / 2.0);
-    double g2 = 2.0 * (sh * sh) / beta;
+    double s, omc;
+    half(x, s, omc);
+    double g1 = s / b;
+    double g2 = omc / beta;
     double g3 = (x - g1) / beta;
     return r0 * g1 + rv0 * g2 + GMS * g3 - dt;
   }
   __device__ __forceinline__ double d(double x) const {
-    double s, c;
-    kep_sincos(b * x, s, c);
-    double g3d = (1.0 - c) / beta;
-    return r0 * c + rv0 * s + GMS * g3d;
+    double s, omc;
+    half(x, s, omc);
+    return r0 * (1.0 - omc) + rv0 * s + GMS * (omc / beta);
   }
-  // f and d at one point, sharing sin(b x)
+  // f and d at one point
   __device__ __forceinline__ void fd(double x, double& fv, double& dv) const {
-    double s, c;
-    kep_sincos(b * x, s, c);
+    double s, omc;
+    half(x, s, omc);
     double g1 = s / b;
-    double sh = kep_sin(b * x / 2.0);
-    double g2 = 2.0 * (sh * sh) / beta;
+    double g2 = omc / beta;
     double g3 = (x - g1) / beta;
     fv = r0 * g1 + rv0 * g2 + GMS * g3 - dt;
-    dv = r0 * c + rv0 * s + GMS * ((1.0 - c) / beta);
+    dv = r0 * (1.0 - omc) + rv0 * s + GMS * g2;
   }
 };
 
@@ -636,7 +645,18 @@ __device__ __forceinline__ int kepler_once(double time, Vec3 p, Vec3 v, Vec3& po
   int it = solve_kepler_universal(time, r0, v0, rv0, s, beta);
   if (it < 0) return -1;
   iters += it;
-  double g1 = g_1(s, beta), g2 = g_2(s, beta);
+  double g1, g2;
+  if (beta >= PARABOLIC_BETA) {
+    // g_1 and g_2 from one sin/cos pair at half of sqrt(beta) s
+    const double bs = sqrt(fabs(beta));
+    double sh, ch;
+    kep_sincos(bs * s / 2.0, sh, ch);
+    g1 = 2.0 * sh * ch / bs;
+    g2 = 2.0 * (sh * sh) / beta;
+  } else {
+    g1 = g_1(s, beta);
+    g2 = g_2(s, beta);
+  }
   double f = 1.0 - GMS / r0 * g2;
   double g = r0 * g1 + rv0 * g2;
   Vec3 np{p.x * f + v.x * g, p.y * f + v.y * g, p.z * f + v.z * g};
