@@ -130,11 +130,9 @@ __global__ void __launch_bounds__(256) fov_prefilter_kernel(FovPrefilter p) {
   }
 }
 
-// check_two_body (fov_like.rs:71-85) for candidates; kind 0 -> hits, kind 1
-// -> n-body list when the two-body position is within the margin.
-__global__ void __launch_bounds__(256) fov_two_body_kernel(FovTwoBody p) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= p.n_cand) return;
+// check_two_body (fov_like.rs:71-85) for one candidate; kind 0 -> hits,
+// kind 1 -> n-body list when the two-body position is within the margin.
+__device__ __forceinline__ void two_body_candidate(const FovTwoBody& p, int64_t q) {
   const int32_t f = p.cand_field[q];
   const int64_t i = p.cand_obj[q];
   const FieldDev& F = p.fields[f];
@@ -170,20 +168,31 @@ __global__ void __launch_bounds__(256) fov_two_body_kernel(FovTwoBody p) {
     if (!inside && dist > max_dist) return;
     const unsigned long long slot = atomicAdd(p.n_nb, 1ull);
     if (slot < (unsigned long long)p.nb_cap) {
-      p.nb_field[slot] = f;
+      p.nb_field[slot] = p.field_base + f;
       p.nb_obj[slot] = i;
+      p.g_jd0[slot] = jd;
+      p.g_jdf[slot] = F.jd;
+      p.g_x[0][slot] = x.x;
+      p.g_x[1][slot] = x.y;
+      p.g_x[2][slot] = x.z;
+      p.g_x[3][slot] = v.x;
+      p.g_x[4][slot] = v.y;
+      p.g_x[5][slot] = v.z;
     }
   }
 }
 
-// Gather the states of the n-body candidates and their target epochs.
-__global__ void fov_nb_gather_kernel(FovNBody p) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= p.m) return;
-  const int64_t i = p.nb_obj[q];
-  p.g_jd0[q] = p.jd[i];
-  for (int k = 0; k < 6; ++k) p.g_x[k][q] = p.x[k][i];
-  p.g_jdf[q] = p.fields[p.nb_field[q]].jd;
+// A fixed grid strides over the candidates the prefilter listed: the count is
+// read here, on the device, so no host round trip separates the two launches.
+// Candidates beyond the list's capacity were not stored; the largest count is
+// kept so that the caller can detect the overflow after the last group.
+__global__ void __launch_bounds__(256) fov_two_body_kernel(FovTwoBody p) {
+  const unsigned long long nc = *p.n_cand;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(p.cand_max, nc);
+  const int64_t m = nc < (unsigned long long)p.cand_cap ? (int64_t)nc : p.cand_cap;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += stride)
+    two_body_candidate(p, q);
 }
 
 // check_n_body tail (fov_like.rs:98-107): light-time step and containment.
@@ -205,7 +214,7 @@ __global__ void fov_nb_finish_kernel(FovNBody p) {
   if (!rect_contains(F.nrm, rel, dist)) return;
   const unsigned long long slot = atomicAdd(p.n_hit, 1ull);
   if (slot < (unsigned long long)p.hit_cap) {
-    p.hit_field[slot] = p.field_base + f;
+    p.hit_field[slot] = f;
     p.hit_obj[slot] = p.nb_obj[q];
     p.hit_state[0][slot] = jd_obs;
     p.hit_state[1][slot] = p2.x;
@@ -436,16 +445,9 @@ int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s) {
 }
 
 int32_t launch_fov_two_body(const FovTwoBody& p, hipStream_t s) {
-  if (p.n_cand <= 0) return 0;
-  hipLaunchKernelGGL(fov_two_body_kernel, dim3((unsigned)((p.n_cand + 255) / 256)), dim3(256), 0,
-                     s, p);
-  return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s) {
-  if (p.m <= 0) return 0;
-  hipLaunchKernelGGL(fov_nb_gather_kernel, dim3((unsigned)((p.m + 255) / 256)), dim3(256), 0, s,
-                     p);
+  if (p.cand_cap <= 0) return 0;
+  const int64_t g = std::min<int64_t>((p.cand_cap + 255) / 256, 2048);
+  hipLaunchKernelGGL(fov_two_body_kernel, dim3((unsigned)g), dim3(256), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -53,10 +53,10 @@ __global__ void zero_counts(unsigned long long* p) {
   if (threadIdx.x < 4) p[threadIdx.x] = 0ull;
 }
 
-// candidate and n-body list counters restart per chunk; the hit count does not
+// per group: the candidate count restarts; hits, n-body re-checks and the
+// largest candidate count accumulate over the groups
 __global__ void zero_list_counts(unsigned long long* p) {
   if (threadIdx.x == 0) p[0] = 0ull;
-  if (threadIdx.x == 2) p[2] = 0ull;
 }
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -159,7 +159,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
 
   // device memory: states (cur, big) with per-object epochs, fields, lists
   int64_t cand_cap = std::max<int64_t>(1 << 20, 8 * n);
-  int64_t nb_cap = std::max<int64_t>(1 << 16, n / 4);
+  int64_t nb_cap = std::max<int64_t>(1 << 16, 2 * n);  // all groups together
   const int64_t hit_cap = std::max<int64_t>(cap, 1);
   for (int attempt = 0; attempt < 3; ++attempt) {
     const size_t bn = al(n * 8), bs = al(n * 4);
@@ -316,17 +316,12 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       pf.cand_kind = cand_kind;
       pf.cand_obj = cand_obj;
       if (launch_fov_prefilter(pf, s)) return kh_fail(KH_E_HIP, "prefilter launch failed");
-      unsigned long long counts[4];
-      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      if ((int64_t)counts[0] > cand_cap) {
-        cand_cap = (int64_t)counts[0] * 2;
-        overflow = true;
-        break;
-      }
-      // phase 2: two-body checks of the candidates
+      // phase 2: two-body checks of the candidates; those whose field is
+      // more than dt_limit from the states' epoch join the n-body list
       FovTwoBody tb{};
-      tb.n_cand = (int64_t)counts[0];
+      tb.n_cand = c->d_counts;
+      tb.cand_max = c->d_counts + 3;
+      tb.cand_cap = cand_cap;
       tb.jd = sjd;
       for (int k = 0; k < 6; ++k) tb.x[k] = sx[k];
       tb.fields = d_fields + f0;
@@ -344,44 +339,45 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       tb.nb_cap = nb_cap;
       tb.nb_field = nb_field;
       tb.nb_obj = nb_obj;
+      tb.g_jd0 = g_jd0;
+      tb.g_jdf = g_jdf;
+      for (int k = 0; k < 6; ++k) tb.g_x[k] = g_x[k];
       if (launch_fov_two_body(tb, s)) return kh_fail(KH_E_HIP, "two-body launch failed");
-      HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      if ((int64_t)counts[2] > nb_cap) {
-        nb_cap = (int64_t)counts[2] * 2;
-        overflow = true;
-        break;
-      }
-      // phase 3: n-body re-checks (fields more than dt_limit from the epoch)
-      if (counts[2] > 0) {
-        FovNBody nb{};
-        nb.m = (int64_t)counts[2];
-        nb.jd = sjd;
-        for (int k = 0; k < 6; ++k) nb.x[k] = sx[k];
-        nb.fields = d_fields + f0;
-        nb.field_base = (int32_t)f0;
-        nb.nb_field = nb_field;
-        nb.nb_obj = nb_obj;
-        nb.g_jd0 = g_jd0;
-        nb.g_jdf = g_jdf;
-        for (int k = 0; k < 6; ++k) {
-          nb.g_x[k] = g_x[k];
-          nb.g_out[k] = g_out[k];
-        }
-        nb.g_status = g_st;
-        nb.n_hit = d_nhit;
-        nb.hit_cap = hit_cap;
-        nb.hit_field = h_field;
-        nb.hit_obj = h_obj;
-        for (int k = 0; k < 7; ++k) nb.hit_state[k] = h_state[k];
-        if (launch_fov_nb_gather(nb, s)) return kh_fail(KH_E_HIP, "gather launch failed");
-        const double* gx[6] = {g_x[0], g_x[1], g_x[2], g_x[3], g_x[4], g_x[5]};
-        r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, include_asteroids,
-                                    nullptr, 1e-6, -1,
-                                    g_out, g_st, nullptr, nullptr, nullptr, 0);
-        if (r) return r;
-        if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");
-      }
+    }
+    // the groups ran without a host round trip; the list sizes are checked
+    // once, here, and an overflow reruns the call with larger lists
+    unsigned long long counts[4];
+    HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if ((int64_t)counts[3] > cand_cap) {
+      cand_cap = (int64_t)counts[3] * 2;
+      overflow = true;
+    }
+    if ((int64_t)counts[2] > nb_cap) {
+      nb_cap = (int64_t)counts[2] * 2;
+      overflow = true;
+    }
+    // phase 3: n-body re-checks of all groups in one launch (fields more than
+    // dt_limit from their group's epoch), from the states gathered in phase 2
+    if (!overflow && counts[2] > 0) {
+      FovNBody nb{};
+      nb.m = (int64_t)counts[2];
+      nb.fields = d_fields;
+      nb.nb_field = nb_field;
+      nb.nb_obj = nb_obj;
+      for (int k = 0; k < 6; ++k) nb.g_out[k] = g_out[k];
+      nb.g_status = g_st;
+      nb.n_hit = d_nhit;
+      nb.hit_cap = hit_cap;
+      nb.hit_field = h_field;
+      nb.hit_obj = h_obj;
+      for (int k = 0; k < 7; ++k) nb.hit_state[k] = h_state[k];
+      const double* gx[6] = {g_x[0], g_x[1], g_x[2], g_x[3], g_x[4], g_x[5]};
+      r = kh_propagate_n_body_dev(c, nb.m, g_jd0, gx, 10, frame, g_jdf, 1, include_asteroids,
+                                  nullptr, 1e-6, -1,
+                                  g_out, g_st, nullptr, nullptr, nullptr, 0);
+      if (r) return r;
+      if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");
     }
     if (overflow) continue;
     unsigned long long total = 0;

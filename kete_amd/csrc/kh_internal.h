@@ -192,7 +192,9 @@ struct FovPrefilter {
 };
 
 struct FovTwoBody {
-  int64_t n_cand;
+  const unsigned long long* n_cand;  // the prefilter's count, read on the device
+  unsigned long long* cand_max;      // largest count seen (overflow check)
+  int64_t cand_cap;
   const double* jd;
   const double* x[6];
   const FieldDev* fields;
@@ -206,27 +208,27 @@ struct FovTwoBody {
   int32_t* hit_field;
   int64_t* hit_obj;
   double* hit_state[7];
+  // n-body re-checks, appended over all groups: the field (sorted index),
+  // the object, and the state the re-check starts from with its epoch and
+  // the field's epoch as the target
   unsigned long long* n_nb;
   int64_t nb_cap;
   int32_t* nb_field;
   int64_t* nb_obj;
+  double* g_jd0;
+  double* g_jdf;
+  double* g_x[6];
 };
 
 struct FovNBody {
   int64_t m;
-  const double* jd;
-  const double* x[6];
-  const FieldDev* fields;
+  const FieldDev* fields;  // all fields, in sorted order
   const int32_t* nb_field;
   const int64_t* nb_obj;
-  double* g_jd0;
-  double* g_jdf;
-  double* g_x[6];
-  double* g_out[6];
+  const double* g_out[6];
   const int32_t* g_status;
   unsigned long long* n_hit;
   int64_t hit_cap;
-  int32_t field_base;
   int32_t* hit_field;
   int64_t* hit_obj;
   double* hit_state[7];
@@ -244,7 +246,6 @@ struct RadecArgs {
 
 int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s);
 int32_t launch_fov_two_body(const FovTwoBody& p, hipStream_t s);
-int32_t launch_fov_nb_gather(const FovNBody& p, hipStream_t s);
 int32_t launch_fov_nb_finish(const FovNBody& p, hipStream_t s);
 int32_t launch_radec_vmag(const RadecArgs& p, hipStream_t s);
 
