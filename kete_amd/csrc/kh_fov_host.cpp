@@ -160,6 +160,12 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
   // device memory: states (cur, big) with per-object epochs, fields, lists
   int64_t cand_cap = std::max<int64_t>(1 << 20, 8 * n);
   int64_t nb_cap = std::max<int64_t>(1 << 16, 2 * n);  // all groups together
+  // KH_FOV_LIST_CAP sets both lists' first capacity (a caller short of device
+  // memory; the lists grow, with a rerun, when a call overflows them)
+  if (const char* lc = getenv("KH_FOV_LIST_CAP")) {
+    const long long v = atoll(lc);
+    if (v > 0) cand_cap = nb_cap = (int64_t)v;
+  }
   const int64_t hit_cap = std::max<int64_t>(cap, 1);
   for (int attempt = 0; attempt < 3; ++attempt) {
     const size_t bn = al(n * 8), bs = al(n * 4);

@@ -60,6 +60,22 @@ def test_fov_parity(ctx, oracle_spk, workload):
     assert order == list(range(len(keys)))
 
 
+def test_fov_list_overflow_reruns(ctx, workload, monkeypatch):
+    """Candidate and n-body lists far too small for the workload: the groups
+    run with truncated lists, the overflow is found once after the last
+    group, and the call reruns with lists sized from the counts it saw.  The
+    detections must be those of the default capacities, bit for bit."""
+    xyz, fjd, obs, nrm, h = workload
+    batch = ka.StateArray(np.full(xyz.shape[1], syn.EPOCH_JD), xyz)
+    ref = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    monkeypatch.setenv("KH_FOV_LIST_CAP", "8")
+    small = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    assert len(ref.field) > 50
+    np.testing.assert_array_equal(small.field, ref.field)
+    np.testing.assert_array_equal(small.obj, ref.obj)
+    np.testing.assert_array_equal(small.state, ref.state)
+
+
 @pytest.mark.parametrize("schedule", ["continuous", "kete"])
 def test_fov_parity_long_span(ctx, oracle_spk, monkeypatch, schedule):
     """400 days of fields: many small steps and kete's big-step resets
