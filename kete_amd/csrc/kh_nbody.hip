@@ -307,12 +307,24 @@ __device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
   const double span = tf - t;
   const double r0 = norm(x);
   double sum = 1.0 / (r0 * sqrt(r0));
+  // each point from the previous one (short solves converge in fewer Newton
+  // iterations than solves from the epoch); after a failed solve the
+  // remaining points count as the epoch's rate
+  Vec3 xc = x, vc = v;
+  bool ok = true;
+  const double dt = span / (ORDER_POINTS - 1);
 #pragma unroll 1
   for (int k = 1; k < ORDER_POINTS; ++k) {
-    Vec3 p, w;
-    double rk = r0;  // a failed solve counts as the epoch's rate
-    if (analytic_2_body(span * ((double)k / (ORDER_POINTS - 1)), x, v, p, w) == ST_OK)
-      rk = norm(p);
+    double rk = r0;
+    if (ok) {
+      Vec3 p, w;
+      ok = analytic_2_body(dt, xc, vc, p, w) == ST_OK;
+      if (ok) {
+        xc = p;
+        vc = w;
+        rk = norm(p);
+      }
+    }
     sum += 1.0 / (rk * sqrt(rk));
   }
   const double key = fabs(span) * (sum / ORDER_POINTS);
