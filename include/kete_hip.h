@@ -182,6 +182,29 @@ int32_t kh_propagate_n_body_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                                 double *const out[6], int32_t *status, int64_t *impact_id,
                                 double *impact_jd, kh_counters *counters, int32_t sync);
 
+/* Largest batch of kh_propagate_n_body_dev_noalloc. */
+#define KH_NOALLOC_MAX_N 2097152
+
+/* kh_propagate_n_body_dev that allocates no device memory.  No kete
+ * counterpart: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Arguments,
+ * kernels and results are those of kh_propagate_n_body_dev; the scratch of
+ * the longest-first object order (at most KH_NOALLOC_MAX_N objects, 12.6 MB)
+ * is a device array built into the library, one per GPU, held until the
+ * process ends.  Calls on one GPU share that array, so calls that overlap
+ * there (several contexts or threads on different streams) race; calls on
+ * one context's stream are ordered.  n > KH_NOALLOC_MAX_N returns KH_E_ARG.
+ * The context itself (created by kh_ctx_create, with the ephemeris already
+ * uploaded) is set up beforehand; this call allocates nothing. */
+int32_t kh_propagate_n_body_dev_noalloc(kh_ctx *ctx, int64_t n, const double *jd0,
+                                        const double *const xyz[6], int64_t center,
+                                        int32_t frame, const double *jd_final,
+                                        int32_t jd_final_stride, int32_t include_asteroids,
+                                        const kh_nongrav *ng, double epsilon, int32_t mode,
+                                        double *const out[6], int32_t *status,
+                                        int64_t *impact_id, double *impact_jd,
+                                        kh_counters *counters, int32_t sync);
+
 /* Edge normals of a rectangular field (OnSkyRectangle::new,
  * src/kete_core/src/fov/patches.rs:122-155; RectangleFOV's constructor
  * src/kete/rust/fovs/definitions.rs:401-418 converts degrees to radians

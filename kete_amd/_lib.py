@@ -29,7 +29,11 @@ EXPORTS = [
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
     "kh_autotune_clear", "kh_fov_spk_check", "kh_propagate_n_body_long",
     "kh_propagate_n_body_long_dev", "kh_moid", "kh_moid_dev",
+    "kh_propagate_n_body_dev_noalloc",
 ]
+
+# Largest batch of kh_propagate_n_body_dev_noalloc (KH_NOALLOC_MAX_N)
+NOALLOC_MAX_N = 2097152
 
 
 class KhSpk2Seg(C.Structure):
@@ -129,6 +133,8 @@ def load():
     L.kh_propagate_n_body.restype = I32
     L.kh_propagate_n_body_dev.argtypes = nb + [I32]
     L.kh_propagate_n_body_dev.restype = I32
+    L.kh_propagate_n_body_dev_noalloc.argtypes = nb + [I32]
+    L.kh_propagate_n_body_dev_noalloc.restype = I32
     L.kh_last_launch.argtypes = [V, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
     L.kh_last_launch.restype = I32
     L.kh_fov_rect_normals.argtypes = [P, D, D, D, P]

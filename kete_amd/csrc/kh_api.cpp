@@ -7,12 +7,29 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "kh_ctx.h"
 
 using namespace kh;
+
+// Scratch of kh_propagate_n_body_dev_noalloc: the longest-first order of at
+// most KH_NOALLOC_MAX_N objects (histogram, offsets, permutation, buckets).
+// A device array built into the library, so that entry point allocates
+// nothing; every GPU the library runs on holds its own copy until the process
+// ends.  Host code never passes this name to a kernel or copy: it uses the
+// device address hipGetSymbolAddress returns, taken once per device at
+// context creation (noalloc_scratch below).
+constexpr size_t kNoallocScratchBytes = order_buf_bytes(KH_NOALLOC_MAX_N);
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` per device
+}  // namespace
 
 static thread_local std::string g_err;
 
@@ -103,6 +120,21 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
   if (upload_radau_tables(t) != 0 || upload_grad_tables(t) != 0) {
     delete c;
     return fail(KH_E_HIP, "upload of Radau tables failed");
+  }
+  // device address of the no-alloc scratch on this device (current above)
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < kMaxDevices && !g_scratch_dev[device]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        delete c;
+        return fail(KH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+      }
+      g_scratch_dev[device] = p;
+    }
+    c->noalloc_scratch = device < kMaxDevices ? g_scratch_dev[device] : nullptr;
   }
   *out = c;
   return KH_OK;
@@ -681,7 +713,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
                          const kh_nongrav* ng, double epsilon,
                          int32_t mode, double* const out[6], int32_t* status, int64_t* impact_id,
                          double* impact_jd, kh_counters* counters, int32_t sync,
-                         const kh::MultiEpochs* me) {
+                         const kh::MultiEpochs* me, void* order_scratch = nullptr) {
   Chain ch;
   int32_t r = check_nbody_args(c, n, jd0, xyz, center, frame, jd_final, epsilon, mode, out, status,
                                ch);
@@ -756,7 +788,8 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     // turns it off
     const char* ord = getenv("KH_ORDER");
     if (!(ord && atoi(ord) == 0)) {
-      if (kh_grow(&c->obuf, &c->obuf_size, order_buf_bytes(n)) == KH_OK) p.order_buf = c->obuf;
+      if (order_scratch) p.order_buf = order_scratch;
+      else if (kh_grow(&c->obuf, &c->obuf_size, order_buf_bytes(n)) == KH_OK) p.order_buf = c->obuf;
       else return KH_E_HIP;
     }
   }
@@ -777,6 +810,26 @@ int32_t kh_propagate_n_body_dev(kh_ctx* c, int64_t n, const double* jd0,
                                 int32_t sync) {
   return nbody_dev(c, n, jd0, xyz, center, frame, jd_final, jd_final_stride, include_asteroids,
                    ng, epsilon, mode, out, status, impact_id, impact_jd, counters, sync, nullptr);
+}
+
+int32_t kh_propagate_n_body_dev_noalloc(kh_ctx* c, int64_t n, const double* jd0,
+                                        const double* const xyz[6], int64_t center,
+                                        int32_t frame, const double* jd_final,
+                                        int32_t jd_final_stride, int32_t include_asteroids,
+                                        const kh_nongrav* ng, double epsilon, int32_t mode,
+                                        double* const out[6], int32_t* status,
+                                        int64_t* impact_id, double* impact_jd,
+                                        kh_counters* counters, int32_t sync) {
+  if (n > KH_NOALLOC_MAX_N)
+    return fail(KH_E_ARG, "kh_propagate_n_body_dev_noalloc takes at most " +
+                              std::to_string(KH_NOALLOC_MAX_N) + " objects, got " +
+                              std::to_string(n));
+  if (!c) return fail(KH_E_ARG, "null context");
+  if (!c->noalloc_scratch)
+    return fail(KH_E_ARG, "no scratch on device " + std::to_string(c->device));
+  return nbody_dev(c, n, jd0, xyz, center, frame, jd_final, jd_final_stride, include_asteroids,
+                   ng, epsilon, mode, out, status, impact_id, impact_jd, counters, sync, nullptr,
+                   c->noalloc_scratch);
 }
 
 int32_t kh_propagate_n_body(kh_ctx* c, int64_t n, const double* jd0,

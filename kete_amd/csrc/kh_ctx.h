@@ -33,6 +33,7 @@ struct kh_ctx {
   size_t dbuf_size = 0;
   void* obuf = nullptr;  // queue-order scratch of the N-body launch
   size_t obuf_size = 0;
+  void* noalloc_scratch = nullptr;  // this device's built-in order scratch (no-alloc entry)
   void* fbuf = nullptr;  // FOV working memory
   size_t fbuf_size = 0;
   void* hbuf = nullptr;  // FOV hit ordering

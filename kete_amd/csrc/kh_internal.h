@@ -76,7 +76,7 @@ struct MultiEpochs {
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.
-inline size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
+constexpr size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket
 // the integrator kernel alone.
