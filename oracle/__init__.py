@@ -62,62 +62,90 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        D, P, I64, I32 = C.c_double, C.POINTER(C.c_double), C.c_int64, C.c_int32
-        PP = C.POINTER(P)
-        L.ko_const.restype = D
-        L.ko_const.argtypes = [C.c_char_p]
-        L.ko_newton_square_root_test.argtypes = [D, D, P]
-        L.ko_analytic_2_body.argtypes = [D, P, P, P, P]
-        L.ko_compute_eccentric_anomaly.argtypes = [D, D, D, P]
-        L.ko_elements_to_state.argtypes = [D, D, D, D, D, D, D, P, P]
-        L.ko_spk_load_file.argtypes = [C.c_char_p]
-        L.ko_spk_get_state.argtypes = [I64, D, I64, I32, P, P]
-        L.ko_chebyshev_evaluate_both.argtypes = [D, P, P, P, C.c_int, P, P]
-        L.ko_spk_accel.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, P, C.POINTER(I64)]
-        L.ko_spk_accel_ext.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, C.c_int, P,
-                                       C.POINTER(I64)]
-        L.ko_spk21_record_eval.argtypes = [P, I64, D, P, P]
-        L.ko_radau_central.argtypes = [D, D, P, P, P, P, C.POINTER(Counters)]
-        L.ko_batch_two_body.argtypes = [I64, P, PP, I64, I32, I64, P, P, PP,
-                                        C.POINTER(I32), C.c_int]
-        L.ko_batch_n_body.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.POINTER(NonGrav),
-                                      PP, C.POINTER(I32), C.POINTER(I64), P,
-                                      C.POINTER(Counters), C.c_int]
-        L.ko_batch_n_body_ext.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.c_int,
-                                          C.POINTER(NonGrav), PP, C.POINTER(I32),
-                                          C.POINTER(I64), P, C.POINTER(Counters), C.c_int]
-        L.ko_rect_normals.argtypes = [P, D, D, D, P]
-        L.ko_rect_contains.argtypes = [P, P, P]
-        L.ko_fov_check_visible.restype = I64
-        L.ko_fov_check_visible.argtypes = [P, D, P, P, I32, I64, P, PP, D,
-                                           C.POINTER(I64), PP, C.c_int]
-        L.ko_fov_state_check.restype = I64
-        L.ko_fov_state_check.argtypes = [I64, D, PP, I32, I64, P, P, P, D, I64,
-                                         C.POINTER(I64), C.POINTER(I64), P, C.c_int]
-        L.ko_fov_state_check_ext.restype = I64
-        L.ko_fov_state_check_ext.argtypes = [I64, D, PP, I32, I64, P, P, P, D, C.c_int, I64,
-                                             C.POINTER(I64), C.POINTER(I64), P, C.c_int]
-        L.ko_propagate_n_body_long.argtypes = [I64, D, PP, I32, D, P, C.POINTER(NonGrav), I64,
-                                               PP, P, C.c_int]
-        L.ko_fov_spk_check.restype = I64
-        L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
-                                       C.POINTER(I64), C.POINTER(I64), P]
-        L.ko_radau_one_body.argtypes = [I64, D, D, P, P, P, P]
-        L.ko_change_frame.argtypes = [C.POINTER(KoState), I32]
-        L.ko_change_center.argtypes = [C.POINTER(KoState), I64]
-        L.ko_comet_elements_peri.restype = None
-        L.ko_comet_elements_peri.argtypes = [D, P, P, P, P]
-        L.ko_moid_batch.restype = None
-        L.ko_moid_batch.argtypes = [I64, P, PP, I32, P, PP, P, P, C.POINTER(I32),
-                                    C.POINTER(I32), C.c_int]
-        L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
-        L.ko_hg_apparent_mag.restype = D
-        L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
-        L.ko_hg_phase_curve_correction.restype = D
-        L.ko_hg_phase_curve_correction.argtypes = [D, D]
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+class use_library:
+    """Context manager: route every oracle call through another build of the
+    oracle (a copy compiled with a planted mistake, for the tests that show
+    the pins catch it), then back to the default library."""
+
+    def __init__(self, path: str):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        lib()
+        self._saved = _lib
+        _lib = _configure(C.CDLL(self.path))
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+        return False
+
+
+def _configure(L):
+    D, P, I64, I32 = C.c_double, C.POINTER(C.c_double), C.c_int64, C.c_int32
+    PP = C.POINTER(P)
+    L.ko_const.restype = D
+    L.ko_const.argtypes = [C.c_char_p]
+    L.ko_newton_square_root_test.argtypes = [D, D, P]
+    L.ko_analytic_2_body.argtypes = [D, P, P, P, P]
+    L.ko_compute_eccentric_anomaly.argtypes = [D, D, D, P]
+    L.ko_elements_to_state.argtypes = [D, D, D, D, D, D, D, P, P]
+    L.ko_spk_load_file.argtypes = [C.c_char_p]
+    L.ko_spk_get_state.argtypes = [I64, D, I64, I32, P, P]
+    L.ko_chebyshev_evaluate_both.argtypes = [D, P, P, P, C.c_int, P, P]
+    L.ko_spk_accel.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, P, C.POINTER(I64)]
+    L.ko_spk_accel_ext.argtypes = [D, P, P, C.POINTER(NonGrav), C.c_int, C.c_int, P,
+                                   C.POINTER(I64)]
+    L.ko_spk21_record_eval.argtypes = [P, I64, D, P, P]
+    L.ko_radau_central.argtypes = [D, D, P, P, P, P, C.POINTER(Counters)]
+    L.ko_batch_two_body.argtypes = [I64, P, PP, I64, I32, I64, P, P, PP,
+                                    C.POINTER(I32), C.c_int]
+    L.ko_batch_n_body.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.POINTER(NonGrav),
+                                  PP, C.POINTER(I32), C.POINTER(I64), P,
+                                  C.POINTER(Counters), C.c_int]
+    L.ko_batch_n_body_ext.argtypes = [I64, P, PP, I64, I32, P, C.c_int, C.c_int,
+                                      C.POINTER(NonGrav), PP, C.POINTER(I32),
+                                      C.POINTER(I64), P, C.POINTER(Counters), C.c_int]
+    L.ko_rect_normals.argtypes = [P, D, D, D, P]
+    L.ko_rect_contains.argtypes = [P, P, P]
+    L.ko_fov_check_visible.restype = I64
+    L.ko_fov_check_visible.argtypes = [P, D, P, P, I32, I64, P, PP, D,
+                                       C.POINTER(I64), PP, C.c_int]
+    L.ko_fov_state_check.restype = I64
+    L.ko_fov_state_check.argtypes = [I64, D, PP, I32, I64, P, P, P, D, I64,
+                                     C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+    L.ko_fov_state_check_ext.restype = I64
+    L.ko_fov_state_check_ext.argtypes = [I64, D, PP, I32, I64, P, P, P, D, C.c_int, I64,
+                                         C.POINTER(I64), C.POINTER(I64), P, C.c_int]
+    L.ko_propagate_n_body_long.argtypes = [I64, D, PP, I32, D, P, C.POINTER(NonGrav), I64,
+                                           PP, P, C.c_int]
+    L.ko_fov_spk_check.restype = I64
+    L.ko_fov_spk_check.argtypes = [I64, C.POINTER(I64), I32, I64, P, P, P, I64,
+                                   C.POINTER(I64), C.POINTER(I64), P]
+    L.ko_radau_one_body.argtypes = [I64, D, D, P, P, P, P]
+    L.ko_change_frame.argtypes = [C.POINTER(KoState), I32]
+    L.ko_change_center.argtypes = [C.POINTER(KoState), I64]
+    L.ko_comet_elements_peri.restype = None
+    L.ko_comet_elements_peri.argtypes = [D, P, P, P, P]
+    L.ko_moid_batch.restype = None
+    L.ko_moid_batch.argtypes = [I64, P, PP, I32, P, PP, P, P, C.POINTER(I32),
+                                C.POINTER(I32), C.c_int]
+    L.ko_ra_dec_with_rates.argtypes = [P, P, P, P, P]
+    L.ko_hg_apparent_mag.restype = D
+    L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
+    L.ko_hg_phase_curve_correction.restype = D
+    L.ko_hg_phase_curve_correction.argtypes = [D, D]
+    L.ko_fov_trace_set.restype = None
+    L.ko_fov_trace_set.argtypes = [P, I64]
+    L.ko_fov_trace_len.restype = I64
+    L.ko_fov_trace_len.argtypes = []
+    return L
 
 
 def const(name: str) -> float:
@@ -292,7 +320,22 @@ def fov_check_visible(normals, obs_jd, obs_pos, obs_vel, jd, xyz, dt_limit=3.0,
 
 
 def fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit=3.0,
-                    frame=FRAME_ECLIPTIC, cap=None, n_threads=None, include_asteroids=False):
+                    frame=FRAME_ECLIPTIC, cap=None, n_threads=None, include_asteroids=False,
+                    trace=False):
+    """fov_checks_py -> (field index, object index, state[n_hits, 7]); with
+    trace=True also the re-propagation schedule as a list of (kind, epoch):
+    ('group', mean epoch), ('big', target), ('reset', epoch), ('small', target)."""
+    if trace:
+        buf = np.zeros(2 * 4096)
+        lib().ko_fov_trace_set(_d(buf), 4096)
+        try:
+            res = fov_state_check(jd0, xyz, field_jd, obs_xyz, normals, dt_limit, frame, cap,
+                                  n_threads, include_asteroids)
+            nt = lib().ko_fov_trace_len()
+        finally:
+            lib().ko_fov_trace_set(None, 0)
+        kinds = ("group", "big", "reset", "small")
+        return res + ([(kinds[int(buf[2 * k])], float(buf[2 * k + 1])) for k in range(nt)],)
     xyz = _soa(xyz)
     n = xyz.shape[1]
     field_jd = np.ascontiguousarray(field_jd, dtype=np.float64)

@@ -1741,6 +1741,30 @@ int64_t ko_fov_check_visible(const double normals[12], double obs_jd,
     return nv;
 }
 
+/* Test hook: the re-propagation schedule of the last ko_fov_state_check_ext
+ * call, as (kind, epoch) pairs in call order -- 0: a field group and its mean
+ * epoch, 1: a big step of the big-step copy to the epoch given, 2: the
+ * small-step states reset from the big-step copy, 3: a small step to the
+ * epoch given.  Off unless a buffer is set. */
+static double *g_fov_trace = NULL;
+static int64_t g_fov_trace_cap = 0, g_fov_trace_n = 0;
+
+void ko_fov_trace_set(double *buf, int64_t cap) {
+    g_fov_trace = buf;
+    g_fov_trace_cap = buf ? cap : 0;
+    g_fov_trace_n = 0;
+}
+
+int64_t ko_fov_trace_len(void) { return g_fov_trace_n; }
+
+static void fov_trace(int kind, double jd) {
+    if (g_fov_trace && g_fov_trace_n < g_fov_trace_cap) {
+        g_fov_trace[2 * g_fov_trace_n] = kind;
+        g_fov_trace[2 * g_fov_trace_n + 1] = jd;
+        g_fov_trace_n++;
+    }
+}
+
 /* fov_checks_py (src/kete/rust/fovs/checks.rs:24-110).  Objects that fail a
  * re-propagation are dropped, as filter_map does there; the surviving
  * objects keep their original index so hits can be matched to inputs. */
@@ -1817,18 +1841,22 @@ int64_t ko_fov_state_check_ext(int64_t n, double jd0, const double *const *xyz,
         int64_t f1 = f0 + 1;
         while (f1 < n_fields && !(fabs(field_jd[f1] - field_jd[f0]) >= 2.0 * dt_limit)) f1++;
         double jd_mean = (field_jd[f1 - 1] + field_jd[f0]) / 2.0;
+        fov_trace(0, jd_mean);
         if (fabs(jd_mean - big_jd) >= dt_limit * 50.0) {
             big_jd = jd_mean;
             nb_prop_ctx pc = {n, big, big_alive, jd, include_asteroids};
+            fov_trace(1, pc.jd);
             parallel_for(n, 1000, n_threads, prop_range, &pc);
         }
         if (fabs(jd_mean - jd) >= dt_limit) {
             if (fabs(jd - big_jd) >= dt_limit * 25.0) {
+                fov_trace(2, jd);
                 memcpy(states, big, sizeof(ko_state) * n);
                 memcpy(alive, big_alive, n);
             }
             jd = jd_mean;
             nb_prop_ctx pc = {n, states, alive, jd, include_asteroids};
+            fov_trace(3, pc.jd);
             parallel_for(n, 1000, n_threads, prop_range, &pc);
         }
         int64_t nf = f1 - f0;

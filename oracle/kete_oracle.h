@@ -235,6 +235,10 @@ double ko_hg_apparent_mag(double h, double g, const double sun2obj[3],
                           const double sun2obs[3]);
 double ko_hg_phase_curve_correction(double g, double phase);
 
+/* Test hook: record the FOV re-propagation schedule ((kind, epoch) pairs). */
+void ko_fov_trace_set(double *buf, int64_t cap);
+int64_t ko_fov_trace_len(void);
+
 #ifdef __cplusplus
 }
 #endif
