@@ -141,6 +141,11 @@ def _configure(L):
     L.ko_hg_apparent_mag.argtypes = [D, D, P, P]
     L.ko_hg_phase_curve_correction.restype = D
     L.ko_hg_phase_curve_correction.argtypes = [D, D]
+    L.ko_chain_observe_batch.restype = C.c_int
+    L.ko_chain_observe_batch.argtypes = [I64, P, PP, I32, C.POINTER(NonGrav), I32, P, P,
+                                         C.c_int, C.c_int, I64, P, C.POINTER(I32),
+                                         C.POINTER(I32), C.POINTER(I64), P, P, C.POINTER(I32),
+                                         C.c_int]
     L.ko_fov_trace_set.restype = None
     L.ko_fov_trace_set.argtypes = [P, I64]
     L.ko_fov_trace_len.restype = I64
@@ -456,3 +461,46 @@ def hg_apparent_mag(h, g, sun2obj, sun2obs):
 
 def hg_phase_curve_correction(g, phase):
     return lib().ko_hg_phase_curve_correction(g, phase)
+
+
+def chain_observe(jd0, xyz, jd_obs, obs, frame=FRAME_ECLIPTIC, nongravs=None, restart=True,
+                  schedule=None, record=False, cap=20000, n_threads=None):
+    """The orbit-fit observable chain for n objects (xyz [6, n], Sun-centred
+    in `frame` at jd0) at m epochs (jd_obs [m] or [m, n]; observer obs [3, m]
+    or [3, m, n], Sun-centred in `frame`): N-body arc to each epoch (restart:
+    a new integrator per epoch, else one integration through them), one
+    light-time two-body step, RA/Dec.  record=True also returns each
+    object's step schedule; schedule=<that> replays it (frozen steps).
+    -> dict(radec [2, m, n], state [6, m, n], status [n], schedule)."""
+    xyz = _soa(xyz)
+    n = xyz.shape[1]
+    jd_obs = np.asarray(jd_obs, dtype=np.float64)
+    m = jd_obs.shape[0]
+    jo = np.ascontiguousarray(np.broadcast_to(jd_obs.reshape(m, -1), (m, n)))
+    ob = np.asarray(obs, dtype=np.float64).reshape(3, m, -1)
+    ob = np.ascontiguousarray(np.broadcast_to(ob, (3, m, n)).transpose(1, 0, 2))  # [m, 3, n]
+    jd0 = np.ascontiguousarray(np.broadcast_to(jd0, (n,)), dtype=np.float64)
+    radec = np.empty((m, 2, n))
+    state = np.empty((m, 6, n))
+    status = np.empty(n, dtype=np.int32)
+    ng = (NonGrav * n)(*nongravs) if nongravs is not None else None
+    mode = 2 if schedule is not None else (1 if record else 0)
+    if schedule is not None:
+        sh, ssw, sacc, sn = schedule
+        cap = sh.shape[1]
+    elif record:
+        sh, ssw, sacc = np.empty((n, cap)), np.empty((n, cap), np.int32), np.empty((n, cap),
+                                                                                 np.int32)
+        sn = np.zeros(n, dtype=np.int64)
+    else:
+        sh = ssw = sacc = sn = None
+    i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+    lib().ko_chain_observe_batch(n, _d(jd0), _ptrs(list(xyz)), frame, ng, m, _d(jo), _d(ob),
+                                 int(bool(restart)), mode, cap,
+                                 _d(sh) if sh is not None else None, i32(ssw), i32(sacc),
+                                 sn.ctypes.data_as(C.POINTER(C.c_int64)) if sn is not None
+                                 else None, _d(radec), _d(state), i32(status),
+                                 n_threads or os.cpu_count())
+    return dict(radec=radec.transpose(1, 0, 2), state=state.transpose(1, 0, 2), status=status,
+                schedule=(sh, ssw, sacc, sn) if mode else None)
+

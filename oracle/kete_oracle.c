@@ -950,6 +950,37 @@ static void radau_tables(void) {
 typedef int (*force_fn)(double t, const double *x, const double *v, int exact,
                         double *a, void *ctx);
 
+/* Step schedule of an integration (test hook, no kete counterpart): mode 1
+ * records each step attempt (its step size, the sweeps it took, whether it
+ * was accepted); mode 2 replays a recording -- the same step sizes and sweep
+ * counts whatever the convergence test says -- so that the computed map is a
+ * smooth function of the initial state and finite differences of it
+ * converge to the derivative of the map with its step sequence held fixed. */
+typedef struct {
+    int mode;            /* 0 off, 1 record, 2 replay */
+    double *h;           /* [cap] step size of each attempt */
+    int32_t *sweeps;     /* [cap] sweeps of each attempt */
+    int32_t *acc;        /* [cap] 1: accepted */
+    int64_t n, cap, pos;
+    int bad;             /* record overflow, or replay past its end */
+} radau_sched;
+
+static void sched_note(radau_sched *S, double h, int sweeps, int acc) {
+    if (!S) return;
+    if (S->mode == 1) {
+        if (S->n < S->cap) {
+            S->h[S->n] = h;
+            S->sweeps[S->n] = sweeps;
+            S->acc[S->n] = acc;
+            S->n++;
+        } else {
+            S->bad = 1;
+        }
+    } else if (S->mode == 2) {
+        S->pos++;
+    }
+}
+
 typedef struct {
     force_fn f;
     void *ctx;
@@ -957,6 +988,7 @@ typedef struct {
     double x[3], v[3], a[3];
     double B[7][3];
     ko_counters *cnt;
+    radau_sched *sched;  /* NULL: kete's adaptive integrator */
 } radau_t;
 
 static inline double max3(const double *v) { return fmax(fmax(v[0], v[1]), v[2]); }
@@ -967,6 +999,14 @@ static int radau_step(radau_t *R, double h, double *next_h) {
     double xs[3], vs[3], ev[3];
     memset(G, 0, sizeof(G));
     memcpy(ev, R->a, 24);
+    radau_sched *S = R->sched;
+    const int replay = S && S->mode == 2;
+    int rsweeps = 10, racc = 0;
+    if (replay) {
+        if (S->pos >= S->n) { S->bad = 1; return KO_CONVERGENCE; }
+        rsweeps = S->sweeps[S->pos];
+        racc = S->acc[S->pos];
+    }
     for (int sweep = 0; sweep < 10; ++sweep) {
         if (R->cnt) R->cnt->n_sweep++;
         double bprev[3] = {R->B[6][0], R->B[6][1], R->B[6][2]};
@@ -1000,7 +1040,8 @@ static int radau_step(radau_t *R, double h, double *next_h) {
             fmx[d] = fabs(ev[d]) + 1e-6;
             ratio[d] = fabs(R->B[6][d] - bprev[d]) / fmx[d];
         }
-        if (max3(ratio) < 1e-14) {
+        if (replay ? (racc && sweep + 1 == rsweeps) : max3(ratio) < 1e-14) {
+            sched_note(S, h, sweep + 1, 1);
             for (int d = 0; d < 3; ++d) {
                 double bw = 0.0, bu = 0.0;
                 for (int k = 0; k < 7; ++k) bw += R->B[k][d] * W_VEC[k];
@@ -1020,6 +1061,7 @@ static int radau_step(radau_t *R, double h, double *next_h) {
             return KO_OK;
         }
     }
+    sched_note(S, h, 10, 0);
     return KO_CONVERGENCE;
 }
 
@@ -1189,6 +1231,172 @@ static void parallel_for(int64_t n, int64_t chunk, int n_threads, range_fn fn, v
     if (n_threads > 512) n_threads = 512;
     for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, pool_worker, &j);
     for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* The astrometric observable chain of an orbit fit (test infrastructure for */
+/* the partials kernel; kete has no derivatives): an N-body arc to each       */
+/* observation epoch, one two-body step back by the light time, RA/Dec of     */
+/* the observer-to-object vector -- the chain of FovLike::check_n_body        */
+/* (fov/fov_like.rs:89-110, simult_states.rs:86-120).                        */
+/*  restart = 1: a new integrator at each epoch (RadauIntegrator::new and     */
+/*    integrate, radau.rs:119-218: B = 0, an exact evaluation, h = 0.1 d),   */
+/*    what successive propagate_n_body calls do, kept in barycentric          */
+/*    equatorial coordinates between arcs;                                    */
+/*  restart = 0: one integration through all epochs: the step that would      */
+/*    pass an epoch is shortened to land on it, B carries on, and the next    */
+/*    arc starts from the step size proposed before the shortening.           */
+/* With a schedule (mode 1 or 2) the step sequence is recorded or replayed.  */
+/* ------------------------------------------------------------------------ */
+static int chain_observe(double jd0, const double *x6, int32_t frame, const ko_nongrav *ng,
+                         int32_t m, const double *jd_obs, const double *obs, int64_t ostride,
+                         int restart, radau_sched *S, double *radec, double *state,
+                         int64_t sstride) {
+    pthread_once(&radau_once, radau_tables);
+    ko_state s0 = {jd0, {x6[0], x6[1], x6[2]}, {x6[3], x6[4], x6[5]}, frame, 10};
+    int finite = isfinite(jd0);
+    for (int k = 0; k < 6; ++k) finite &= isfinite(x6[k]);
+    if (!finite) return KO_VALUE_ERROR;
+    int rc = ko_change_center(&s0, 0);
+    if (!rc) rc = ko_change_frame(&s0, KO_FRAME_EQUATORIAL);
+    if (rc) return rc;
+    spk_ctx c = {ng, 0, 0, 0.0};
+    radau_t R;
+    memset(&R, 0, sizeof(R));
+    R.f = spk_force; R.ctx = &c; R.t = jd0; R.sched = S;
+    memcpy(R.x, s0.pos, 24);
+    memcpy(R.v, s0.vel, 24);
+    rc = spk_force(R.t, R.x, R.v, 1, R.a, &c);
+    if (rc) return rc;
+    const int replay = S && S->mode == 2;
+    double h = 0.0, h_prop = 0.0;
+    int clamped = 0;
+    for (int32_t j = 0; j < m; ++j) {
+        const double tf = jd_obs[j * ostride];
+        int steps = 1;
+        if (j > 0 && restart) {
+            memset(R.B, 0, sizeof(R.B));
+            rc = spk_force(R.t, R.x, R.v, 1, R.a, &c);
+            if (rc) return rc;
+        }
+        if (j == 0 || restart) {
+            if (fabs(tf - R.t) < 1e-10) steps = 0;
+            else h = copysign(0.1, tf - R.t);
+        } else if (fabs(tf - R.t) < 1e-10) {
+            rc = spk_force(R.t, R.x, R.v, 1, R.a, &c);   /* the same epoch again */
+            if (rc) return rc;
+            steps = 0;
+        } else {
+            h = copysign(fmax(fabs(clamped ? h_prop : h), MIN_STEP), tf - R.t);
+        }
+        int fails = 0;
+        while (steps) {
+            if (replay) {
+                if (S->pos >= S->n) { S->bad = 1; return KO_CONVERGENCE; }
+                h = S->h[S->pos];
+            }
+            h_prop = h;
+            clamped = fabs(R.t - tf) <= fabs(h);
+            if (clamped && !replay) h = tf - R.t;
+            double nh;
+            rc = radau_step(&R, h, &nh);
+            if (rc == KO_OK) {
+                h = nh;
+                if (fabs(R.t - tf) < 1e-12) break;
+                fails = 0;
+            } else if (rc == KO_IMPACT || rc == KO_DAF_LIMITS || (S && S->bad)) {
+                return rc;
+            } else {
+                fails++;
+                h *= 0.7;
+                if (fails > 10) return KO_CONVERGENCE;
+            }
+            if (fabs(h) < MIN_STEP) h = copysign(MIN_STEP, h);
+        }
+        /* observables at tf: Sun-centred state, light time, RA/Dec */
+        double sp[3], sv[3], xh[3], vh[3], ob[3];
+        rc = ko_spk_get_state(10, tf, 0, KO_FRAME_EQUATORIAL, sp, sv);
+        if (rc) return rc;
+        for (int k = 0; k < 3; ++k) {
+            xh[k] = R.x[k] - sp[k];
+            vh[k] = R.v[k] - sv[k];
+            ob[k] = obs[(3 * j + k) * ostride];
+        }
+        if (frame == KO_FRAME_ECLIPTIC) {
+            ko_state o = {tf, {ob[0], ob[1], ob[2]}, {0, 0, 0}, KO_FRAME_ECLIPTIC, 10};
+            ko_change_frame(&o, KO_FRAME_EQUATORIAL);
+            memcpy(ob, o.pos, 24);
+        }
+        double d[3] = {xh[0] - ob[0], xh[1] - ob[1], xh[2] - ob[2]};
+        double po[3], vo[3], rd[4], zero[3] = {0, 0, 0};
+        rc = ko_analytic_2_body(-(norm3(d) * C_AU_PER_DAY_INV), xh, vh, po, vo);
+        if (rc) return rc;
+        ko_ra_dec_with_rates(po, vo, ob, zero, rd);
+        radec[(2 * j) * ostride] = rd[0];
+        radec[(2 * j + 1) * ostride] = rd[1];
+        if (state) {
+            ko_state o = {tf, {xh[0], xh[1], xh[2]}, {vh[0], vh[1], vh[2]}, KO_FRAME_EQUATORIAL, 10};
+            ko_change_frame(&o, frame);
+            for (int k = 0; k < 3; ++k) {
+                state[(6 * j + k) * sstride] = o.pos[k];
+                state[(6 * j + 3 + k) * sstride] = o.vel[k];
+            }
+        }
+    }
+    return KO_OK;
+}
+
+typedef struct {
+    int64_t n;
+    const double *jd0;
+    const double *const *xyz;
+    int32_t frame;
+    const ko_nongrav *ng;
+    int32_t m;
+    const double *jd_obs, *obs;
+    int restart, mode;
+    int64_t cap;
+    double *sh;
+    int32_t *ssw, *sacc;
+    int64_t *sn;
+    double *radec, *state;
+    int32_t *status;
+} chain_ctx;
+
+static void chain_range(int64_t lo, int64_t hi, void *vctx) {
+    chain_ctx *c = vctx;
+    for (int64_t i = lo; i < hi; ++i) {
+        double x6[6];
+        for (int k = 0; k < 6; ++k) x6[k] = c->xyz[k][i];
+        radau_sched S = {c->mode, c->sh ? c->sh + i * c->cap : NULL,
+                         c->ssw ? c->ssw + i * c->cap : NULL,
+                         c->sacc ? c->sacc + i * c->cap : NULL,
+                         c->mode == 2 ? c->sn[i] : 0, c->cap, 0, 0};
+        for (int32_t j = 0; j < c->m; ++j) {
+            c->radec[(2 * j) * c->n + i] = NAN;
+            c->radec[(2 * j + 1) * c->n + i] = NAN;
+            for (int k = 0; c->state && k < 6; ++k) c->state[(6 * j + k) * c->n + i] = NAN;
+        }
+        int rc = chain_observe(c->jd0[i], x6, c->frame, c->ng ? &c->ng[i] : NULL, c->m,
+                               c->jd_obs + i, c->obs + i, c->n, c->restart,
+                               c->mode ? &S : NULL, c->radec + i, c->state ? c->state + i : NULL,
+                               c->n);
+        if (!rc && S.bad) rc = KO_VALUE_ERROR;
+        c->status[i] = rc;
+        if (c->mode == 1) c->sn[i] = S.n;
+    }
+}
+
+int ko_chain_observe_batch(int64_t n, const double *jd0, const double *const *xyz,
+                           int32_t frame, const ko_nongrav *ng, int32_t m,
+                           const double *jd_obs, const double *obs, int restart, int mode,
+                           int64_t cap, double *sched_h, int32_t *sched_sweeps,
+                           int32_t *sched_acc, int64_t *sched_n, double *radec, double *state,
+                           int32_t *status, int n_threads) {
+    chain_ctx c = {n, jd0, xyz, frame, ng, m, jd_obs, obs, restart, mode, cap, sched_h,
+                   sched_sweeps, sched_acc, sched_n, radec, state, status};
+    parallel_for(n, 8, n_threads, chain_range, &c);
+    return KO_OK;
 }
 
 /* ---- propagation_kepler_py (src/kete/rust/kepler.rs:68-108) ------------ */

@@ -235,6 +235,18 @@ double ko_hg_apparent_mag(double h, double g, const double sun2obj[3],
                           const double sun2obs[3]);
 double ko_hg_phase_curve_correction(double g, double phase);
 
+/* The orbit-fit observable chain (N-body arc to each epoch, light-time step,
+ * RA/Dec) for n objects at m epochs; jd_obs [m][n], obs [3*m][n] (observer,
+ * Sun-centred, frame), radec [2*m][n], state [6*m][n] (nullable).  mode 0:
+ * kete's adaptive steps; 1: record each object's step attempts into
+ * sched_* [n][cap] (sched_n [n] receives the counts); 2: replay them. */
+int ko_chain_observe_batch(int64_t n, const double *jd0, const double *const *xyz,
+                           int32_t frame, const ko_nongrav *ng, int32_t m,
+                           const double *jd_obs, const double *obs, int restart, int mode,
+                           int64_t cap, double *sched_h, int32_t *sched_sweeps,
+                           int32_t *sched_acc, int64_t *sched_n, double *radec, double *state,
+                           int32_t *status, int n_threads);
+
 /* Test hook: record the FOV re-propagation schedule ((kind, epoch) pairs). */
 void ko_fov_trace_set(double *buf, int64_t cap);
 int64_t ko_fov_trace_len(void);
