@@ -153,6 +153,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->hbuf) (void)hipFree(c->hbuf);
   if (c->mbuf) (void)hipFree(c->mbuf);
   if (c->lbuf) (void)hipFree(c->lbuf);
+  if (c->pbuf) (void)hipFree(c->pbuf);
   if (c->d_counts) (void)hipFree(c->d_counts);
   if (c->d_range) (void)hipFree(c->d_range);
   if (c->d_queue) (void)hipFree(c->d_queue);

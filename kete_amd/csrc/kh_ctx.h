@@ -42,6 +42,8 @@ struct kh_ctx {
   size_t mbuf_size = 0;
   void* lbuf = nullptr;  // propagate_n_body_long: planets at jd0
   size_t lbuf_size = 0;
+  void* pbuf = nullptr;  // partials: states at the epochs
+  size_t pbuf_size = 0;
   unsigned long long* d_queue = nullptr;
   unsigned long long* d_range = nullptr;  // epoch range of the last N-body batch [2]
   unsigned long long* d_counts = nullptr;  // FOV list counters [4]

@@ -708,6 +708,21 @@ __device__ __forceinline__ int32_t analytic_2_body(double time, Vec3 pos, Vec3 v
   return ST_OK;
 }
 
+// y = x^(1/7) for the step-size ratio (radau.rs:339-342), by two Newton steps
+// from a single-precision seed; inputs outside [3e-5, 3e4] only matter through
+// the clamp to [0.25, 4] that follows, so they are clamped first.
+__device__ __forceinline__ double root7(double x) {
+  if (isnan(x)) return x;
+  const double xc = fmin(fmax(x, 3e-5), 3e4);
+  double y = (double)exp2f(log2f((float)xc) * (1.0f / 7.0f));
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double y2 = y * y, y3 = y2 * y, y6 = y3 * y3, y7 = y6 * y;
+    y = y - (y7 - xc) / (7.0 * y6);
+  }
+  return y;
+}
+
 // ---------------------------------------------------------------------------
 // Force model: spk_accel with PLANETS (acceleration.rs:121-168),
 // GravParams::add_acceleration (gravity.rs:171-253), JplComet non-grav

@@ -27,8 +27,12 @@
 //  - Light-time step: the value uses the universal Kepler solver; its
 //    derivative uses the two-body state transition matrix expanded to third
 //    order in the step (|dt| = light time < 0.06 d: relative error < 1e-9).
+#include <cstdlib>
+#include <cstring>
+
 #include "kh_device.h"
 #include "kh_internal.h"
+#include "kh_jac.h"
 
 namespace kh {
 
@@ -227,6 +231,11 @@ struct GradArgs {
   double* dstate[6];        // [9][m][n] or null
   int32_t* status;          // [n]
   int32_t* counters;        // [n][4] or null
+  // wide kernel: the states at each epoch, for the observation kernel
+  double* ep_state;         // [m][n][10][6]: columns 0..8 (derivatives), 9 (nominal)
+  double* ep_sun;           // [m][n][6]: the Sun's barycentric state at the epoch
+  int32_t* ep_done;         // [n]: epochs completed
+  int32_t* lt_fail;         // [n]: first epoch whose light-time step failed (m: none)
 };
 
 enum : int32_t { GP_INIT = 0, GP_SUB = 1, GP_EXACT = 2 };
@@ -592,6 +601,732 @@ void radau_partials_kernel(const SpkDev* __restrict__ spk, const double* __restr
   }
 }
 
+// ===========================================================================
+// Wide-lane partials kernel: sixteen lanes per object (one DPP row), four
+// objects per wavefront.  Lane l of an object's group
+//  - carries one column of the integrator state: l = 0..8 the derivative of
+//    the state along parameter l (x, y, z, vx, vy, vz, A1, A2, A3), l = 9 the
+//    nominal orbit, l = 10..15 a zero column;
+//  - evaluates one body of the force model: l = 0..9 kete's PLANETS in order
+//    (gravity.rs:145-148), l = 10 the Earth-Moon barycentre segment, which
+//    lanes 3 (Earth) and 4 (Moon) add to their positions: the body's Newton,
+//    GR and J2 terms and their Jacobians with respect to the object's
+//    position and velocity (kh_jac.h).
+// Per force evaluation the group sums the acceleration and the Jacobians
+// over its lanes (DPP row reductions); the nominal lane adds the non-grav
+// term, and each tangent lane forms its column's acceleration
+// J_r dx + J_v dv plus the directional derivative of the non-grav term
+// (dual numbers along its own column).  The nominal lane's predicted state,
+// the Sun's state and the integrator's decisions reach the rest of the group
+// by lane shuffles.  A lane thus does the work of one body and one column,
+// where the one-lane-per-parameter kernel above evaluates the whole force
+// model and the nominal orbit in every lane.  The arithmetic of each column
+// is the dual-number kernel's; only the order in which body terms are summed
+// differs (a row reduction instead of kete's sequential sum).
+// ===========================================================================
+constexpr int WL = 16;
+constexpr int WL_NOMINAL = 9;
+constexpr int WL_EMB = 10;
+
+// v from another lane of the same DPP row (a 64-bit move is two 32-bit ones)
+// JplComet non-grav acceleration (nongrav.rs:114-157) a and its derivative
+// da along (dp, dv, dA) at position p, velocity v relative to the Sun:
+// a = g(r) (A1 r^ + A2 t^ + A3 n^), g = alpha (r/r_0)^-m (1 + (r/r_0)^n)^-k.
+// Norms come from reciprocal square roots and kete's divisions by them
+// become multiplications (last-bit differences); with kete's default
+// exponents (m = 2, k = 0) g = alpha (r/r_0)^-2 needs no pow.
+__device__ __forceinline__ void nongrav_dir(const double p[3], const double dp[3],
+                                            const double v[3], const double dv[3],
+                                            const double A[3], const double dA[3], double alpha,
+                                            double r_0, double m, double ne, double k,
+                                            double a[3], double da[3]) {
+  auto dot3 = [](const double* x, const double* y) {
+    return x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+  };
+  const double ip = rsqrt_nr(dot3(p, p));
+  const double n = dot3(p, p) * ip;
+  const double dn = dot3(p, dp) * ip;
+  double pn[3], dpn[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pn[c] = p[c] * ip;
+    dpn[c] = (dp[c] - pn[c] * dn) * ip;
+  }
+  const double vd = dot3(v, pn);
+  const double dvd = dot3(dv, pn) + dot3(v, dpn);
+  double tv[3], dtv[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    tv[c] = v[c] - pn[c] * vd;
+    dtv[c] = dv[c] - dpn[c] * vd - pn[c] * dvd;
+  }
+  const double it = rsqrt_nr(dot3(tv, tv));
+  const double dtn = dot3(tv, dtv) * it;
+  double tu[3], dtu[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    tu[c] = tv[c] * it;
+    dtu[c] = (dtv[c] - tu[c] * dtn) * it;
+  }
+  const double nv[3] = {tu[1] * pn[2] - tu[2] * pn[1], tu[2] * pn[0] - tu[0] * pn[2],
+                        tu[0] * pn[1] - tu[1] * pn[0]};
+  const double dnv[3] = {dtu[1] * pn[2] - dtu[2] * pn[1] + tu[1] * dpn[2] - tu[2] * dpn[1],
+                         dtu[2] * pn[0] - dtu[0] * pn[2] + tu[2] * dpn[0] - tu[0] * dpn[2],
+                         dtu[0] * pn[1] - dtu[1] * pn[0] + tu[0] * dpn[1] - tu[1] * dpn[0]};
+  const double in_ = rsqrt_nr(dot3(nv, nv));
+  const double dnn = dot3(nv, dnv) * in_;
+  double nu[3], dnu[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    nu[c] = nv[c] * in_;
+    dnu[c] = (dnv[c] - nu[c] * dnn) * in_;
+  }
+  const double rr0 = n / r_0, drr0 = dn / r_0;
+  double g, dg;
+  if (m == 2.0 && k == 0.0) {
+    const double ir = 1.0 / rr0;
+    g = alpha * (ir * ir);
+    dg = -2.0 * g * drr0 * ir;
+  } else {
+    const double q = pow(rr0, ne);
+    g = alpha * pow(rr0, -m) * pow(1.0 + q, -k);
+    dg = g * (-m * drr0 / rr0 - k * ne * q * drr0 / (rr0 * (1.0 + q)));
+  }
+  double s[3], ds[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    s[c] = g * A[c];
+    ds[c] = dg * A[c] + g * dA[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    a[c] = pn[c] * s[0] + tu[c] * s[1] + nu[c] * s[2];
+    da[c] = dpn[c] * s[0] + pn[c] * ds[0] + dtu[c] * s[1] + tu[c] * ds[1] + dnu[c] * s[2] +
+            nu[c] * ds[2];
+  }
+}
+
+// Chebyshev position and derivative (interpolation.rs:25-77, forward T/U
+// recurrences in ascending order, as cheb_both_n) of a 14-coefficient record
+// in LDS, [mid, radius, x[14], y[14], z[14]]; the coefficients are read in
+// chunks of four orders so that few of them are held in registers at once.
+__device__ __forceinline__ void cheb_both_lds(const double* rec, double x, Vec3& p, Vec3& d) {
+  const double* c = rec + 2;
+  const double x2 = 2.0 * x;
+  double vx = c[0] + c[1] * x, vy = c[14] + c[15] * x, vz = c[28] + c[29] * x;
+  double dx = c[1], dy = c[15], dz = c[29];
+  double t2 = 1.0, t1 = x, u2 = 1.0, u1 = x2;
+#pragma unroll
+  for (int k0 = 2; k0 < 14; k0 += 4) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = k0; k < k0 + 4 && k < 14; ++k) {
+      const double t = x2 * t1 - t2;
+      vx += c[k] * t;
+      vy += c[14 + k] * t;
+      vz += c[28 + k] * t;
+      t2 = t1;
+      t1 = t;
+      const double u = x2 * u1 - u2;
+      const double kk = (double)k;
+      dx += c[k] * u1 * kk;
+      dy += c[14 + k] * u1 * kk;
+      dz += c[28 + k] * u1 * kk;
+      u2 = u1;
+      u1 = u;
+    }
+  }
+  p = Vec3{vx, vy, vz};
+  d = Vec3{dx, dy, dz};
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi2, lo2);
+}
+
+// v of lane N of this lane's DPP row (row_newbcast)
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+  return dpp_row<0x150 + N>(v);
+}
+template <int N>
+__device__ __forceinline__ int row_bcast_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + N, 0xF, 0xF, false);
+}
+
+// Sum over the 16 lanes of a row, the same bits in every lane (each step
+// adds a commuted pair): quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror.
+__device__ __forceinline__ double row_sum(double v) {
+  v += dpp_row<0xB1>(v);
+  v += dpp_row<0x4E>(v);
+  v += dpp_row<0x141>(v);
+  v += dpp_row<0x140>(v);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void radau_partials_wide_kernel(
+    const SpkDev* __restrict__ spk, const double* __restrict__ coef, GradArgs p) {
+  // per-substep factors, indexed by each group's own substep
+  __shared__ double s_h[8], s_inv_h[8], s_rdiff[8][8];
+  // the Chebyshev record each segment lane last used, per object of the block
+  __shared__ __align__(16) double s_rec[256 / WL][WL_EMB + 1][44];
+  for (int q = threadIdx.x; q < 64; q += blockDim.x) {
+    s_rdiff[q / 8][q % 8] = RTG.rdiff[q / 8][q % 8];
+    if (q < 8) {
+      s_h[q] = RTG.h[q];
+      s_inv_h[q] = RTG.inv_h[q];
+    }
+  }
+  __syncthreads();
+  const int64_t gl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t o = gl / WL;
+  if (o >= p.n) return;
+  const int lane = threadIdx.x & 63;
+  const int l = lane & (WL - 1);
+  const int g0 = lane & ~(WL - 1);
+  const int64_t n = p.n;
+  const int32_t m = p.m;
+  const double nan = __builtin_nan("");
+  const bool ecl = p.frame == 17;
+  const bool central = p.force == 1;
+  const bool nominal = l == WL_NOMINAL;
+  const bool tangent = l < 9;
+
+  // JplComet parameters (new_asteroid defaults and A = 0 for kind 0)
+  double a1 = 0.0, a2 = 0.0, a3 = 0.0, alpha = 1.0, r_0 = 1.0, mexp = 2.0, nexp = 1.0,
+         kexp = 0.0;
+  if (p.ng != nullptr && p.ng[o].kind == 1) {
+    const NonGravDev& g = p.ng[o];
+    a1 = g.a1;
+    a2 = g.a2;
+    a3 = g.a3;
+    alpha = g.alpha;
+    r_0 = g.r_0;
+    mexp = g.m;
+    nexp = g.n;
+    kexp = g.k;
+  }
+  const double ngA[3] = {a1, a2, a3};
+  const double ngdA[3] = {l == 6 ? 1.0 : 0.0, l == 7 ? 1.0 : 0.0, l == 8 ? 1.0 : 0.0};
+
+  // nominal initial state, barycentric equatorial (every lane)
+  int32_t st = ST_OK;
+  double t = p.jd0[o];
+  Vec3 x0{p.xyz[0][o], p.xyz[1][o], p.xyz[2][o]};
+  Vec3 v0{p.xyz[3][o], p.xyz[4][o], p.xyz[5][o]};
+  if (!(isfinite(t) && isfinite(x0.x) && isfinite(x0.y) && isfinite(x0.z) && isfinite(v0.x) &&
+        isfinite(v0.y) && isfinite(v0.z)))
+    st = ST_VALUE_ERROR;
+  if (st == ST_OK && !central) {
+    Vec3 sp, sv;
+    st = chain_state(spk, coef, p.sun, t, sp, sv);
+    if (ecl) {
+      sp = equ_to_ecl(sp);
+      sv = equ_to_ecl(sv);
+    }
+    x0 = Vec3{x0.x + sp.x, x0.y + sp.y, x0.z + sp.z};
+    v0 = Vec3{v0.x + sv.x, v0.y + sv.y, v0.z + sv.z};
+  }
+  if (ecl) {
+    x0 = ecl_to_equ(x0);
+    v0 = ecl_to_equ(v0);
+  }
+  // this lane's column: the nominal state, or the seed of parameter l
+  Vec3 sx{l == 0 ? 1.0 : 0.0, l == 1 ? 1.0 : 0.0, l == 2 ? 1.0 : 0.0};
+  Vec3 sv{l == 3 ? 1.0 : 0.0, l == 4 ? 1.0 : 0.0, l == 5 ? 1.0 : 0.0};
+  if (ecl) {
+    sx = ecl_to_equ(sx);
+    sv = ecl_to_equ(sv);
+  }
+  double X[3] = {nominal ? x0.x : sx.x, nominal ? x0.y : sx.y, nominal ? x0.z : sx.z};
+  double V[3] = {nominal ? v0.x : sv.x, nominal ? v0.y : sv.y, nominal ? v0.z : sv.z};
+  double A[3] = {0.0, 0.0, 0.0};
+  double B[7][3], G[7][3];
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[i][c] = G[i][c] = 0.0;
+
+  // this lane's body (lanes 0..9) or segment (lane 10); lanes 11..15 none
+  const bool is_body = l < N_PLANETS && (!central || l == 0);
+  const bool is_seg = !central && l <= WL_EMB;
+  const BodyDev bd = spk->body[l < N_PLANETS ? l : 0];
+  const int32_t seg = l == WL_EMB ? spk->emb_seg : bd.seg;
+  const SegDev sg = spk->seg[seg];
+  const bool all14 = spk->all14 != 0;
+  const bool cache_rec = all14 && sg.rsize == 44;
+  int32_t rec_cached = -1;
+  const double mass = central ? GMS : bd.mass;
+  const int gr = !central && bd.kind == 1;
+  const double j2 = central ? 0.0 : bd.j2;
+  const int ecl_j2 = bd.kind == 1;
+  // the GR bodies of kete's PLANETS, the Sun (lane 0) and Jupiter (lane 6):
+  // the tangent lanes differentiate their GR terms along their own column
+  constexpr int JUP = 6;
+  const double m_sun = spk->body[0].mass, m_jup = spk->body[JUP].mass;
+
+  double h = 0.0, next_h = 0.0, tf = 0.0, h_prop = 0.0;
+  bool clamped = false;
+  double bprev[3] = {0.0, 0.0, 0.0};
+  int32_t phase = GP_INIT, sub = 1, sweep = 0, fails = 0, j = 0, impact = 0;
+  int4 cnt{0, 0, 0, 0};
+  if (st == ST_OK && m > 0) tf = p.jd_obs[o];
+  Vec3 sunp{0.0, 0.0, 0.0}, sunv{0.0, 0.0, 0.0};
+
+  while (st == ST_OK && j < m) {
+    // predicted column state at the evaluation point
+    double te = t;
+    double P[3] = {X[0], X[1], X[2]}, Q[3] = {V[0], V[1], V[2]};
+    const bool exact = phase != GP_SUB;
+    if (phase == GP_SUB) {
+      const double s = s_h[sub];
+      te = t + s * h;
+      const double sh = s * h, s2h2 = (s * s) * (h * h);
+      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double sp = 1.0;
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        sp *= s;
+        const double wq = sp * RTC.w[q], uq = sp * RTC.u[q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bw[c] += B[q][c] * wq;
+          bu[c] += B[q][c] * uq;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        P[c] = X[c] + V[c] * sh + (A[c] / 2.0 + bw[c]) * s2h2;
+        Q[c] = V[c] + (A[c] + bu[c]) * sh;
+      }
+    }
+    const Vec3 xe{row_bcast<WL_NOMINAL>(P[0]), row_bcast<WL_NOMINAL>(P[1]),
+                  row_bcast<WL_NOMINAL>(P[2])};
+    const Vec3 ve{row_bcast<WL_NOMINAL>(Q[0]), row_bcast<WL_NOMINAL>(Q[1]),
+                  row_bcast<WL_NOMINAL>(Q[2])};
+
+    // this lane's segment at te, its record cached in LDS
+    int32_t lst = ST_OK;
+    Vec3 bp{0.0, 0.0, 0.0}, bv{0.0, 0.0, 0.0};
+    if (is_seg) {
+      if (te < sg.jd_start || te > sg.jd_end) lst = ST_DAF_LIMITS;
+      const double js = (te - 2451545.0) * 86400.0;
+      const int32_t ri = record_index(sg, js);
+      const double* __restrict__ grec = coef + sg.off + (int64_t)ri * sg.rsize;
+      double ir, xs;
+      Vec3 pp, dd;
+      if (cache_rec) {
+        double* slot = &s_rec[threadIdx.x / WL][l][0];
+        if (ri != rec_cached) {
+          const double2* __restrict__ src = reinterpret_cast<const double2*>(grec);
+#pragma unroll
+          for (int q = 0; q < 22; ++q) reinterpret_cast<double2*>(slot)[q] = src[q];
+          rec_cached = ri;
+        }
+        ir = 1.0 / slot[1];
+        xs = (js - slot[0]) * ir;
+        cheb_both_lds(slot, xs, pp, dd);
+      } else {
+        ir = 1.0 / grec[1];
+        xs = (js - grec[0]) * ir;
+        if (all14) cheb_both_n<14>(grec + 2, xs, pp, dd);
+        else cheb_both_dyn(grec + 2, sg.n_coef, xs, pp, dd);
+      }
+      const double vsc = 86400.0 * ir * INV_AU_KM;
+      bp = Vec3{pp.x * INV_AU_KM, pp.y * INV_AU_KM, pp.z * INV_AU_KM};
+      bv = Vec3{dd.x * vsc, dd.y * vsc, dd.z * vsc};
+    }
+    Vec3 jupp{0.0, 0.0, 0.0}, jupv{0.0, 0.0, 0.0};
+    if (!central) {
+      const Vec3 emb{row_bcast<WL_EMB>(bp.x), row_bcast<WL_EMB>(bp.y),
+                     row_bcast<WL_EMB>(bp.z)};
+      if (l < N_PLANETS && bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+      sunp = Vec3{row_bcast<0>(bp.x), row_bcast<0>(bp.y), row_bcast<0>(bp.z)};
+      sunv = Vec3{row_bcast<0>(bv.x), row_bcast<0>(bv.y), row_bcast<0>(bv.z)};
+      jupp = Vec3{row_bcast<JUP>(bp.x), row_bcast<JUP>(bp.y), row_bcast<JUP>(bp.z)};
+      jupv = Vec3{row_bcast<JUP>(bv.x), row_bcast<JUP>(bv.y), row_bcast<JUP>(bv.z)};
+    }
+    // this lane's body: acceleration and the symmetric Jacobian (Newton, J2)
+    double ba[3] = {0.0, 0.0, 0.0}, bj[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (is_body) {
+      const double rp[3] = {xe.x - bp.x, xe.y - bp.y, xe.z - bp.z};
+      const double rv[3] = {ve.x - bv.x, ve.y - bv.y, ve.z - bv.z};
+      const double r2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
+      const double inv_r = rsqrt_nr(r2);
+      if (exact && !central && r2 * inv_r <= bd.radius && lst == ST_OK) lst = ST_IMPACT;
+      body_terms_sym(mass, bd.radius, j2, gr, ecl_j2, ROT_C, ROT_S, C_AU_PER_DAY_INV_SQUARED,
+                     rp, rv, inv_r, ba, bj);
+    }
+    // the first failing body of the group, in kete's order
+    const uint64_t bad = __ballot(lst != ST_OK);
+    const uint32_t gbad = (uint32_t)((bad >> g0) & 0xFFFFull);
+    cnt.x++;
+    if (gbad) {
+      const int first = __builtin_ctz(gbad);
+      st = __shfl(lst, g0 + first, 64);
+      if (st == ST_IMPACT) impact = spk->body[first < N_PLANETS ? first : 0].id;
+      break;
+    }
+    // group sums; a tangent column's acceleration J dx is accumulated as the
+    // sums arrive
+    double ev[3];
+    {
+      double js[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) js[q] = row_sum(bj[q]);
+      const double as0 = row_sum(ba[0]), as1 = row_sum(ba[1]), as2 = row_sum(ba[2]);
+      ev[0] = nominal ? as0 : js[0] * P[0] + js[1] * P[1] + js[2] * P[2];
+      ev[1] = nominal ? as1 : js[1] * P[0] + js[3] * P[1] + js[4] * P[2];
+      ev[2] = nominal ? as2 : js[2] * P[0] + js[4] * P[1] + js[5] * P[2];
+    }
+    if (!central) {
+      // derivatives of the GR terms of the Sun and Jupiter along this column
+      const double dp[3] = {nominal ? 0.0 : P[0], nominal ? 0.0 : P[1], nominal ? 0.0 : P[2]};
+      const double dq[3] = {nominal ? 0.0 : Q[0], nominal ? 0.0 : Q[1], nominal ? 0.0 : Q[2]};
+      const Vec3 gb[2][2] = {{sunp, sunv}, {jupp, jupv}};
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const double rp[3] = {xe.x - gb[b][0].x, xe.y - gb[b][0].y, xe.z - gb[b][0].z};
+        const double rv[3] = {ve.x - gb[b][1].x, ve.y - gb[b][1].y, ve.z - gb[b][1].z};
+        double dg[3];
+        gr_dir(b == 0 ? m_sun : m_jup, C_AU_PER_DAY_INV_SQUARED, rp, rv, dp, dq,
+               rsqrt_nr(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]), dg);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ev[c] += dg[c];
+      }
+    }
+    if (!central) {
+      // the non-grav term on the Sun's (acceleration.rs:161-165) and its
+      // derivative along this column (the nominal lane takes the value)
+      const double rp[3] = {xe.x - sunp.x, xe.y - sunp.y, xe.z - sunp.z};
+      const double rv[3] = {ve.x - sunv.x, ve.y - sunv.y, ve.z - sunv.z};
+      const double dp[3] = {nominal ? 0.0 : P[0], nominal ? 0.0 : P[1], nominal ? 0.0 : P[2]};
+      const double dq[3] = {nominal ? 0.0 : Q[0], nominal ? 0.0 : Q[1], nominal ? 0.0 : Q[2]};
+      double an[3], dan[3];
+      nongrav_dir(rp, dp, rv, dq, ngA, ngdA, alpha, r_0, mexp, nexp, kexp, an, dan);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ev[c] += nominal ? an[c] : dan[c];
+    }
+
+    bool start_step = false, arc_done = false;
+    if (phase == GP_INIT) {
+      A[0] = ev[0];
+      A[1] = ev[1];
+      A[2] = ev[2];
+      if (fabs(tf - t) < 1e-10) {
+        arc_done = true;
+      } else {
+        h = copysign(0.1, tf - t);
+        fails = 0;
+        start_step = true;
+      }
+    } else if (phase == GP_SUB) {
+      // G column `sub` (radau.rs:292-308)
+      const double ih = s_inv_h[sub];
+      double gk[3] = {(ev[0] - A[0]) * ih, (ev[1] - A[1]) * ih, (ev[2] - A[2]) * ih};
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = s_rdiff[sub][z];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gk[c] = z < sub ? (gk[c] - G[z - 1][c]) * rd : gk[c];
+      }
+#pragma unroll
+      for (int z = 0; z < 7; ++z) {
+        const bool put = z == sub - 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G[z][c] = put ? gk[c] : G[z][c];
+      }
+      if (sub < 7) {
+        ++sub;
+      } else {
+        // B = G C (radau.rs:313)
+#pragma unroll
+        for (int q = 0; q < 7; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            double acc = G[q][c];
+#pragma unroll
+            for (int r = q + 1; r < 7; ++r) acc += G[r][c] * RTC.c[r][q];
+            B[q][c] = acc;
+          }
+        // the nominal column decides (radau.rs:314-319)
+        const double f0 = fabs(ev[0]) + 1e-6, f1 = fabs(ev[1]) + 1e-6, f2 = fabs(ev[2]) + 1e-6;
+        const double my_ratio = fmax(fmax(fabs(B[6][0] - bprev[0]) / f0,
+                                          fabs(B[6][1] - bprev[1]) / f1),
+                                     fabs(B[6][2] - bprev[2]) / f2);
+        const double ratio = row_bcast<WL_NOMINAL>(my_ratio);
+        if (ratio < 1e-14) {
+          // accept (radau.rs:320-342)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            double bw = 0.0, bu = 0.0;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) {
+              bw += B[q][c] * RTC.w[q];
+              bu += B[q][c] * RTC.u[q];
+            }
+            const double h2 = h * h;
+            X[c] = X[c] + V[c] * h + (A[c] * 0.5 + bw) * h2;
+            V[c] = V[c] + (A[c] + bu) * h;
+          }
+          t += h;
+          const double fm = fmax(fmax(f0, f1), f2);
+          const double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
+          double r7 = root7(p.epsilon * fm / bm);
+          r7 = r7 < 0.25 ? 0.25 : (r7 > 4.0 ? 4.0 : r7);
+          r7 = row_bcast<WL_NOMINAL>(r7);
+          next_h = h * r7;
+          cnt.z++;
+          phase = GP_EXACT;
+        } else if (sweep < 9) {
+          ++sweep;
+          sub = 1;
+          bprev[0] = B[6][0];
+          bprev[1] = B[6][1];
+          bprev[2] = B[6][2];
+          cnt.y++;
+        } else {
+          cnt.w++;
+          ++fails;
+          h *= 0.7;
+          if (fails > 10) {
+            st = ST_CONVERGENCE;
+            break;
+          }
+          if (fabs(h) < 0.005) h = copysign(0.005, h);
+          start_step = true;
+        }
+      }
+    } else {  // GP_EXACT
+      A[0] = ev[0];
+      A[1] = ev[1];
+      A[2] = ev[2];
+      h = next_h;
+      if (fabs(t - tf) < 1e-12) {
+        arc_done = true;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        start_step = true;
+      }
+    }
+    if (start_step) {
+      h_prop = h;
+      clamped = fabs(t - tf) <= fabs(h);
+      if (clamped) h = tf - t;
+      sweep = 0;
+      sub = 1;
+#pragma unroll
+      for (int i = 0; i < 7; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) G[i][c] = 0.0;
+      bprev[0] = B[6][0];
+      bprev[1] = B[6][1];
+      bprev[2] = B[6][2];
+      phase = GP_SUB;
+      cnt.y++;
+    }
+    if (arc_done) {
+      // the states at epoch j (barycentric equatorial), for the observation
+      // kernel: this lane's column, and from the nominal lane the Sun's state
+      // (evaluated at te = t = tf)
+      const int64_t jo = (int64_t)j * n + o;
+      if (l <= WL_NOMINAL) {
+        double* dst = p.ep_state + (jo * 10 + l) * 6;
+        dst[0] = X[0];
+        dst[1] = X[1];
+        dst[2] = X[2];
+        dst[3] = V[0];
+        dst[4] = V[1];
+        dst[5] = V[2];
+      }
+      if (nominal) {
+        double* ds = p.ep_sun + jo * 6;
+        ds[0] = central ? 0.0 : sunp.x;
+        ds[1] = central ? 0.0 : sunp.y;
+        ds[2] = central ? 0.0 : sunp.z;
+        ds[3] = central ? 0.0 : sunv.x;
+        ds[4] = central ? 0.0 : sunv.y;
+        ds[5] = central ? 0.0 : sunv.z;
+      }
+      ++j;
+      if (j < m) {
+        tf = p.jd_obs[(int64_t)j * n + o];
+        if (p.restart) {
+#pragma unroll
+          for (int i = 0; i < 7; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) B[i][c] = 0.0;
+          phase = GP_INIT;
+        } else if (fabs(tf - t) < 1e-10) {
+          phase = GP_EXACT;
+        } else {
+          h = copysign(fmax(fabs(clamped ? h_prop : h), 0.005), tf - t);
+          fails = 0;
+          h_prop = h;
+          clamped = fabs(t - tf) <= fabs(h);
+          if (clamped) h = tf - t;
+          sweep = 0;
+          sub = 1;
+#pragma unroll
+          for (int i = 0; i < 7; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) G[i][c] = 0.0;
+          bprev[0] = B[6][0];
+          bprev[1] = B[6][1];
+          bprev[2] = B[6][2];
+          phase = GP_SUB;
+          cnt.y++;
+        }
+      }
+    }
+  }
+
+  if (nominal) {
+    p.ep_done[o] = j;
+    p.status[o] = st;
+    p.lt_fail[o] = m;
+    if (p.counters) reinterpret_cast<int4*>(p.counters)[o] = cnt;
+  }
+  (void)impact;
+}
+
+// Observables at the epochs the wide kernel reached, one lane per (epoch,
+// object): the light-time step (value by the universal Kepler solver,
+// derivative by light_time_step) and RA/Dec of the observer-to-object vector
+// (simult_states.rs:86-120) with their derivatives along the nine columns;
+// the geometric states in the input frame when asked for.  A failed
+// light-time solve records its epoch for partials_finish_kernel.
+__global__ __launch_bounds__(256) void partials_observe_kernel(GradArgs p) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = p.n;
+  const int32_t m = p.m;
+  if (q >= (int64_t)m * n) return;
+  const int32_t j = (int32_t)(q / n);
+  const int64_t o = q - (int64_t)j * n;
+  const int64_t jo = q;
+  const double nan = __builtin_nan("");
+  const bool ecl = p.frame == 17;
+  bool ok = j < p.ep_done[o];
+  Vec3 po{nan, nan, nan}, vo{nan, nan, nan}, ob{0.0, 0.0, 0.0}, xh{0, 0, 0}, vh{0, 0, 0};
+  const double* es = p.ep_state + jo * 60;
+  double dtv = 0.0;
+  if (ok) {
+    const double* sn = p.ep_sun + jo * 6;
+    xh = Vec3{es[54] - sn[0], es[55] - sn[1], es[56] - sn[2]};
+    vh = Vec3{es[57] - sn[3], es[58] - sn[4], es[59] - sn[5]};
+    ob = Vec3{p.obs[0][jo], p.obs[1][jo], p.obs[2][jo]};
+    if (ecl) ob = ecl_to_equ(ob);
+    const Vec3 rho0{xh.x - ob.x, xh.y - ob.y, xh.z - ob.z};
+    dtv = -(sqrt(dot(rho0, rho0)) * C_AU_PER_DAY_INV);
+    if (analytic_2_body(dtv, xh, vh, po, vo) != ST_OK) {
+      ok = false;
+      atomicMin(&p.lt_fail[o], j);
+    }
+  }
+  const Vec3 r{po.x - ob.x, po.y - ob.y, po.z - ob.z};
+  const double rxy2 = r.x * r.x + r.y * r.y;
+  const double rn2 = rxy2 + r.z * r.z;
+  const double rn = sqrt(rn2);
+  double ra = nan, dec = nan;
+  if (ok) {
+    ra = 0.0;
+    dec = 0.0;
+    if (!(rn < 1e-10)) {
+      const double zr = fmin(fmax(r.z / rn, -1.0), 1.0);
+      dec = rem_tau(3.0 * (3.141592653589793 / 2.0) - acos(zr)) - 3.141592653589793;
+      ra = rem_tau(atan2(r.y, r.x));
+    }
+  }
+  p.radec[0][jo] = ra;
+  p.radec[1][jo] = dec;
+  if (p.state[0] != nullptr) {
+    Vec3 so = ok ? xh : Vec3{nan, nan, nan}, sv = ok ? vh : Vec3{nan, nan, nan};
+    if (ecl) {
+      so = equ_to_ecl(so);
+      sv = equ_to_ecl(sv);
+    }
+    p.state[0][jo] = so.x;
+    p.state[1][jo] = so.y;
+    p.state[2][jo] = so.z;
+    p.state[3][jo] = sv.x;
+    p.state[4][jo] = sv.y;
+    p.state[5][jo] = sv.z;
+  }
+  // the light-time step's derivative: Phi(dt) (third order) on the state
+  // derivative plus the end velocity times the derivative of dt
+  const double r2h = dot(xh, xh);
+  const double irh = 1.0 / sqrt(r2h);
+  const double g0 = GMS * irh * irh * irh;
+  const Vec3 rho0{xh.x - ob.x, xh.y - ob.y, xh.z - ob.z};
+  const double irho = 1.0 / sqrt(dot(rho0, rho0));
+  const double t = dtv, t2 = t * t;
+#pragma unroll 1
+  for (int k = 0; k < 9; ++k) {
+    const int64_t jk = ((int64_t)k * m + j) * n + o;
+    double dra = nan, ddec = nan;
+    Vec3 dxo{nan, nan, nan}, dvo{nan, nan, nan};
+    if (ok) {
+      const Vec3 dx{es[6 * k], es[6 * k + 1], es[6 * k + 2]};
+      const Vec3 dv{es[6 * k + 3], es[6 * k + 4], es[6 * k + 5]};
+      const double ddt = -(dot(rho0, dx) * irho) * C_AU_PER_DAY_INV;
+      const double rdx = 3.0 * dot(xh, dx) / r2h, rdv = 3.0 * dot(xh, dv) / r2h;
+      const Vec3 gdx{g0 * (xh.x * rdx - dx.x), g0 * (xh.y * rdx - dx.y), g0 * (xh.z * rdx - dx.z)};
+      const Vec3 gdv{g0 * (xh.x * rdv - dv.x), g0 * (xh.y * rdv - dv.y), g0 * (xh.z * rdv - dv.z)};
+      const Vec3 dr{dx.x + 0.5 * t2 * gdx.x + t * (dv.x + t2 * gdv.x / 6.0) + vo.x * ddt,
+                    dx.y + 0.5 * t2 * gdx.y + t * (dv.y + t2 * gdv.y / 6.0) + vo.y * ddt,
+                    dx.z + 0.5 * t2 * gdx.z + t * (dv.z + t2 * gdv.z / 6.0) + vo.z * ddt};
+      dra = (r.x * dr.y - r.y * dr.x) / rxy2;
+      ddec = (dr.z * rxy2 - r.z * (r.x * dr.x + r.y * dr.y)) / (rn2 * sqrt(rxy2));
+      dxo = ecl ? equ_to_ecl(dx) : dx;
+      dvo = ecl ? equ_to_ecl(dv) : dv;
+    }
+    p.jac[0][jk] = dra;
+    p.jac[1][jk] = ddec;
+    if (p.dstate[0] != nullptr) {
+      p.dstate[0][jk] = dxo.x;
+      p.dstate[1][jk] = dxo.y;
+      p.dstate[2][jk] = dxo.z;
+      p.dstate[3][jk] = dvo.x;
+      p.dstate[4][jk] = dvo.y;
+      p.dstate[5][jk] = dvo.z;
+    }
+  }
+}
+
+// Per object: a failed light-time step is the object's error (the analytic
+// two-body error, as a propagate_two_body call would return it) and leaves
+// its epoch and the later ones undefined.
+__global__ void partials_finish_kernel(GradArgs p) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= p.n) return;
+  const int32_t f = p.lt_fail[o];
+  if (f >= p.ep_done[o]) return;
+  if (p.status[o] == ST_OK) p.status[o] = ST_CONVERGENCE;
+  const double nan = __builtin_nan("");
+  const int64_t n = p.n;
+  const int32_t m = p.m;
+  for (int32_t j = f; j < m; ++j) {
+    const int64_t jo = (int64_t)j * n + o;
+    p.radec[0][jo] = nan;
+    p.radec[1][jo] = nan;
+    if (p.state[0] != nullptr)
+      for (int c = 0; c < 6; ++c) p.state[c][jo] = nan;
+    for (int k = 0; k < 9; ++k) {
+      const int64_t jk = ((int64_t)k * m + j) * n + o;
+      p.jac[0][jk] = nan;
+      p.jac[1][jk] = nan;
+      if (p.dstate[0] != nullptr)
+        for (int c = 0; c < 6; ++c) p.dstate[c][jk] = nan;
+    }
+  }
+}
+
 int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradLaunch& g,
                               hipStream_t stream) {
   GradArgs p{};
@@ -617,10 +1352,29 @@ int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradL
   p.jac[1] = g.jac[1];
   p.status = g.status;
   p.counters = g.counters;
-  const int64_t lanes = g.n * 9;
-  if (lanes <= 0) return 0;
-  const unsigned blocks = (unsigned)((lanes + 63) / 64);
-  hipLaunchKernelGGL(radau_partials_kernel, dim3(blocks), dim3(64), 0, stream, spk, coef, p);
+  if (g.n <= 0) return 0;
+  // KH_PARTIALS=lane9: the one-lane-per-parameter dual-number kernel
+  const char* sel = getenv("KH_PARTIALS");
+  if (sel && strcmp(sel, "lane9") == 0) {
+    const int64_t lanes = g.n * 9;
+    const unsigned blocks = (unsigned)((lanes + 63) / 64);
+    hipLaunchKernelGGL(radau_partials_kernel, dim3(blocks), dim3(64), 0, stream, spk, coef, p);
+  } else {
+    p.ep_state = g.ep_state;
+    p.ep_sun = g.ep_sun;
+    p.ep_done = g.ep_done;
+    p.lt_fail = g.lt_fail;
+    const int64_t lanes = g.n * WL;
+    const unsigned blocks = (unsigned)((lanes + 255) / 256);
+    hipLaunchKernelGGL(radau_partials_wide_kernel, dim3(blocks), dim3(256), 0, stream, spk,
+                       coef, p);
+    const int64_t mn = (int64_t)g.m * g.n;
+    if (mn > 0)
+      hipLaunchKernelGGL(partials_observe_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256),
+                         0, stream, p);
+    hipLaunchKernelGGL(partials_finish_kernel, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0,
+                       stream, p);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

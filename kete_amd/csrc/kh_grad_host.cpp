@@ -86,6 +86,19 @@ int32_t run_partials(kh_ctx* c, int64_t n, const double* jd0, const double* cons
   g.jac[1] = jac[1];
   g.status = status;
   g.counters = reinterpret_cast<int32_t*>(counters);
+  // working memory: the states at every epoch, for the observation kernel
+  {
+    const size_t mn = (size_t)m * (size_t)n;
+    int32_t r = kh_grow(&c->pbuf, &c->pbuf_size, partials_buf_bytes(n, m));
+    if (r) return r;
+    char* q = (char*)c->pbuf;
+    g.ep_state = (double*)q;
+    g.ep_sun = g.ep_state + mn * 60;
+    q += (mn * 66 * 8 + 255) / 256 * 256;
+    g.ep_done = (int32_t*)q;
+    q += ((size_t)n * 4 + 255) / 256 * 256;
+    g.lt_fail = (int32_t*)q;
+  }
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (launch_radau_partials(c->d_spk, c->d_coef, g, c->stream))
     return fail(KH_E_HIP, "partials launch failed");

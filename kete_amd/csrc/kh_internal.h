@@ -23,6 +23,27 @@ struct RadauTables {
 };
 
 void fill_radau_tables(RadauTables& t);
+
+// The same tables as compile-time constants, for the fixed-index uses (C, W,
+// U), which then become instruction literals instead of registers.
+constexpr RadauTables make_tables() {
+  RadauTables t{};
+  const double H[8] = {0.0, 0.05626256053692215, 0.18024069173689236, 0.3526247171131696,
+                       0.5471536263305554, 0.7342101772154105, 0.8853209468390958,
+                       0.9775206135612875};
+  for (int i = 0; i < 8; ++i) t.h[i] = H[i];
+  for (int i = 0; i < 7; ++i) {
+    t.w[i] = 1.0 / (double)((i + 2) * (i + 3));
+    t.u[i] = 1.0 / (double)(i + 2);
+  }
+  for (int i = 0; i < 7; ++i) t.c[i][i] = 1.0;
+  for (int i = 0; i < 7; ++i) {
+    if (i > 0) t.c[i][0] = -H[i] * t.c[i - 1][0];
+    for (int j = 1; j < i; ++j) t.c[i][j] = t.c[i - 1][j - 1] - H[i] * t.c[i - 1][j];
+  }
+  return t;
+}
+constexpr RadauTables RTC = make_tables();
 int32_t upload_radau_tables(const RadauTables& t);
 
 struct NBodyLaunch {
@@ -302,7 +323,17 @@ struct GradLaunch {
   double* dstate[6];
   int32_t* status;
   int32_t* counters;
+  // working memory of the wide kernel (partials_buf_bytes)
+  double* ep_state;  // [m][n][10][6]
+  double* ep_sun;    // [m][n][6]
+  int32_t* ep_done;  // [n]
+  int32_t* lt_fail;  // [n]
 };
+
+// Bytes of the wide partials kernel's working memory for n objects, m epochs.
+constexpr size_t partials_buf_bytes(int64_t n, int32_t m) {
+  return ((size_t)m * (size_t)n * 66 * 8 + 255) / 256 * 256 + 2 * (((size_t)n * 4 + 255) / 256 * 256);
+}
 
 int32_t upload_grad_tables(const RadauTables& t);
 int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradLaunch& g,

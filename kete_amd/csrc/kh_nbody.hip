@@ -32,26 +32,6 @@ namespace kh {
 // Radau tables (radau.rs:24-68), filled by the host from the same formulas.
 __constant__ RadauTables RT;
 
-// The same tables as compile-time constants, for the fixed-index uses (C, W,
-// U), which then become instruction literals instead of registers.
-constexpr RadauTables make_tables() {
-  RadauTables t{};
-  const double H[8] = {0.0, 0.05626256053692215, 0.18024069173689236, 0.3526247171131696,
-                       0.5471536263305554, 0.7342101772154105, 0.8853209468390958,
-                       0.9775206135612875};
-  for (int i = 0; i < 8; ++i) t.h[i] = H[i];
-  for (int i = 0; i < 7; ++i) {
-    t.w[i] = 1.0 / (double)((i + 2) * (i + 3));
-    t.u[i] = 1.0 / (double)(i + 2);
-  }
-  for (int i = 0; i < 7; ++i) t.c[i][i] = 1.0;
-  for (int i = 0; i < 7; ++i) {
-    if (i > 0) t.c[i][0] = -H[i] * t.c[i - 1][0];
-    for (int j = 1; j < i; ++j) t.c[i][j] = t.c[i - 1][j - 1] - H[i] * t.c[i - 1][j];
-  }
-  return t;
-}
-constexpr RadauTables RTC = make_tables();
 
 struct NBodyArgs {
   int64_t n;
@@ -398,21 +378,6 @@ __global__ void order_scatter_kernel(int64_t n, const uint16_t* __restrict__ bkt
     if (cnt[k] != 0) base[k] = atomicAdd(&off[k], cnt[k]);
   __syncthreads();
   if (i < n) perm[base[b] + rank] = (uint32_t)i;
-}
-
-// y = x^(1/7) for the step-size ratio (radau.rs:339-342), by two Newton steps
-// from a single-precision seed; inputs outside [3e-5, 3e4] only matter through
-// the clamp to [0.25, 4] that follows, so they are clamped first.
-__device__ __forceinline__ double root7(double x) {
-  if (isnan(x)) return x;
-  const double xc = fmin(fmax(x, 3e-5), 3e4);
-  double y = (double)exp2f(log2f((float)xc) * (1.0f / 7.0f));
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    const double y2 = y * y, y3 = y2 * y, y6 = y3 * y3, y7 = y6 * y;
-    y = y - (y7 - xc) / (7.0 * y6);
-  }
-  return y;
 }
 
 // LDS copy of the per-substep factors, indexed by each lane's substep j.
