@@ -291,7 +291,9 @@ int32_t kh_radec_partials(kh_ctx *ctx, int64_t n, const double *jd0,
                           int32_t force, int32_t restart, double epsilon, double *const radec[2],
                           double *const jac[2], double *const state[6],
                           double *const dstate[6], int32_t *status, kh_counters *counters);
-/* Device-pointer variant (all arrays in device memory). */
+/* Device-pointer variant (all arrays in device memory).  Non-grav models are
+ * checked per object on the device: a dust model, a JplComet delay or an
+ * unknown kind ends that object with ValueError (2) instead of KH_E_ARG. */
 int32_t kh_radec_partials_dev(kh_ctx *ctx, int64_t n, const double *jd0,
                               const double *const xyz[6], int32_t frame, const kh_nongrav *ng,
                               int32_t m, const double *jd_obs, const double *const obs[3],

@@ -44,6 +44,15 @@ def _asteroid_models(a123: torch.Tensor) -> torch.Tensor:
     return rec.contiguous()
 
 
+def _library_stream(ctx):
+    """The context's HIP stream as a torch stream object."""
+    s = getattr(ctx, "_torch_stream", None)
+    if s is None:
+        s = torch.cuda.ExternalStream(ctx.L.kh_ctx_stream(ctx.h), device=f"cuda:{ctx.device}")
+        ctx._torch_stream = s
+    return s
+
+
 def radec_partials_dev(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, epsilon=1e-6,
                        ctx=None, counters=None, restart=False):
     """One launch on device tensors.
@@ -64,7 +73,11 @@ def radec_partials_dev(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, 
     radec = torch.empty((2, m, n), dtype=torch.float64, device=dev)
     jac = torch.empty((2, 9, m, n), dtype=torch.float64, device=dev)
     status = torch.empty(n, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize(dev)   # inputs written on torch's stream are complete
+    # the library's stream waits for torch's work on the inputs, and torch's
+    # stream for the library's results: stream order, no host round trip
+    lib_stream = _library_stream(ctx)
+    cur = torch.cuda.current_stream(dev)
+    lib_stream.wait_stream(cur)
     _lib.check(ctx.L.kh_radec_partials_dev(
         ctx.h, n, _lib.raw(jd0.data_ptr()), _lib.ptr_array([c.data_ptr() for c in cols]), frame,
         _lib.raw(ng.data_ptr(), _lib.KhNonGrav), m, _lib.raw(jd_obs.data_ptr()),
@@ -72,7 +85,10 @@ def radec_partials_dev(params, jd0, jd_obs, obs, frame=FRAME_ECLIPTIC, force=0, 
         _lib.ptr_array([radec[0].data_ptr(), radec[1].data_ptr()]),
         _lib.ptr_array([jac[0].data_ptr(), jac[1].data_ptr()]), None, None,
         _lib.raw(status.data_ptr(), _lib.C.c_int32),
-        _lib.raw(counters.data_ptr(), _lib.KhCounters) if counters is not None else None, 1))
+        _lib.raw(counters.data_ptr(), _lib.KhCounters) if counters is not None else None, 0))
+    cur.wait_stream(lib_stream)
+    for t in (radec, jac, status) + ((counters,) if counters is not None else ()):
+        t.record_stream(lib_stream)
     return radec, jac, status
 
 

@@ -630,7 +630,8 @@ constexpr int WL_EMB = 10;
 
 // v from another lane of the same DPP row (a 64-bit move is two 32-bit ones)
 // JplComet non-grav acceleration (nongrav.rs:114-157) a and its derivative
-// da along (dp, dv, dA) at position p, velocity v relative to the Sun:
+// da along (dp, dv, dA) at position p (ip = 1/|p|), velocity v relative to
+// the Sun:
 // a = g(r) (A1 r^ + A2 t^ + A3 n^), g = alpha (r/r_0)^-m (1 + (r/r_0)^n)^-k.
 // Norms come from reciprocal square roots and kete's divisions by them
 // become multiplications (last-bit differences); with kete's default
@@ -639,11 +640,10 @@ __device__ __forceinline__ void nongrav_dir(const double p[3], const double dp[3
                                             const double v[3], const double dv[3],
                                             const double A[3], const double dA[3], double alpha,
                                             double r_0, double m, double ne, double k,
-                                            double a[3], double da[3]) {
+                                            double ip, double a[3], double da[3]) {
   auto dot3 = [](const double* x, const double* y) {
     return x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
   };
-  const double ip = rsqrt_nr(dot3(p, p));
   const double n = dot3(p, p) * ip;
   const double dn = dot3(p, dp) * ip;
   double pn[3], dpn[3];
@@ -718,7 +718,7 @@ __device__ __forceinline__ void cheb_both_lds(const double* rec, double x, Vec3&
   double t2 = 1.0, t1 = x, u2 = 1.0, u1 = x2;
 #pragma unroll
   for (int k0 = 2; k0 < 14; k0 += 4) {
-    asm volatile("" ::: "memory");
+    // (chunk boundary)
 #pragma unroll
     for (int k = k0; k < k0 + 4 && k < 14; ++k) {
       const double t = x2 * t1 - t2;
@@ -797,9 +797,16 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
   const bool nominal = l == WL_NOMINAL;
   const bool tangent = l < 9;
 
-  // JplComet parameters (new_asteroid defaults and A = 0 for kind 0)
+  // JplComet parameters (new_asteroid defaults and A = 0 for kind 0); a
+  // dust model, a JplComet delay or an unknown kind has nothing to
+  // differentiate here: the object ends with ValueError
   double a1 = 0.0, a2 = 0.0, a3 = 0.0, alpha = 1.0, r_0 = 1.0, mexp = 2.0, nexp = 1.0,
          kexp = 0.0;
+  bool bad_model = false;
+  if (p.ng != nullptr) {
+    const int32_t kind = p.ng[o].kind;
+    bad_model = kind < 0 || kind > 1 || (kind == 1 && p.ng[o].dt != 0.0);
+  }
   if (p.ng != nullptr && p.ng[o].kind == 1) {
     const NonGravDev& g = p.ng[o];
     a1 = g.a1;
@@ -822,6 +829,7 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
   if (!(isfinite(t) && isfinite(x0.x) && isfinite(x0.y) && isfinite(x0.z) && isfinite(v0.x) &&
         isfinite(v0.y) && isfinite(v0.z)))
     st = ST_VALUE_ERROR;
+  if (bad_model) st = ST_VALUE_ERROR;
   if (st == ST_OK && !central) {
     Vec3 sp, sv;
     st = chain_state(spk, coef, p.sun, t, sp, sv);
@@ -923,12 +931,14 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
       if (cache_rec) {
         double* slot = &s_rec[threadIdx.x / WL][l][0];
         if (ri != rec_cached) {
+          // the record, with word 1 (the radius) stored as its reciprocal
           const double2* __restrict__ src = reinterpret_cast<const double2*>(grec);
 #pragma unroll
           for (int q = 0; q < 22; ++q) reinterpret_cast<double2*>(slot)[q] = src[q];
+          slot[1] = 1.0 / grec[1];
           rec_cached = ri;
         }
-        ir = 1.0 / slot[1];
+        ir = slot[1];
         xs = (js - slot[0]) * ir;
         cheb_both_lds(slot, xs, pp, dd);
       } else {
@@ -953,11 +963,12 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
     }
     // this lane's body: acceleration and the symmetric Jacobian (Newton, J2)
     double ba[3] = {0.0, 0.0, 0.0}, bj[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double inv_r = 0.0;
     if (is_body) {
       const double rp[3] = {xe.x - bp.x, xe.y - bp.y, xe.z - bp.z};
       const double rv[3] = {ve.x - bv.x, ve.y - bv.y, ve.z - bv.z};
       const double r2 = rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2];
-      const double inv_r = rsqrt_nr(r2);
+      inv_r = rsqrt_nr(r2);
       if (exact && !central && r2 * inv_r <= bd.radius && lst == ST_OK) lst = ST_IMPACT;
       body_terms_sym(mass, bd.radius, j2, gr, ecl_j2, ROT_C, ROT_S, C_AU_PER_DAY_INV_SQUARED,
                      rp, rv, inv_r, ba, bj);
@@ -984,6 +995,8 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
       ev[1] = nominal ? as1 : js[1] * P[0] + js[3] * P[1] + js[4] * P[2];
       ev[2] = nominal ? as2 : js[2] * P[0] + js[4] * P[1] + js[5] * P[2];
     }
+    // inverse distances to the Sun and Jupiter, from their body lanes
+    const double ir_sun = row_bcast<0>(inv_r), ir_jup = row_bcast<JUP>(inv_r);
     if (!central) {
       // derivatives of the GR terms of the Sun and Jupiter along this column
       const double dp[3] = {nominal ? 0.0 : P[0], nominal ? 0.0 : P[1], nominal ? 0.0 : P[2]};
@@ -995,7 +1008,7 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
         const double rv[3] = {ve.x - gb[b][1].x, ve.y - gb[b][1].y, ve.z - gb[b][1].z};
         double dg[3];
         gr_dir(b == 0 ? m_sun : m_jup, C_AU_PER_DAY_INV_SQUARED, rp, rv, dp, dq,
-               rsqrt_nr(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]), dg);
+               b == 0 ? ir_sun : ir_jup, dg);
 #pragma unroll
         for (int c = 0; c < 3; ++c) ev[c] += dg[c];
       }
@@ -1008,7 +1021,7 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
       const double dp[3] = {nominal ? 0.0 : P[0], nominal ? 0.0 : P[1], nominal ? 0.0 : P[2]};
       const double dq[3] = {nominal ? 0.0 : Q[0], nominal ? 0.0 : Q[1], nominal ? 0.0 : Q[2]};
       double an[3], dan[3];
-      nongrav_dir(rp, dp, rv, dq, ngA, ngdA, alpha, r_0, mexp, nexp, kexp, an, dan);
+      nongrav_dir(rp, dp, rv, dq, ngA, ngdA, alpha, r_0, mexp, nexp, kexp, ir_sun, an, dan);
 #pragma unroll
       for (int c = 0; c < 3; ++c) ev[c] += nominal ? an[c] : dan[c];
     }
