@@ -769,6 +769,7 @@ __device__ __forceinline__ double row_sum(double v) {
   return v;
 }
 
+template <bool CENTRAL>
 __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
     const SpkDev* __restrict__ spk, const double* __restrict__ coef, GradArgs p) {
   // per-substep factors, indexed by each group's own substep
@@ -793,7 +794,7 @@ __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
   const int32_t m = p.m;
   const double nan = __builtin_nan("");
   const bool ecl = p.frame == 17;
-  const bool central = p.force == 1;
+  constexpr bool central = CENTRAL;
   const bool nominal = l == WL_NOMINAL;
   const bool tangent = l < 9;
 
@@ -1379,8 +1380,12 @@ int32_t launch_radau_partials(const SpkDev* spk, const double* coef, const GradL
     p.lt_fail = g.lt_fail;
     const int64_t lanes = g.n * WL;
     const unsigned blocks = (unsigned)((lanes + 255) / 256);
-    hipLaunchKernelGGL(radau_partials_wide_kernel, dim3(blocks), dim3(256), 0, stream, spk,
-                       coef, p);
+    if (g.force == 1)
+      hipLaunchKernelGGL(radau_partials_wide_kernel<true>, dim3(blocks), dim3(256), 0, stream,
+                         spk, coef, p);
+    else
+      hipLaunchKernelGGL(radau_partials_wide_kernel<false>, dim3(blocks), dim3(256), 0, stream,
+                         spk, coef, p);
     const int64_t mn = (int64_t)g.m * g.n;
     if (mn > 0)
       hipLaunchKernelGGL(partials_observe_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256),
