@@ -216,3 +216,138 @@ def test_partials_error_semantics(ctx):
     with pytest.raises(ka.KeteHipError):
         ctx.radec_partials(JD0, xyz[:, [0]], jd_obs, earth[:3],
                            non_gravs=[ka.NonGravModel.new_dust(0.01)])
+
+
+def planet_min_distance(oracle, jd0, xyz, nongravs, days):
+    """Smallest distance of each object to Mercury, Venus, the Earth, the Moon
+    and Mars, sampled every 0.1 day along the oracle's N-body integration."""
+    jd = jd0 + np.arange(0.1, days, 0.1)
+    r = oracle.chain_observe(jd0, xyz, jd, np.ones((3, len(jd))), nongravs=nongravs,
+                             restart=False, n_threads=16)
+    dmin = np.full(xyz.shape[1], np.inf)
+    for pid in (1, 2, 399, 301, 4):
+        st = [oracle.spk_state(pid, t, 10, oracle.FRAME_ECLIPTIC) for t in jd]
+        assert all(x[0] == 0 for x in st)
+        pp = np.array([x[1] for x in st]).T
+        d = np.linalg.norm(r["state"][:3] - pp[:, :, None], axis=0)
+        dmin = np.minimum(dmin, d.min(axis=0))
+    return dmin
+
+
+@pytest.mark.parametrize("restart", [True, False])
+def test_config5_slice_against_frozen_step_differences(ctx, oracle_spk, restart):
+    """256 orbits of the orbit-fit workload (NEO-like, A1..A3 ~ N(0, 1e-12)
+    AU/d^2, 60 epochs over two years, the Earth as observer): all nine columns
+    of d(RA, Dec)/dp against 5-point central differences of the oracle's
+    chain with its step schedule frozen (recorded on the nominal orbit and
+    replayed, so the map is smooth: differences at steps h and 2 h agree to
+    5e-8 of each column's largest entry for the state columns and 4e-7 for
+    A3, whose column is the smallest; steps: 2e-6 AU, 2e-8 AU/d, 1e-10
+    AU/d^2, where the truncation error (O(h^4), growing through close
+    approaches) and the rounding error (~1e-15 rad per evaluation over 12 h)
+    cross).  The partials are the derivative of the computed map with the
+    steps held fixed, so this is the reference they must match; plain
+    differences of the adaptive map scatter by 1e-5 (its step decisions jump
+    with the input) and, at the 1e-9 AU/d^2 steps a 2-year arc tolerates
+    poorly, by up to 1e-3 near the celestial poles.  Bar: 1e-6 of each
+    column's largest entry, on orbits that stay 0.05 AU from the inner
+    planets and the Moon: a close approach amplifies the 1e-11 rad
+    differences of the two integrators' step sequences (restart=False: one
+    integration across all epochs) by up to 1e7, after which the two chains
+    are different orbits.  Values: 1e-8 rad on those orbits."""
+    oracle = oracle_spk
+    n, m = 256, 60
+    rng = np.random.default_rng(syn.SEED_BASE + 5)
+    xyz = syn.n_body_workload(n=4096, neo_fraction=1.0, seed_index=5)[:, :n].copy()
+    a = rng.normal(0.0, 1e-12, (4096, 3))[:n].T.copy()
+    jd_obs = np.sort(JD0 + rng.uniform(0.0, 2.0 * 365.25, m))
+    earth, _ = ctx.spk_state(399, jd_obs, 10, ka.Frames.Ecliptic)
+    obs = earth[:3]
+    print(f"config-5 slice: {n} orbits x {m} epochs, restart={restart}", flush=True)
+    g = ctx.radec_partials(JD0, xyz, jd_obs, obs, restart=restart,
+                           non_gravs=[ka.NonGravModel.new_asteroid(*a[:, i]) for i in range(n)])
+    assert np.all(g["status"] == 0)
+
+    def ng_list(aa):
+        return [oracle.NonGrav(1, *aa[:, i], 1.0, 1.0, 2.0, 1.0, 0.0, 0.0, 0.0)
+                for i in range(aa.shape[1])]
+
+    rec = oracle.chain_observe(JD0, xyz, jd_obs, obs, nongravs=ng_list(a), restart=restart,
+                               record=True, n_threads=16)
+    assert np.all(rec["status"] == 0)
+    dmin = planet_min_distance(oracle, JD0, xyz, ng_list(a), 2 * 365.25 + 1)
+    far = dmin > 0.05
+    # values: the same chain to the integrators' agreement
+    dra = (g["radec"][0] - rec["radec"][0] + np.pi) % (2 * np.pi) - np.pi
+    dv = np.maximum(np.abs(dra * np.cos(rec["radec"][1])),
+                    np.abs(g["radec"][1] - rec["radec"][1])).max(axis=0)
+    print(f"  values: beyond 0.05 AU max {dv[far].max():.1e} rad; within: "
+          + " ".join(f"{d:.3f} AU {v:.1e} rad" for d, v in zip(dmin[~far], dv[~far])), flush=True)
+    assert dv[far].max() < 1e-8
+
+    base = np.array([2e-6] * 3 + [2e-8] * 3 + [1e-10] * 3)
+    w = {-2: 1.0, -1: -8.0, 1: 8.0, 2: -1.0}
+
+    def frozen_fd(scale):
+        J = np.zeros((2, 9, m, n))
+        for k in range(9):
+            h = base[k] * scale
+            ev = {}
+            for c in w:
+                x, aa = xyz.copy(), a.copy()
+                if k < 6:
+                    x[k] += c * h
+                else:
+                    aa[k - 6] += c * h
+                r = oracle.chain_observe(JD0, x, jd_obs, obs, nongravs=ng_list(aa),
+                                         restart=restart, schedule=rec["schedule"],
+                                         n_threads=16)
+                ev[c] = r["radec"]
+            ra = sum(w[c] * ((ev[c][0] - ev[-2][0] + np.pi) % (2 * np.pi) - np.pi) for c in w)
+            J[0, k] = ra / (12 * h)
+            J[1, k] = sum(w[c] * ev[c][1] for c in w) / (12 * h)
+        return J
+
+    j1, j2 = frozen_fd(1.0), frozen_fd(2.0)
+    scale = np.abs(j1).max(axis=(0, 2), keepdims=True)       # per orbit and column
+
+    def rel(x):
+        return (np.abs(x - j1) / scale).max(axis=(0, 1, 2))
+
+    err, fd_err = rel(g["jac"]), rel(j2)
+    print(f"  orbits beyond 0.05 AU of the inner planets: {int(far.sum())} of {n}; "
+          f"partials vs frozen FD: median {np.median(err[far]):.2e} max {err[far].max():.2e}; "
+          f"FD h vs 2h max {fd_err[far].max():.2e}", flush=True)
+    col = (np.abs(g["jac"] - j1) / scale)[..., far].max(axis=(0, 2, 3))
+    print("  per column max: " + " ".join(f"{c:.1e}" for c in col), flush=True)
+    if (~far).any():
+        print(f"  within 0.05 AU: max {err[~far].max():.2e}", flush=True)
+    assert far.sum() > 0.8 * n
+    assert fd_err[far].max() < 5e-7
+    assert err[far].max() < 1e-6
+
+
+def test_device_models_checked_per_object(ctx, monkeypatch):
+    """The device entry point checks each object's non-grav record on the
+    device: a dust model ends that object with ValueError (2) and leaves the
+    others untouched."""
+    torch = pytest.importorskip("torch")
+    from kete_amd import astrometry as ast
+    w = ast.synthetic_observations(n=8, n_obs=6, years=0.5, seed_index=5, ctx=ctx)
+    good = ast.radec_partials_dev(w["params_true"], w["jd0"], w["jd_obs"], w["obs"])
+    make = ast._asteroid_models
+
+    def with_dust(a123):
+        rec = make(a123)
+        rec.view(torch.int64)[3, 0] = 2            # kind 2: dust
+        return rec
+
+    monkeypatch.setattr(ast, "_asteroid_models", with_dust)
+    radec, jac, status = ast.radec_partials_dev(w["params_true"], w["jd0"], w["jd_obs"],
+                                                w["obs"])
+    st = status.cpu().numpy()
+    assert st[3] == 2 and np.all(np.delete(st, 3) == 0)
+    assert torch.isnan(radec[:, :, 3]).all() and torch.isnan(jac[..., 3]).all()
+    keep = [i for i in range(8) if i != 3]
+    assert torch.equal(radec[:, :, keep], good[0][:, :, keep])
+    assert torch.equal(jac[..., keep], good[1][..., keep])
