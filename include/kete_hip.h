@@ -396,6 +396,13 @@ int32_t kh_last_launch(kh_ctx *ctx, int32_t *block, int32_t *grid, int32_t *mode
  * N-body path, measured with HIP events on the context's stream. */
 double kh_last_kernel_ms(kh_ctx *ctx);
 
+/* Prefilter phase of the last kh_fov_state_check on this context (a
+ * measurement hook; kete has no counterpart): total milliseconds of the
+ * timed prefilter launches (HIP events on the context's stream, the first
+ * 4096 launches of the call), how many were timed, and the (state, field)
+ * pairs they scanned.  KH_E_ARG when no FOV call has completed. */
+int32_t kh_fov_last_prefilter(kh_ctx *ctx, double *ms, int64_t *launches, int64_t *pairs);
+
 #ifdef __cplusplus
 }
 #endif

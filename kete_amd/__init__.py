@@ -306,6 +306,14 @@ class Context:
     def last_kernel_ms(self) -> float:
         return self.L.kh_last_kernel_ms(self.h)
 
+    def last_fov_prefilter(self):
+        """(milliseconds, launches, state-field pairs) of the timed prefilter
+        launches of the last fov_state_check on this context."""
+        ms, nl, pairs = C.c_double(), C.c_int64(), C.c_int64()
+        _lib.check(self.L.kh_fov_last_prefilter(self.h, C.byref(ms), C.byref(nl),
+                                                C.byref(pairs)))
+        return ms.value, nl.value, pairs.value
+
     def last_launch(self):
         b, g, m = C.c_int32(), C.c_int32(), C.c_int32()
         _lib.check(self.L.kh_last_launch(self.h, C.byref(b), C.byref(g), C.byref(m)))

@@ -29,7 +29,7 @@ EXPORTS = [
     "kh_radec_vmag", "kh_radec_partials", "kh_radec_partials_dev", "kh_autotune_n_body",
     "kh_autotune_clear", "kh_fov_spk_check", "kh_propagate_n_body_long",
     "kh_propagate_n_body_long_dev", "kh_moid", "kh_moid_dev",
-    "kh_propagate_n_body_dev_noalloc",
+    "kh_propagate_n_body_dev_noalloc", "kh_fov_last_prefilter",
 ]
 
 # Largest batch of kh_propagate_n_body_dev_noalloc (KH_NOALLOC_MAX_N)
@@ -142,6 +142,8 @@ def load():
     L.kh_fov_state_check.argtypes = [V, I64, D, PP, I32, I64, P, PP, P, D, I32, I64,
                                      C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
     L.kh_fov_state_check.restype = I32
+    L.kh_fov_last_prefilter.argtypes = [V, C.POINTER(D), C.POINTER(I64), C.POINTER(I64)]
+    L.kh_fov_last_prefilter.restype = I32
     L.kh_fov_spk_check.argtypes = [V, I64, C.POINTER(I64), I32, I64, P, PP, P, I64,
                                    C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), PP]
     L.kh_fov_spk_check.restype = I32

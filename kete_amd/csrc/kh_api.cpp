@@ -160,6 +160,7 @@ int32_t kh_ctx_destroy(kh_ctx* c) {
   if (c->d_split) (void)hipFree(c->d_split);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->fov_ev) (void)hipEventDestroy(e);
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);

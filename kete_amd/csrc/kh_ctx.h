@@ -48,6 +48,11 @@ struct kh_ctx {
   unsigned long long* d_range = nullptr;  // epoch range of the last N-body batch [2]
   unsigned long long* d_counts = nullptr;  // FOV list counters [4]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // FOV prefilter timing of the last kh_fov_state_check: event pairs around
+  // each prefilter launch (the first FOV_TIMED_LAUNCHES of a call)
+  std::vector<hipEvent_t> fov_ev;
+  double fov_pf_ms = -1.0;
+  int64_t fov_pf_launches = 0, fov_pf_pairs = 0;
   hipStream_t stream2 = nullptr;  // the N-body latency kernel runs here
   int64_t* d_split = nullptr;     // head of the N-body order it takes
   hipEvent_t fork = nullptr, join = nullptr;

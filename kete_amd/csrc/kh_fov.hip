@@ -44,15 +44,13 @@ __device__ __forceinline__ bool rect_contains(const double* __restrict__ nrm, Ve
   return true;
 }
 
-constexpr int MAX_GROUP_FIELDS = 256;
-
 // Candidates collect in a per-block LDS list and reach the global list with
 // one atomic per block: a single global counter bumped per wavefront and
 // field serialised ~1e5 atomics per launch (5x the kernel's arithmetic).
 constexpr int PF_BLOCK_CAND = 1024;
 
 __global__ void __launch_bounds__(256) fov_prefilter_kernel(FovPrefilter p) {
-  // the group's fields, sized to the group (dynamic LDS)
+  // the tile's fields (at most FOV_MAX_TILE_FIELDS; dynamic LDS)
   extern __shared__ FieldDev sf[];
   __shared__ int32_t c_field[PF_BLOCK_CAND], c_kind[PF_BLOCK_CAND];
   __shared__ int64_t c_obj[PF_BLOCK_CAND];
@@ -438,7 +436,7 @@ __global__ void radec_vmag_kernel(RadecArgs p) {
 
 int32_t launch_fov_prefilter(const FovPrefilter& p, hipStream_t s) {
   if (p.n <= 0 || p.nf <= 0) return 0;
-  if (p.nf > MAX_GROUP_FIELDS) return -1;
+  if (p.nf > FOV_MAX_TILE_FIELDS) return -1;
   hipLaunchKernelGGL(fov_prefilter_kernel, dim3((unsigned)((p.n + 255) / 256)), dim3(256),
                      (size_t)p.nf * sizeof(FieldDev), s, p);
   return hipGetLastError() == hipSuccess ? 0 : -1;

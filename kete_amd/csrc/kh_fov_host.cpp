@@ -61,6 +61,11 @@ __global__ void zero_list_counts(unsigned long long* p) {
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// fields per prefilter launch (the LDS tile of fov_prefilter_kernel)
+constexpr int64_t FOV_TILE_FIELDS = kh::FOV_MAX_TILE_FIELDS;
+// prefilter launches timed per call (event pairs held by the context)
+constexpr int64_t FOV_TIMED_LAUNCHES = 4096;
+
 }  // namespace
 
 extern "C" {
@@ -122,10 +127,18 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
     chunks.push_back({k, e});
     k = e;
   }
-  int64_t max_group = 0;
-  for (auto& ch : chunks) max_group = std::max(max_group, ch.second - ch.first);
-  if (max_group > 256)
-    return kh_fail(KH_E_ARG, "more than 256 fields within 2*dt_limit of each other");
+  // a chunk's fields are checked in tiles of at most FOV_TILE_FIELDS (the
+  // prefilter stages one tile in LDS); the tiles share the chunk's states
+  int64_t n_tiles = 0;
+  for (auto& ch : chunks) n_tiles += (ch.second - ch.first + FOV_TILE_FIELDS - 1) / FOV_TILE_FIELDS;
+  const int64_t n_timed = std::min<int64_t>(n_tiles, FOV_TIMED_LAUNCHES);
+  while ((int64_t)c->fov_ev.size() < 2 * n_timed) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->fov_ev.push_back(e);
+  }
+  c->fov_pf_ms = -1.0;
+  c->fov_pf_launches = c->fov_pf_pairs = 0;
 
   // The states each chunk is checked against.  kete re-propagates all states
   // at each chunk (small steps, checks.rs:74-85) and every 50 dt_limit from a
@@ -233,7 +246,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
 
     // propagate states (in place) to jd_target, then set their epochs
     auto propagate = [&](double* const* x, double* xjd, double jd_target) -> int32_t {
-      HIPCHK(hipMemcpyAsync(d_scalar, &jd_target, 8, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(fill_f64, dim3(1), dim3(64), 0, s, d_scalar, (int64_t)1, jd_target);
       const double* xin[6] = {x[0], x[1], x[2], x[3], x[4], x[5]};
       int32_t rr = kh_propagate_n_body_dev(c, n, xjd, xin, 10, frame, d_scalar, 0, include_asteroids,
                                            nullptr, 1e-6,
@@ -245,6 +258,7 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
 
     double jd = jd0, big_epoch_ref = jd0;
     bool overflow = false;
+    int64_t tile = 0;  // prefilter launches so far (timed while < n_timed)
     int64_t w0 = -1, w1 = -1;  // epochs [w0, w1) are stored in m_out
     for (size_t ci_ = 0; ci_ < chunks.size(); ++ci_) {
       auto& ch = chunks[ci_];
@@ -307,48 +321,54 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
           if (r) return r;
         }
       }
-      // phase 1: prefilter over (object, field) pairs of this chunk
-      hipLaunchKernelGGL(zero_list_counts, dim3(1), dim3(64), 0, s, c->d_counts);
-      FovPrefilter pf{};
-      pf.n = n;
-      pf.jd = sjd;
-      for (int k = 0; k < 6; ++k) pf.x[k] = sx[k];
-      pf.fields = d_fields + f0;
-      pf.nf = (int32_t)(f1 - f0);
-      pf.dt_limit = dt_limit;
-      pf.n_cand = c->d_counts;
-      pf.cap = cand_cap;
-      pf.cand_field = cand_field;
-      pf.cand_kind = cand_kind;
-      pf.cand_obj = cand_obj;
-      if (launch_fov_prefilter(pf, s)) return kh_fail(KH_E_HIP, "prefilter launch failed");
-      // phase 2: two-body checks of the candidates; those whose field is
-      // more than dt_limit from the states' epoch join the n-body list
-      FovTwoBody tb{};
-      tb.n_cand = c->d_counts;
-      tb.cand_max = c->d_counts + 3;
-      tb.cand_cap = cand_cap;
-      tb.jd = sjd;
-      for (int k = 0; k < 6; ++k) tb.x[k] = sx[k];
-      tb.fields = d_fields + f0;
-      tb.field_base = (int32_t)f0;
-      tb.dt_limit = dt_limit;
-      tb.cand_field = cand_field;
-      tb.cand_kind = cand_kind;
-      tb.cand_obj = cand_obj;
-      tb.n_hit = d_nhit;
-      tb.hit_cap = hit_cap;
-      tb.hit_field = h_field;
-      tb.hit_obj = h_obj;
-      for (int k = 0; k < 7; ++k) tb.hit_state[k] = h_state[k];
-      tb.n_nb = c->d_counts + 2;
-      tb.nb_cap = nb_cap;
-      tb.nb_field = nb_field;
-      tb.nb_obj = nb_obj;
-      tb.g_jd0 = g_jd0;
-      tb.g_jdf = g_jdf;
-      for (int k = 0; k < 6; ++k) tb.g_x[k] = g_x[k];
-      if (launch_fov_two_body(tb, s)) return kh_fail(KH_E_HIP, "two-body launch failed");
+      for (int64_t t0 = f0; t0 < f1; t0 += FOV_TILE_FIELDS, ++tile) {
+        const int64_t t1 = std::min<int64_t>(f1, t0 + FOV_TILE_FIELDS);
+        // phase 1: prefilter over (object, field) pairs of this tile
+        hipLaunchKernelGGL(zero_list_counts, dim3(1), dim3(64), 0, s, c->d_counts);
+        FovPrefilter pf{};
+        pf.n = n;
+        pf.jd = sjd;
+        for (int k = 0; k < 6; ++k) pf.x[k] = sx[k];
+        pf.fields = d_fields + t0;
+        pf.nf = (int32_t)(t1 - t0);
+        pf.dt_limit = dt_limit;
+        pf.n_cand = c->d_counts;
+        pf.cap = cand_cap;
+        pf.cand_field = cand_field;
+        pf.cand_kind = cand_kind;
+        pf.cand_obj = cand_obj;
+        if (tile < n_timed) HIPCHK(hipEventRecord(c->fov_ev[2 * tile], s));
+        if (launch_fov_prefilter(pf, s)) return kh_fail(KH_E_HIP, "prefilter launch failed");
+        if (tile < n_timed) HIPCHK(hipEventRecord(c->fov_ev[2 * tile + 1], s));
+        if (tile < n_timed) c->fov_pf_pairs += n * (t1 - t0);
+        // phase 2: two-body checks of the candidates; those whose field is
+        // more than dt_limit from the states' epoch join the n-body list
+        FovTwoBody tb{};
+        tb.n_cand = c->d_counts;
+        tb.cand_max = c->d_counts + 3;
+        tb.cand_cap = cand_cap;
+        tb.jd = sjd;
+        for (int k = 0; k < 6; ++k) tb.x[k] = sx[k];
+        tb.fields = d_fields + t0;
+        tb.field_base = (int32_t)t0;
+        tb.dt_limit = dt_limit;
+        tb.cand_field = cand_field;
+        tb.cand_kind = cand_kind;
+        tb.cand_obj = cand_obj;
+        tb.n_hit = d_nhit;
+        tb.hit_cap = hit_cap;
+        tb.hit_field = h_field;
+        tb.hit_obj = h_obj;
+        for (int k = 0; k < 7; ++k) tb.hit_state[k] = h_state[k];
+        tb.n_nb = c->d_counts + 2;
+        tb.nb_cap = nb_cap;
+        tb.nb_field = nb_field;
+        tb.nb_obj = nb_obj;
+        tb.g_jd0 = g_jd0;
+        tb.g_jdf = g_jdf;
+        for (int k = 0; k < 6; ++k) tb.g_x[k] = g_x[k];
+        if (launch_fov_two_body(tb, s)) return kh_fail(KH_E_HIP, "two-body launch failed");
+      }
     }
     // the groups ran without a host round trip; the list sizes are checked
     // once, here, and an overflow reruns the call with larger lists
@@ -385,7 +405,20 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
       if (r) return r;
       if (launch_fov_nb_finish(nb, s)) return kh_fail(KH_E_HIP, "n-body finish failed");
     }
-    if (overflow) continue;
+    if (overflow) {
+      c->fov_pf_pairs = 0;
+      continue;
+    }
+    {
+      float ms = 0.0f;
+      double sum = 0.0;
+      for (int64_t k = 0; k < n_timed; ++k) {
+        HIPCHK(hipEventElapsedTime(&ms, c->fov_ev[2 * k], c->fov_ev[2 * k + 1]));
+        sum += ms;
+      }
+      c->fov_pf_ms = sum;
+      c->fov_pf_launches = n_timed;
+    }
     unsigned long long total = 0;
     HIPCHK(hipMemcpyAsync(&total, d_nhit, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -428,6 +461,15 @@ int32_t kh_fov_state_check(kh_ctx* c, int64_t n, double jd0, const double* const
     return KH_OK;
   }
   return kh_fail(KH_E_HIP, "candidate lists overflowed twice");
+}
+
+int32_t kh_fov_last_prefilter(kh_ctx* c, double* ms, int64_t* launches, int64_t* pairs) {
+  if (!c || !ms || !launches || !pairs) return kh_fail(KH_E_ARG, "null argument");
+  if (c->fov_pf_ms < 0.0) return kh_fail(KH_E_ARG, "no completed fov_state_check call");
+  *ms = c->fov_pf_ms;
+  *launches = c->fov_pf_launches;
+  *pairs = c->fov_pf_pairs;
+  return KH_OK;
 }
 
 int32_t kh_fov_spk_check(kh_ctx* c, int64_t n_ids, const int64_t* obj_ids, int32_t frame,

@@ -198,6 +198,10 @@ struct FieldDev {
   double pad;
 };
 
+// fields one prefilter launch stages in LDS (160 B each); the host splits
+// larger groups into tiles of this size
+constexpr int FOV_MAX_TILE_FIELDS = 256;
+
 struct FovPrefilter {
   int64_t n;
   const double* jd;

@@ -91,6 +91,8 @@ def _configure(L):
     D, P, I64, I32 = C.c_double, C.POINTER(C.c_double), C.c_int64, C.c_int32
     PP = C.POINTER(P)
     L.ko_const.restype = D
+    L.ko_set_radau_epsilon.restype = D
+    L.ko_set_radau_epsilon.argtypes = [D]
     L.ko_const.argtypes = [C.c_char_p]
     L.ko_newton_square_root_test.argtypes = [D, D, P]
     L.ko_analytic_2_body.argtypes = [D, P, P, P, P]
@@ -204,6 +206,12 @@ def spk21_record_eval(record, n_coef, js):
     p, v = np.zeros(3), np.zeros(3)
     st = lib().ko_spk21_record_eval(_d(rec), n_coef, js, _d(p), _d(v))
     return st, p, v
+
+
+def radau_epsilon(eps=0.0):
+    """Set the oracle's Radau tolerance (kete's 1e-6 unless changed);
+    returns the previous value (eps <= 0 only reads it)."""
+    return lib().ko_set_radau_epsilon(float(eps))
 
 
 def spk_state(naif_id, jd, center=0, frame=FRAME_EQUATORIAL):

@@ -944,7 +944,14 @@ static void radau_tables(void) {
 }
 
 #define MIN_RATIO 0.25
-#define RADAU_EPSILON 1e-6
+/* kete's EPSILON (radau.rs:71) unless a caller sets another tolerance for a
+ * labelled run (set before a batch call, not during one). */
+static double RADAU_EPSILON = 1e-6;
+double ko_set_radau_epsilon(double eps) {
+    const double old = RADAU_EPSILON;
+    if (eps > 0.0) RADAU_EPSILON = eps;
+    return old;
+}
 #define MIN_STEP 0.005
 
 typedef int (*force_fn)(double t, const double *x, const double *v, int exact,

@@ -251,6 +251,10 @@ int ko_chain_observe_batch(int64_t n, const double *jd0, const double *const *xy
 void ko_fov_trace_set(double *buf, int64_t cap);
 int64_t ko_fov_trace_len(void);
 
+/* Radau tolerance used by every oracle integration (1e-6, kete's EPSILON,
+ * by default); returns the previous value; eps <= 0 only reads it. */
+double ko_set_radau_epsilon(double eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,3 +136,36 @@ def test_fov_empty_and_far_fields(ctx, workload):
     # a field 200 days after the states' epoch exercises the re-propagation
     far = fov_state_check(batch, None, fields=(fjd[:1] + 200.0, obs[:, :1], nrm[:1]))
     assert far.field.shape == far.obj.shape
+
+
+def test_fov_group_wider_than_one_tile(ctx, oracle_spk):
+    """1000 fields within 2*dt_limit of the first: one group of kete's
+    schedule (checks.rs:34-57), checked in four LDS tiles of at most 256
+    fields.  Fields up to 2.95 d from the states' epoch take the linear
+    check, the later ones the two-body margin and the n-body re-check; the
+    visible set must equal the oracle's (edge exemption as above)."""
+    xyz, fjd, obs, pts, h = syn.fov_workload(n=2000, n_fields=1000, span_days=5.9, aimed=0.25,
+                                             seed_index=6)
+    w = math.radians(6.9)
+    nrm = np.array([oracle_spk.rect_normals(p, 0.0, w, w).reshape(12) for p in pts])
+    assert fjd.max() - fjd.min() < 6.0
+    n = xyz.shape[1]
+    batch = ka.StateArray(np.full(n, syn.EPOCH_JD), xyz)
+    print("fov: 2000 states x 1000 fields in one group", flush=True)
+    hits = fov_state_check(batch, None, dt_limit=3.0, fields=(fjd, obs, nrm))
+    ms, launches, pairs = ctx.last_fov_prefilter()
+    assert launches == 4 and pairs == n * 1000 and ms > 0.0
+    of, oo, os_ = oracle_spk.fov_state_check(syn.EPOCH_JD, xyz, fjd, obs.T.copy(), nrm,
+                                             dt_limit=3.0, cap=100000)
+    assert len(of) > 2000
+    assert np.any(fjd[of] - syn.EPOCH_JD > 3.0) and np.any(fjd[of] - syn.EPOCH_JD < 3.0)
+    g = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(hits.field, hits.obj))}
+    r = {(int(f), int(i)): k for k, (f, i) in enumerate(zip(of, oo))}
+    for key in set(g) ^ set(r):
+        f, i = key
+        st = os_[r[key]] if key in r else hits.state[:, g[key]]
+        assert edge_distance(nrm[f], obs[:, f], st) < 1e-9, key
+    common = sorted(set(g) & set(r))
+    gs = np.array([hits.state[:, g[k]] for k in common])
+    rs = np.array([os_[r[k]] for k in common])
+    assert np.abs(gs[:, 1:4] - rs[:, 1:4]).max() < 1e-9

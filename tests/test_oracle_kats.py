@@ -148,6 +148,26 @@ def test_radau_matches_kepler():
         assert np.linalg.norm(pr - pk) < 1e-7
 
 
+def test_radau_epsilon_setting():
+    """A tighter Radau tolerance (the labelled epsilon-1e-9 run) takes more
+    steps, about (1e3)^(1/7) = 2.7x (the step scales as epsilon^(1/7)), and
+    stays on the Kepler solution; kete's 1e-6 is restored afterwards."""
+    p, v = np.array([1.2, 0.0, 0.0]), np.array([0.0, 0.019, 0.001])
+    st2, pk, _ = o.analytic_2_body(300.0, p, v)
+    _, p6, _, c6 = o.radau_central(0.0, 300.0, p, v)
+    old = o.radau_epsilon(1e-9)
+    try:
+        assert old == 1e-6
+        _, p9, _, c9 = o.radau_central(0.0, 300.0, p, v)
+    finally:
+        o.radau_epsilon(old)
+    assert o.radau_epsilon() == 1e-6
+    assert 2.0 < c9.n_acc / c6.n_acc < 3.5
+    assert np.linalg.norm(p9 - pk) < 1e-12 and np.linalg.norm(p6 - pk) < 1e-12
+    _, p6b, _, _ = o.radau_central(0.0, 300.0, p, v)
+    assert np.array_equal(p6b, p6)
+
+
 def test_chebyshev_closed_forms():
     """T_k(cos t) = cos kt and dT_k/dx = k U_{k-1} = k sin kt / sin t."""
     n = 14
