@@ -17,6 +17,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkete_oracle.so")
+# the same restatement built with FMA contraction (-ffp-contract=fast): the
+# rounding yardstick of the long-arc parity tests (use with use_library)
+FMA_LIB_PATH = os.path.join(HERE, "libkete_oracle_fma.so")
 
 OK, CONVERGENCE, VALUE_ERROR, DAF_LIMITS, UNKNOWN_FRAME, IO_ERROR, IMPACT = range(7)
 FRAME_EQUATORIAL, FRAME_ECLIPTIC = 1, 17

@@ -19,6 +19,10 @@ VEL_TOL = 1e-11
 JD0 = syn.EPOCH_JD
 
 
+def oracle_spk_path():
+    return syn.ephemeris_path()
+
+
 @pytest.fixture(scope="module")
 def mix():
     xyz = syn.n_body_workload(n=1200, neo_fraction=0.25, seed_index=2)
@@ -83,15 +87,50 @@ def test_three_lane_error_semantics(ctx, oracle_spk, mix):
 
 
 def test_uneven_spans_parity(ctx, oracle_spk):
-    xyz = syn.n_body_workload(n=400, neo_fraction=0.1, seed_index=3)
-    span = syn.uneven_spans(400) / 4.0  # keep the oracle fast: up to 912 d
+    """Config 3 at its own spans: 4096 objects, |span| log-uniform in [1,
+    3650] d with random signs.  The GPU and the oracle run the same
+    algorithm with different roundings (FMA contraction, operation order),
+    and the adaptive Radau decisions (step sizes from a noisy b6, the
+    corrector's sweep count) turn last-ulp differences into integration-
+    error-sized ones, which planetary encounters on the 10-year NEO arcs
+    amplify.  The yardstick is therefore the oracle against itself: the same
+    restatement built with -ffp-contract=fast (a second, equally valid
+    rounding).  Per decade of |span| the GPU's differences from the default
+    oracle must be distributed like the FMA build's: quantiles 50/90/99 % at
+    most 2x, the maximum at most 5x, plus 1e-11 AU (the corrector's
+    convergence level: one sweep more or fewer moves a short arc by up to
+    1.1e-11 AU).  The factors bound the sampling spread of those statistics,
+    measured on 20000 objects in 4096-object chunks (quantile ratios 0.87 -
+    1.66, maximum 0.41 - 4.2).  Statuses must be identical."""
+    n = 4096
+    xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=3)
+    span = syn.uneven_spans(n)
     jf = JD0 + span
+    print(f"uneven spans: {n} objects, |span| {np.abs(span).min():.1f} - "
+          f"{np.abs(span).max():.1f} d", flush=True)
     g, r = run_both(ctx, oracle_spk, xyz, jf)
+    import oracle
+    with oracle.use_library(oracle.FMA_LIB_PATH):
+        assert oracle.spk_load(oracle_spk_path()) == 0
+        f = oracle.batch_n_body(JD0, xyz, jf)
     assert np.array_equal(g["status"], r["status"])
-    ok = g["status"] == 0
-    dp = np.abs(g["out"][:3, ok] - r["out"][:3, ok]).max()
-    # tolerance scales with arc length beyond one year
-    assert dp < POS_TOL * 3, dp
+    assert np.array_equal(f["status"], r["status"])
+    ok = r["status"] == 0
+    assert ok.mean() > 0.99
+    dg = np.linalg.norm(g["out"][:3] - r["out"][:3], axis=0)
+    df = np.linalg.norm(f["out"][:3] - r["out"][:3], axis=0)
+    a = np.abs(span)
+    for lo, hi in ((1, 10), (10, 100), (100, 1000), (1000, 3651)):
+        s = ok & (a >= lo) & (a < hi)
+        qg = np.quantile(dg[s], [0.5, 0.9, 0.99])
+        qf = np.quantile(df[s], [0.5, 0.9, 0.99])
+        print(f"  {lo}-{hi} d ({s.sum()}): gpu q50/90/99/max "
+              + " ".join(f"{x:.1e}" for x in (*qg, dg[s].max()))
+              + " | fma build " + " ".join(f"{x:.1e}" for x in (*qf, df[s].max())), flush=True)
+        assert np.all(qg <= 2.0 * qf + 1e-11), (lo, hi)
+        assert dg[s].max() <= 5.0 * df[s].max() + 1e-11, (lo, hi)
+    # the one-year bar holds on the arcs up to a year
+    assert dg[ok & (a <= 366.0)].max() < POS_TOL
 
 
 def test_frames_and_centers(ctx, oracle_spk, mix):
