@@ -170,6 +170,26 @@ def measured_traffic(kernel, workload):
     return float(rec["bytes_per_launch"])
 
 
+def executed_roofline(kernel, workload, k_ms):
+    """FP64 rate of the instructions the kernel executes (64 lanes x (2 FMA
+    + MUL + ADD) per wave instruction, weighted by the active-lane fraction)
+    from the committed counter pass on this workload
+    (profiles/valu_counts.json) over this run's kernel time; None when that
+    workload was not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "valu_counts.json")) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload") != workload:
+        return None
+    achieved = rec["executed_flop_per_launch"] / (k_ms / 1000.0) / 1e12
+    return {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "flop_per_launch": rec["executed_flop_per_launch"], "kernel": kernel,
+            "source": rec["source"]}
+
+
 def radau_roofline(cnt, k_ms, workload, include_asteroids=False):
     n_force, n_sweep, n_acc = (int(cnt[:, k].sum()) for k in range(3))
     f_force = F_FORCE + (F_FORCE_AST if include_asteroids else 0.0)
@@ -223,6 +243,7 @@ def run_nbody(args, uneven=False, epsilon=1e-6, extras=True):
     if epsilon != 1e-6:
         workload += f"_eps{epsilon:g}"
     roof = radau_roofline(cnt, k_ms, workload)
+    roof_x = executed_roofline("radau_nbody_kernel", workload, k_ms)
     log(f"[bench] {ms_per_step:.1f} ms/step, kernel {k_ms:.1f} ms, {value:.3e} obj-yr/s, "
         f"{roof['achieved']:.2f} TFLOP/s FP64 ({100 * roof['frac']:.1f}% of peak)")
     res_gpu = t["out"].cpu().numpy()
@@ -349,6 +370,7 @@ def run_nbody(args, uneven=False, epsilon=1e-6, extras=True):
                    "seed": syn.SEED_BASE + seed_index,
                    "launch": {"block": block, "grid": grid, "mode": used_mode}},
         "roofline": roof,
+        "roofline_executed": roof_x,
         "cpu_baseline": cpu,
         "parity": parity,
         "status_counts": {str(k): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
@@ -518,6 +540,9 @@ def run_adjoint(args):
                          "kernel_ms": k_ms, "flop_per_launch": flop,
                          "work": {"force_evals": n_force, "sweeps": n_sweep,
                                   "accepted_steps": n_acc}},
+            "roofline_executed": executed_roofline(
+                "radau_partials_wide_kernel+partials_observe_kernel",
+                "orbit_fit_gradients_4096_neo_x_60_obs_2yr", k_ms),
             "cpu_baseline": cpu,
             "finite_differences": fd, "failed_objects": n_bad,
             "accuracy_vs_frozen_step_fd": frozen}
@@ -701,7 +726,9 @@ def run_two_body(args):
                          "flop_per_launch": flop,
                          "transcendentals_per_launch": 7 * solves + 6 * n_newton,
                          "newton_iterations_per_solve": n_newton / solves,
-                         "hbm_write_gbs": bytes_out / (k_ms / 1000.0) / 1e9}}
+                         "hbm_write_gbs": bytes_out / (k_ms / 1000.0) / 1e9},
+            "roofline_executed": executed_roofline(
+                "two_body_kernel", "universal_kepler_1e5_states_x_64_epochs", k_ms)}
     log(f"[bench] {ms:.3f} ms/step, kernel {k_ms:.3f} ms, {solves / (ms / 1000.0):.3e} "
         f"solves/s, {n_newton / solves:.2f} Newton iterations per solve, "
         f"{achieved:.2f} TFLOP/s")
@@ -764,7 +791,9 @@ def run_fov(args):
     pf_gbs = pf_bytes / (pf_ms / 1000.0) / 1e9
     pf_tflops = F_PREFILTER * pf_pairs / (pf_ms / 1000.0) / 1e12
     roof = {"bound": "fp64-valu", "achieved": pf_tflops, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": pf_tflops / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": pf_tflops / FP64_PEAK_TFLOPS,
+            "traffic": measured_traffic("fov_prefilter_kernel",
+                                        "fov_rect_6.9deg_1e6_states_x_3000_fields"),
             "kernel": "fov_prefilter_kernel", "kernel_ms": pf_ms / max(pf_launches, 1),
             "launches": pf_launches, "pairs": pf_pairs,
             "hbm": {"achieved": pf_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -778,7 +807,11 @@ def run_fov(args):
             "config": {"workload": "fov_rect_6.9deg_1e6_states_x_3000_fields", "states": n,
                        "fields": nf, "dt_limit": 3.0,
                        "schedule": os.environ.get("KH_FOV_SCHEDULE", "continuous")},
-            "roofline": roof, "hits": len(hits)}
+            "roofline": roof,
+            "roofline_executed": executed_roofline(
+                "fov_prefilter_kernel", "fov_rect_6.9deg_1e6_states_x_3000_fields",
+                pf_ms / max(pf_launches, 1)),
+            "hits": len(hits)}
     if not args.no_cpu:
         # kete's schedule on the CPU restatement, a bounded sample of the
         # states against all fields; the visible sets must be identical
@@ -864,6 +897,8 @@ def run_moid(args):
                          "work": {"kepler_solves": solves, "newton_iterations": newton,
                                   "nm_iterations_median": float(np.median(
                                       d_it.cpu().numpy()))}},
+            "roofline_executed": executed_roofline(
+                "moid_grid_kernel+moid_nm_kernel", "moid_1e6_objects_vs_earth", k_ms),
             "failed": int((st != 0).sum())}
     log(f"[bench] moid {ms:.2f} ms/step, kernels {k_ms:.2f} ms, {n / (ms / 1000.0):.3e} pairs/s, "
         f"{solves / n:.1f} solves and {newton / max(solves, 1):.2f} Newton iterations per solve, "
@@ -945,6 +980,8 @@ def run_long(args):
                          "work": {"force_evals": int(cnt[:, 0].sum()),
                                   "sweeps": int(cnt[:, 1].sum()),
                                   "accepted_steps": int(cnt[:, 2].sum())}},
+            "roofline_executed": executed_roofline(
+                "radau_long_kernel", "n_body_long_1e5_objects_1yr_batch10", k_ms),
             "failed_batches": int((st != 0).sum())}
     log(f"[bench] long {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {value:.3e} obj-yr/s, "
         f"{achieved:.2f} TFLOP/s")
