@@ -96,6 +96,8 @@ def _configure(L):
     L.ko_const.restype = D
     L.ko_set_radau_epsilon.restype = D
     L.ko_set_radau_epsilon.argtypes = [D]
+    L.ko_rk45_test.argtypes = [C.c_int, D, D, D, D, P, C.POINTER(I64)]
+    L.ko_compute_state_transition.argtypes = [P, P, D, D, D, C.c_int, P, P, P, C.POINTER(I64)]
     L.ko_const.argtypes = [C.c_char_p]
     L.ko_newton_square_root_test.argtypes = [D, D, P]
     L.ko_analytic_2_body.argtypes = [D, P, P, P, P]
@@ -209,6 +211,32 @@ def spk21_record_eval(record, n_coef, js):
     p, v = np.zeros(3), np.zeros(3)
     st = lib().ko_spk21_record_eval(_d(rec), n_coef, js, _d(p), _d(v))
     return st, p, v
+
+
+def rk45_test(kind, y0, t0, t1, tol):
+    """kete's RK45 on y' = -y (kind 0) or y' = 2 (kind 1) -> (status, y1, evaluations)."""
+    y1, ne = C.c_double(), C.c_int64()
+    st = lib().ko_rk45_test(int(kind), float(y0), float(t0), float(t1), float(tol),
+                            C.byref(y1), C.byref(ne))
+    return st, y1.value, ne.value
+
+
+STM_FIX_GRAD, STM_FIX_ACCEL = 1, 2
+
+
+def compute_state_transition(pos, vel, jd0, jd1, central_mass=1.0, fix=0):
+    """kete's compute_state_transition (state_transition.rs:52-112) ->
+    (status, final pos, final vel, the 6x6 matrix kete returns, evaluations).
+    kete's matrix is the transpose of d x(jd1) / d x(jd0).  fix: bits
+    STM_FIX_GRAD / STM_FIX_ACCEL replace the two defects of its arithmetic
+    (the cross-check's diagnosis; 0 is kete's own)."""
+    p, v = _v3(pos), _v3(vel)
+    po, vo, m = np.zeros(3), np.zeros(3), np.zeros(36)
+    ne = C.c_int64()
+    st = lib().ko_compute_state_transition(_d(p), _d(v), float(jd0), float(jd1),
+                                           float(central_mass), int(fix), _d(po), _d(vo),
+                                           _d(m), C.byref(ne))
+    return st, po, vo, m.reshape(6, 6), ne.value
 
 
 def radau_epsilon(eps=0.0):

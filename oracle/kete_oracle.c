@@ -2425,3 +2425,211 @@ void ko_moid_batch(int64_t n, const double *jd_a, const double *const *xyz_a, in
     moid_ctx c = {jd_a, jd_b, xyz_a, xyz_b, frame, out, grid_min, status, iters};
     parallel_for(n, 16, n_threads, moid_range, &c);
 }
+
+/* ------------------------------------------------------------------------ */
+/* RK4(5) Fehlberg and kete's two-body STM                                   */
+/* (propagation/runge_kutta.rs:33-147, state_transition.rs:6-112,            */
+/*  acceleration.rs:254-290)                                                 */
+/* ------------------------------------------------------------------------ */
+#define RK_MAX_DIM 42
+
+typedef int (*rk_fn)(double t, const double *y, double *dy, int dim, void *meta);
+
+typedef struct {
+    rk_fn f;
+    void *meta;
+    int dim;
+    double t, tf, tol;
+    double y[RK_MAX_DIM], der[RK_MAX_DIM];
+    int64_t n_eval;
+} rk45;
+
+static int rk_eval(rk45 *I, double t, const double *y, double *dy) {
+    I->n_eval++;
+    return I->f(t, y, dy, I->dim, I->meta);
+}
+
+/* RK45Integrator::step (runge_kutta.rs:33-117): the Fehlberg stages with
+ * k1 = h * der; a step whose error norm exceeds tol is retried at the
+ * shrunk size (kete recurses; a loop here); a zero error doubles the next
+ * step. */
+static int rk_step(rk45 *I, double h, double *next) {
+    const int D = I->dim;
+    double k2[RK_MAX_DIM], k3[RK_MAX_DIM], k4[RK_MAX_DIM], k5[RK_MAX_DIM], k6[RK_MAX_DIM];
+    double tmp[RK_MAX_DIM];
+    for (;;) {
+        if (h < 1e-30) return KO_CONVERGENCE;   /* step_size < 1e-30 */
+        const double *y = I->y, *d = I->der;
+        for (int i = 0; i < D; ++i) tmp[i] = y[i] + d[i] * (h / 4.0);
+        if (rk_eval(I, I->t + h / 4.0, tmp, k2)) return KO_CONVERGENCE;
+        for (int i = 0; i < D; ++i) k2[i] *= h;
+        for (int i = 0; i < D; ++i) tmp[i] = y[i] + d[i] * (3.0 * h / 32.0) + k2[i] * (9.0 / 32.0);
+        if (rk_eval(I, I->t + h * (3.0 / 8.0), tmp, k3)) return KO_CONVERGENCE;
+        for (int i = 0; i < D; ++i) k3[i] *= h;
+        for (int i = 0; i < D; ++i)
+            tmp[i] = y[i] + d[i] * (h * 1932.0 / 2197.0) - k2[i] * (7200.0 / 2197.0) +
+                     k3[i] * (7296.0 / 2197.0);
+        if (rk_eval(I, I->t + h * (12.0 / 13.0), tmp, k4)) return KO_CONVERGENCE;
+        for (int i = 0; i < D; ++i) k4[i] *= h;
+        for (int i = 0; i < D; ++i)
+            tmp[i] = y[i] + d[i] * (h * 439.0 / 216.0) - k2[i] * 8.0 + k3[i] * (3680.0 / 513.0) -
+                     k4[i] * (845.0 / 4104.0);
+        if (rk_eval(I, I->t + h, tmp, k5)) return KO_CONVERGENCE;
+        for (int i = 0; i < D; ++i) k5[i] *= h;
+        for (int i = 0; i < D; ++i)
+            tmp[i] = y[i] - d[i] * (h * 8.0 / 27.0) + k2[i] * 2.0 - k3[i] * (3544.0 / 2565.0) +
+                     k4[i] * (1859.0 / 4104.0) - k5[i] * (11.0 / 40.0);
+        if (rk_eval(I, I->t + h * 0.5, tmp, k6)) return KO_CONVERGENCE;
+        for (int i = 0; i < D; ++i) k6[i] *= h;
+        double err2 = 0.0;
+        for (int i = 0; i < D; ++i) {
+            const double e = d[i] * (h / -360.) + k3[i] * (128.0 / 4275.0) +
+                             k4[i] * (2197.0 / 75240.0) - k5[i] / 50.0 - k6[i] * (2.0 / 55.0);
+            err2 += e * e;
+        }
+        const double trunc_err = sqrt(err2);
+        double new_step;
+        if (trunc_err == 0.0) {
+            new_step = 2.0 * h;
+        } else {
+            new_step = 0.9 * h * pow(I->tol / trunc_err, 1.0 / 5.0);
+            if (trunc_err > I->tol) {
+                h = new_step;
+                continue;
+            }
+        }
+        for (int i = 0; i < D; ++i)
+            I->y[i] += d[i] * (h * 16.0 / 135.0) + k3[i] * (6656.0 / 12825.0) +
+                       k4[i] * (28561.0 / 56430.0) - k5[i] * (9.0 / 50.0) + k6[i] * (2.0 / 55.0);
+        I->t += h;
+        if (rk_eval(I, I->t, I->y, I->der)) return KO_CONVERGENCE;
+        *next = new_step;
+        return KO_OK;
+    }
+}
+
+/* RK45Integrator::integrate (runge_kutta.rs:119-147): the first step tries
+ * the whole interval; later steps are clipped to land on the final time;
+ * the loop ends when |t - tf| <= tol (the same tolerance as the error). */
+static int rk_integrate(rk_fn f, void *meta, int dim, const double *y0, double t0, double tf,
+                        double tol, double *y_out, int64_t *n_eval) {
+    rk45 I;
+    I.f = f;
+    I.meta = meta;
+    I.dim = dim;
+    I.t = t0;
+    I.tf = tf;
+    I.tol = tol;
+    I.n_eval = 0;
+    memcpy(I.y, y0, dim * sizeof(double));
+    if (fabs(t0 - tf) < 1e-30) {
+        memcpy(y_out, y0, dim * sizeof(double));
+        if (n_eval) *n_eval = 0;
+        return KO_OK;
+    }
+    if (rk_eval(&I, t0, I.y, I.der)) return KO_CONVERGENCE;
+    double next = tf - I.t;
+    int st = rk_step(&I, next, &next);
+    /* kete loops without bound; the cap (far beyond any arc here) turns a
+     * time that cannot reach tf within tol into an error instead of a hang */
+    int64_t steps = 0;
+    while (st == KO_OK && fabs(I.t - I.tf) > tol) {
+        if (++steps > 10000000) return KO_CONVERGENCE;
+        if (fabs(I.t - I.tf) < fabs(next)) next = I.tf - I.t;
+        st = rk_step(&I, next, &next);
+    }
+    memcpy(y_out, I.y, dim * sizeof(double));
+    if (n_eval) *n_eval = I.n_eval;
+    return st;
+}
+
+/* Test equations: kete's RK45 tests (runge_kutta.rs:158-197), kind 0
+ * y' = -y and kind 1 y' = 2; kind 2 y' = 5 t^4, whose solution t^5 + c the
+ * fifth-order Fehlberg weights integrate exactly (a wrong weight or node
+ * shows). */
+static int rk_test_fn(double t, const double *y, double *dy, int dim, void *meta) {
+    (void)dim;
+    const int kind = *(const int *)meta;
+    dy[0] = kind == 0 ? -y[0] : (kind == 1 ? 2.0 : 5.0 * t * t * t * t);
+    return 0;
+}
+
+int ko_rk45_test(int kind, double y0, double t0, double t1, double tol, double *y1,
+                 int64_t *n_eval) {
+    return rk_integrate(rk_test_fn, &kind, 1, &y0, t0, t1, tol, y1, n_eval);
+}
+
+/* stm_ivp_eqn (state_transition.rs:6-47) in kete's scaled time tau = k t
+ * (k = GMS_SQRT): y = (pos, vel / k, STM column-major).  fix bits switch
+ * off the two defects of kete's arithmetic (for the cross-check only):
+ *   1: accel_grad (acceleration.rs:267-290) returns
+ *      (r^2 I - 3 p p^T) / (2 r^5) * m, i.e. -1/2 the Jacobian of -m p / r^3;
+ *      with the bit, m (3 p p^T - r^2 I) / r^5;
+ *   2: the trajectory's acceleration is central_accel (acceleration.rs:
+ *      66-80), -p GMS / r^3, in day units although the equation runs in
+ *      tau (and mass_scaling is ignored); with the bit, -m p / r^3. */
+typedef struct {
+    double mass;
+    int fix;
+} stm_meta;
+
+static int stm_fn(double t, const double *y, double *dy, int dim, void *meta) {
+    (void)t;
+    (void)dim;
+    const stm_meta *M = (const stm_meta *)meta;
+    const double p[3] = {y[0], y[1], y[2]};
+    const double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const double ac = (M->fix & 2) ? M->mass : GMS;
+    for (int i = 0; i < 3; ++i) {
+        dy[i] = y[3 + i];
+        dy[3 + i] = -p[i] * pow(r, -3) * ac;
+    }
+    /* accel_grad: kete's form, or the true Jacobian */
+    double G[3][3];
+    const double r2 = r * r, r5 = pow(r, 5);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double e = (i == j ? r2 : 0.0) - 3.0 * p[i] * p[j];
+            G[i][j] = (M->fix & 1) ? -e / r5 * M->mass : e / (2.0 * r5) * M->mass;
+        }
+    /* d STM / d tau = A STM, A = [[0, I], [G, 0]]; STM column-major */
+    const double *S = y + 6;
+    double *dS = dy + 6;
+    for (int c = 0; c < 6; ++c) {
+        const double *col = S + 6 * c;
+        double *dcol = dS + 6 * c;
+        for (int i = 0; i < 3; ++i) dcol[i] = col[3 + i];
+        for (int i = 0; i < 3; ++i) dcol[3 + i] = G[i][0] * col[0] + G[i][1] * col[1] + G[i][2] * col[2];
+    }
+    return 0;
+}
+
+/* compute_state_transition (state_transition.rs:52-112): RK45 at tol 1e-12
+ * over tau from jd0 k to jd1 k.  Returns the final (pos, vel) and the 6x6
+ * matrix kete returns, row-major: diag(1,1,1,1/k,1/k,1/k) S^T
+ * diag(1,1,1,k,k,k) -- the transpose of d x(jd1) / d x(jd0) (kete
+ * transposes the integrated matrix before scaling). */
+int ko_compute_state_transition(const double pos[3], const double vel[3], double jd0,
+                                double jd1, double central_mass, int fix, double out_pos[3],
+                                double out_vel[3], double stm[36], int64_t *n_eval) {
+    double y0[42] = {0}, y1[42];
+    for (int i = 0; i < 3; ++i) {
+        y0[i] = pos[i];
+        y0[3 + i] = vel[i] / GMS_SQRT;
+    }
+    for (int i = 0; i < 6; ++i) y0[6 + 7 * i] = 1.0;
+    stm_meta M = {central_mass, fix};
+    const int st = rk_integrate(stm_fn, &M, 42, y0, jd0 * GMS_SQRT, jd1 * GMS_SQRT, 1e-12, y1,
+                                n_eval);
+    if (st) return st;
+    for (int i = 0; i < 3; ++i) {
+        out_pos[i] = y1[i];
+        out_vel[i] = y1[3 + i] * GMS_SQRT;
+    }
+    const double sa[6] = {1, 1, 1, 1.0 / GMS_SQRT, 1.0 / GMS_SQRT, 1.0 / GMS_SQRT};
+    const double sb[6] = {1, 1, 1, GMS_SQRT, GMS_SQRT, GMS_SQRT};
+    /* S column-major: S[row][col] = y1[6 + 6 col + row]; S^T[i][j] = S[j][i] */
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) stm[6 * i + j] = sa[i] * y1[6 + 6 * i + j] * sb[j];
+    return KO_OK;
+}

@@ -255,6 +255,18 @@ int64_t ko_fov_trace_len(void);
  * by default); returns the previous value; eps <= 0 only reads it. */
 double ko_set_radau_epsilon(double eps);
 
+/* RK45Integrator::integrate (propagation/runge_kutta.rs:119-147) on kete's
+ * two test equations (kind 0: y' = -y, kind 1: y' = 2). */
+int ko_rk45_test(int kind, double y0, double t0, double t1, double tol, double *y1,
+                 int64_t *n_eval);
+/* compute_state_transition (propagation/state_transition.rs:52-112); fix
+ * bits: 1 the true accel_grad, 2 the acceleration in scaled time with the
+ * central mass (0 = kete's arithmetic).  stm: row-major, the matrix kete
+ * returns (the transpose of d x(jd1) / d x(jd0)). */
+int ko_compute_state_transition(const double pos[3], const double vel[3], double jd0,
+                                double jd1, double central_mass, int fix, double out_pos[3],
+                                double out_vel[3], double stm[36], int64_t *n_eval);
+
 #ifdef __cplusplus
 }
 #endif
