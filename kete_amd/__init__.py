@@ -494,10 +494,11 @@ def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppre
         i = int(np.flatnonzero(st != 0)[0])
         raise ValueError(f"object {i}: {STATUS_NAMES.get(int(st[i]), st[i])}")
     if not suppress_impact_errors:
+        from .naming import impact_message
         for i in np.flatnonzero(st == 6):
-            d = batch.desigs[i] if batch.desigs else i
-            print(f"Impact detected between ({d}) <-> ({r['impact_id'][i]}) at time "
-                  f"{r['impact_jd'][i]}", file=sys.stderr)
+            d = batch.desigs[i] if batch.desigs else None
+            print(impact_message(d, int(r["impact_id"][i]), float(r["impact_jd"][i])),
+                  file=sys.stderr)
     res = StateArray(np.full(len(batch), float(jd)), r["out"], batch.frame, batch.center_id,
                      batch.desigs)
     return res if soa else res.to_states()

@@ -178,7 +178,7 @@ def test_error_semantics(ctx, oracle_spk, mix):
     assert np.abs(g["out"] - mix[:, :4]).max() < 1e-15
 
 
-def test_impact_during_arc(ctx, oracle_spk):
+def test_impact_during_arc(ctx, oracle_spk, capsys):
     """An object aimed at the Earth impacts on an exact evaluation."""
     earth, _ = ctx.spk_state(399, [JD0 + 20.0], 10, ka.Frames.Ecliptic)
     e0, _ = ctx.spk_state(399, [JD0], 10, ka.Frames.Ecliptic)
@@ -192,6 +192,12 @@ def test_impact_during_arc(ctx, oracle_spk):
     if r["status"][0] == 6:
         assert g["impact_id"][0] == r["impact_id"][0]
         assert abs(g["impact_jd"][0] - r["impact_jd"][0]) < 1e-6
+        # kete's report on stderr: the body's name and the UTC time
+        from kete_amd.naming import impact_message
+        ka.propagate_n_body(ka.StateArray(np.full(1, JD0), xyz, desigs=["aimed"]), JD0 + 40.0)
+        err = capsys.readouterr().err
+        assert impact_message("aimed", int(g["impact_id"][0]), float(g["impact_jd"][0])) in err
+        assert "<-> (earth)" in err or "<-> (moon)" in err
 
 
 def test_nongrav_models(ctx, oracle_spk, mix):
