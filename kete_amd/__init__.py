@@ -357,15 +357,65 @@ def _as_batch(states):
     return StateArray.from_states(states), False
 
 
+def _per_frame_group(states, fn):
+    """Apply fn(StateArray) -> StateArray to a state list whose members may
+    differ in frame or center (kete handles each state on its own,
+    src/kete/rust/propagation.rs:105-140): one batch per (frame, center),
+    results back in input order.  fn also gets the members' indices."""
+    if isinstance(states, StateArray):
+        return fn(states, np.arange(len(states)))
+    if isinstance(states, State):
+        states = [states]
+    states = list(states)
+    if not states:
+        raise ValueError("empty state list")
+    keys = [(s.frame, s.center_id) for s in states]
+    if len(set(keys)) == 1:
+        return fn(StateArray.from_states(states), np.arange(len(states))).to_states()
+    out = [None] * len(states)
+    for key in dict.fromkeys(keys):
+        idx = np.array([i for i, k in enumerate(keys) if k == key])
+        res = fn(StateArray.from_states([states[i] for i in idx]), idx).to_states()
+        for j, i in enumerate(idx):
+            out[i] = res[j]
+    return out
+
+
 def propagate_two_body(states, jd, observer_pos=None):
     """kete ``propagate_two_body`` (``src/kete/rust/kepler.rs:68-108``): two-body
     motion about the Sun to ``jd``, with an optional one-step light delay as
     seen from ``observer_pos``.  Failed objects come back as NaN states."""
-    batch, soa = _as_batch(states)
-    out, _st = get_context().two_body(batch, [jd], observer_pos)
-    res = StateArray(np.full(len(batch), float(jd)), out[:, 0, :], batch.frame, batch.center_id,
-                     batch.desigs)
-    return res if soa else res.to_states()
+    def run(batch, idx):
+        out, _st = get_context().two_body(batch, [jd], observer_pos)
+        return StateArray(np.full(len(batch), float(jd)), out[:, 0, :], batch.frame,
+                          batch.center_id, batch.desigs)
+    return _per_frame_group(states, run)
+
+
+# largest propagate_n_body_long batch: the batch and its nine planets in one
+# wavefront of 64 lanes (kete accepts any size)
+LONG_MAX_BATCH = 55
+
+
+def _planet_array(planet_states, jd0):
+    """The nine planet states of propagate_n_body_long as [9, 6] Sun-centred
+    equatorial rows.  kete passes them to the integrator as given
+    (propagation/mod.rs:186-190), i.e. it assumes equatorial input; states
+    in the ecliptic frame are rotated here instead of being mixed in."""
+    pb, _ = _as_batch(planet_states)
+    if len(pb) != 9:
+        raise ValueError("Input planet states must contain the correct number of states.")
+    if pb.jd[0] != jd0:
+        raise ValueError("Planet states JD must match JD of input state.")
+    pl = np.empty((9, 6))
+    c, sn = np.cos(0.40909280422232897), np.sin(0.40909280422232897)
+    for k in range(9):
+        p, v = pb.xyz[:3, k], pb.xyz[3:, k]
+        if pb.frame == FRAME_ECLIPTIC:
+            p = np.array([p[0], c * p[1] - sn * p[2], sn * p[1] + c * p[2]])
+            v = np.array([v[0], c * v[1] - sn * v[2], sn * v[1] + c * v[2]])
+        pl[k, :3], pl[k, 3:] = p, v
+    return pl
 
 
 def propagate_n_body_long(states, jd_final, planet_states=None, non_gravs=None, batch_size=10):
@@ -387,22 +437,11 @@ def propagate_n_body_long(states, jd_final, planet_states=None, non_gravs=None, 
         raise ValueError("All input states must have the same JD")
     if non_gravs is not None and len(non_gravs) != n:
         raise ValueError("Number of non-grav models doesnt match the number of provided objects.")
+    if not 1 <= int(batch_size) <= LONG_MAX_BATCH:
+        raise ValueError(f"batch_size must be 1 .. {LONG_MAX_BATCH} (a batch and its nine "
+                         "planets share one wavefront)")
+    pl = None if planet_states is None else _planet_array(planet_states, jd0)
     ctx = get_context()
-    pl = None
-    if planet_states is not None:
-        pb, _ = _as_batch(planet_states)
-        if len(pb) != 9:
-            raise ValueError("Input planet states must contain the correct number of states.")
-        if pb.jd[0] != jd0:
-            raise ValueError("Planet states JD must match JD of input state.")
-        pl = np.empty((9, 6))
-        for k in range(9):
-            p, v = pb.xyz[:3, k], pb.xyz[3:, k]
-            if pb.frame == FRAME_ECLIPTIC:
-                c, sn = np.cos(0.40909280422232897), np.sin(0.40909280422232897)
-                p = np.array([p[0], c * p[1] - sn * p[2], sn * p[1] + c * p[2]])
-                v = np.array([v[0], c * v[1] - sn * v[2], sn * v[1] + c * v[2]])
-            pl[k, :3], pl[k, 3:] = p, v
     ng = None
     if non_gravs is not None:
         ng_arr = non_gravs if isinstance(non_gravs, np.ndarray) else nongrav_array(non_gravs, n)
@@ -484,21 +523,28 @@ def propagate_n_body(states, jd, include_asteroids=False, non_gravs=None, suppre
     the loaded SPK, and with ``include_asteroids`` Ceres, Pallas, Vesta,
     Hygiea and Interamnia from their Type 21 segments.  Failed objects come back as NaN states; with
     ``suppress_errors=False`` the first failure raises ``ValueError``."""
-    batch, soa = _as_batch(states)
-    if non_gravs is not None and len(non_gravs) != len(batch):
+    n_all = len(states) if not isinstance(states, State) else 1
+    if non_gravs is not None and len(non_gravs) != n_all:
         raise ValueError("non_gravs must be the same length as states.")
-    r = get_context().n_body(batch, [jd], non_gravs=non_gravs,
-                             include_asteroids=include_asteroids)
-    st = r["status"]
-    if not suppress_errors and np.any(st != 0):
-        i = int(np.flatnonzero(st != 0)[0])
-        raise ValueError(f"object {i}: {STATUS_NAMES.get(int(st[i]), st[i])}")
-    if not suppress_impact_errors:
-        from .naming import impact_message
-        for i in np.flatnonzero(st == 6):
-            d = batch.desigs[i] if batch.desigs else None
-            print(impact_message(d, int(r["impact_id"][i]), float(r["impact_jd"][i])),
-                  file=sys.stderr)
-    res = StateArray(np.full(len(batch), float(jd)), r["out"], batch.frame, batch.center_id,
-                     batch.desigs)
-    return res if soa else res.to_states()
+
+    def run(batch, idx):
+        if non_gravs is None:
+            ng = None
+        elif isinstance(non_gravs, np.ndarray):
+            ng = np.ascontiguousarray(non_gravs[idx])
+        else:
+            ng = [non_gravs[i] for i in idx]
+        r = get_context().n_body(batch, [jd], non_gravs=ng, include_asteroids=include_asteroids)
+        st = r["status"]
+        if not suppress_errors and np.any(st != 0):
+            i = int(np.flatnonzero(st != 0)[0])
+            raise ValueError(f"object {int(idx[i])}: {STATUS_NAMES.get(int(st[i]), st[i])}")
+        if not suppress_impact_errors:
+            from .naming import impact_message
+            for i in np.flatnonzero(st == 6):
+                d = batch.desigs[i] if batch.desigs else None
+                print(impact_message(d, int(r["impact_id"][i]), float(r["impact_jd"][i])),
+                      file=sys.stderr)
+        return StateArray(np.full(len(batch), float(jd)), r["out"], batch.frame, batch.center_id,
+                          batch.desigs)
+    return _per_frame_group(states, run)

@@ -9,7 +9,12 @@
 // the observer-to-object vector in the equatorial frame
 // (simult_states.rs:86-120, frames/definitions.rs:337-345).
 //
-// Layout and launch
+// Two kernels compute the same columns.  The default,
+// radau_partials_wide_kernel (below), spreads an object over sixteen lanes:
+// one body of the force model and one column of the state per lane, with
+// analytic force Jacobians, then partials_observe_kernel forms the
+// observables.  radau_partials_kernel (KH_PARTIALS=lane9) is the first
+// design, kept as a second implementation to compare against:
 //  - One lane per (object, parameter): 9 lanes per object.  Every lane runs
 //    the full nominal integration in dual numbers (value, derivative along its
 //    own parameter); all step-size and convergence decisions read the values

@@ -230,3 +230,27 @@ def test_python_surface(ctx):
     with pytest.raises(ValueError):
         ka.propagate_n_body(bad, JD0 + 1.0, suppress_errors=False)
     assert not ka.propagate_n_body(bad, JD0 + 1.0)[0].is_finite
+
+
+def test_mixed_frame_list(ctx, mix):
+    """A state list mixing frames and centers (kete propagates each state on
+    its own): each state's result equals that of a uniform batch."""
+    c, s = np.cos(0.40909280422232897), np.sin(0.40909280422232897)
+    xyz = mix[:, :6].copy()
+    sts = []
+    for k in range(6):
+        p, v = xyz[:3, k].copy(), xyz[3:, k].copy()
+        f = ka.Frames.Ecliptic
+        if k % 2:
+            p = np.array([p[0], c * p[1] - s * p[2], s * p[1] + c * p[2]])
+            v = np.array([v[0], c * v[1] - s * v[2], s * v[1] + c * v[2]])
+            f = ka.Frames.Equatorial
+        sts.append(ka.State(f"m{k}", JD0, p, v, f, 10))
+    out = ka.propagate_n_body(sts, JD0 + 50.0)
+    ref = ka.propagate_n_body(ka.StateArray(np.full(6, JD0), xyz), JD0 + 50.0)
+    for k in range(6):
+        assert out[k].desig == f"m{k}" and out[k].frame == sts[k].frame
+        p = out[k].pos
+        if k % 2:
+            p = np.array([p[0], c * p[1] + s * p[2], -s * p[1] + c * p[2]])
+        assert np.abs(p - ref.xyz[:3, k]).max() < 1e-12
