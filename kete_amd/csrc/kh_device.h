@@ -512,9 +512,13 @@ __device__ __forceinline__ double kep_sin(double x) {
 // Elliptic branch.  sin(b x) and 1 - cos(b x) come from one sin/cos pair at
 // the half angle: sin(b x) = 2 sh ch and 1 - cos(b x) = 2 sh^2, sh the
 // sin(b x / 2) kete's g_2 already takes (the identities change only the
-// rounding of the last bits).
+// rounding of the last bits).  The divisions by b and beta of kete's g1, g2,
+// g3 are multiplications by reciprocals formed once per solve (a division is
+// a ten-instruction sequence on the FP64 pipe; kete's iteration makes five
+// per step): the quotients move by at most an ulp, and the iteration still
+// stops at |f| < 1e-11.
 struct KepEll {
-  double r0, rv0, beta, b, dt;
+  double r0, rv0, beta, b, dt, inv_b, inv_beta;
   __device__ __forceinline__ void half(double x, double& s, double& omc) const {
     double sh, ch;
     kep_sincos(b * x / 2.0, sh, ch);
@@ -524,23 +528,23 @@ struct KepEll {
   __device__ __forceinline__ double f(double x) const {
     double s, omc;
     half(x, s, omc);
-    double g1 = s / b;
-    double g2 = omc / beta;
-    double g3 = (x - g1) / beta;
+    double g1 = s * inv_b;
+    double g2 = omc * inv_beta;
+    double g3 = (x - g1) * inv_beta;
     return r0 * g1 + rv0 * g2 + GMS * g3 - dt;
   }
   __device__ __forceinline__ double d(double x) const {
     double s, omc;
     half(x, s, omc);
-    return r0 * (1.0 - omc) + rv0 * s + GMS * (omc / beta);
+    return r0 * (1.0 - omc) + rv0 * s + GMS * (omc * inv_beta);
   }
   // f and d at one point
   __device__ __forceinline__ void fd(double x, double& fv, double& dv) const {
     double s, omc;
     half(x, s, omc);
-    double g1 = s / b;
-    double g2 = omc / beta;
-    double g3 = (x - g1) / beta;
+    double g1 = s * inv_b;
+    double g2 = omc * inv_beta;
+    double g3 = (x - g1) * inv_beta;
     fv = r0 * g1 + rv0 * g2 + GMS * g3 - dt;
     dv = r0 * (1.0 - omc) + rv0 * s + GMS * g2;
   }
@@ -604,7 +608,7 @@ __device__ __forceinline__ int solve_kepler_universal(double dt, double r0, doub
   if (beta >= PARABOLIC_BETA) {
     double period = GMS * (1.0 / (b * b * b));
     dt = fmod(dt, period * TAU);
-    KepEll k{r0, rv0, beta, b, dt};
+    KepEll k{r0, rv0, beta, b, dt, 1.0 / b, 1.0 / beta};
     double guess = (1.0 / b) * dt / period;
     return newton(k, guess, 1e-11, s);
   } else if (fabs(beta) < PARABOLIC_BETA) {
