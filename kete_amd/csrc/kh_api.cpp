@@ -757,10 +757,24 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   }
   p.n_cu = c->n_cu;
   if (const char* g = getenv("KH_GRID")) p.grid = atoi(g);
-  // plain launch on the shared record grid: up to 8 x n_cu of the slowest
-  // objects run in the latency kernel beside the main launch, those whose
-  // cost outweighs a lane's share of the batch (KH_LATENCY_K sets the cap)
-  p.latency_k = c->spk.shared_grid ? (int64_t)c->n_cu * 8 : 0;
+  // plain launch on the shared record grid: the slowest objects run in the
+  // latency kernel beside the main launch, those whose cost outweighs a
+  // lane's share of the batch, at most a cap (KH_LATENCY_K sets it)
+  // The latency kernel: sixteen lanes per object for batches of uneven arcs
+  // (per-object final epochs; planets only), whose heavy tail it shortens
+  // (2e6 objects over 1-3650 d: 449 ms with up to 32 x n_cu objects in it,
+  // 484 ms with the one-lane variant); the one-lane variant, up to 8 x n_cu
+  // objects, for a common final epoch, where the main launch sets the time
+  // either way and the one-lane variant takes fewer SIMDs from it.
+  // KH_LATENCY=wide or =ilp chooses.
+  {
+    const char* lv = getenv("KH_LATENCY");
+    bool wide = jd_final_stride != 0;
+    if (lv && std::string(lv) == "wide") wide = true;
+    if (lv && std::string(lv) == "ilp") wide = false;
+    p.wide_latency = wide && !include_asteroids;
+    p.latency_k = c->spk.shared_grid ? (int64_t)c->n_cu * (p.wide_latency ? 32 : 8) : 0;
+  }
   if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;
   if (tuned) {
     p.mode = c->tune.mode;

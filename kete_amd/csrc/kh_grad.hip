@@ -711,69 +711,6 @@ __device__ __forceinline__ void nongrav_dir(const double p[3], const double dp[3
   }
 }
 
-// Chebyshev position and derivative (interpolation.rs:25-77, forward T/U
-// recurrences in ascending order, as cheb_both_n) of a 14-coefficient record
-// in LDS, [mid, radius, x[14], y[14], z[14]]; the coefficients are read in
-// chunks of four orders so that few of them are held in registers at once.
-__device__ __forceinline__ void cheb_both_lds(const double* rec, double x, Vec3& p, Vec3& d) {
-  const double* c = rec + 2;
-  const double x2 = 2.0 * x;
-  double vx = c[0] + c[1] * x, vy = c[14] + c[15] * x, vz = c[28] + c[29] * x;
-  double dx = c[1], dy = c[15], dz = c[29];
-  double t2 = 1.0, t1 = x, u2 = 1.0, u1 = x2;
-#pragma unroll
-  for (int k0 = 2; k0 < 14; k0 += 4) {
-    // (chunk boundary)
-#pragma unroll
-    for (int k = k0; k < k0 + 4 && k < 14; ++k) {
-      const double t = x2 * t1 - t2;
-      vx += c[k] * t;
-      vy += c[14 + k] * t;
-      vz += c[28 + k] * t;
-      t2 = t1;
-      t1 = t;
-      const double u = x2 * u1 - u2;
-      const double kk = (double)k;
-      dx += c[k] * u1 * kk;
-      dy += c[14 + k] * u1 * kk;
-      dz += c[28 + k] * u1 * kk;
-      u2 = u1;
-      u1 = u;
-    }
-  }
-  p = Vec3{vx, vy, vz};
-  d = Vec3{dx, dy, dz};
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_row(double v) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
-  return __hiloint2double(hi2, lo2);
-}
-
-// v of lane N of this lane's DPP row (row_newbcast)
-template <int N>
-__device__ __forceinline__ double row_bcast(double v) {
-  return dpp_row<0x150 + N>(v);
-}
-template <int N>
-__device__ __forceinline__ int row_bcast_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x150 + N, 0xF, 0xF, false);
-}
-
-// Sum over the 16 lanes of a row, the same bits in every lane (each step
-// adds a commuted pair): quad_perm [1,0,3,2], quad_perm [2,3,0,1],
-// row_half_mirror, row_mirror.
-__device__ __forceinline__ double row_sum(double v) {
-  v += dpp_row<0xB1>(v);
-  v += dpp_row<0x4E>(v);
-  v += dpp_row<0x141>(v);
-  v += dpp_row<0x140>(v);
-  return v;
-}
-
 template <bool CENTRAL>
 __global__ __launch_bounds__(256) void radau_partials_wide_kernel(
     const SpkDev* __restrict__ spk, const double* __restrict__ coef, GradArgs p) {

@@ -73,6 +73,9 @@ struct NBodyLaunch {
   // mode 0: the first latency_k objects of the order run in the latency
   // kernel on stream2 (fork/join events order it with `stream`)
   int64_t latency_k;
+  // the latency kernel: sixteen lanes per object (planets only) instead of
+  // one lane per object
+  int32_t wide_latency;
   int64_t* split;  // device int64: head of the order the latency kernel takes
   hipStream_t stream2;
   hipEvent_t fork, join;

@@ -22,6 +22,8 @@
 //  - Persistent mode: the grid is sized to the machine and a lane that
 //    finishes an object takes the next one from a device-side counter, so
 //    objects with very different step counts do not leave lanes idle.
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 
 #include "kh_device.h"
@@ -1025,6 +1027,290 @@ __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 // Blocks of `kernel` that are resident at once on the whole device: the
 // persistent launch must not exceed it, or the blocks beyond it start only
 // after the first ones have drained the queue.
+// ---------------------------------------------------------------------------
+// Wide latency variant for the slowest objects (the head of the longest-
+// first order, on the second stream beside the main launch): sixteen lanes
+// per object, one DPP row, four objects per wavefront.  Lane l < 10
+// evaluates body l of kete's PLANETS list (gravity.rs:145-148) -- its
+// Chebyshev record cached in LDS, its Newton, GR and J2 terms and, for the
+// Sun, the non-grav term (add_body_at) -- and lane 10 the Earth-Moon
+// barycentre segment the Earth and Moon lanes add to their positions.  The
+// row sums the terms (row_sum: the same bits in every lane), and every lane
+// of the row then runs the same integrator arithmetic as the one-lane kernel
+// on identical values, so the row takes identical decisions without
+// exchanging them.  A force evaluation thus costs one body's terms and the
+// integrator's bookkeeping per lane instead of the whole force sum: the
+// latency of the slowest objects' serial chains drops, at a sixteenth of
+// the one-lane kernel's objects per wavefront.  Results differ from the
+// one-lane kernel's only by the order in which the body terms are summed.
+// Planets only (the asteroid list runs in the one-lane latency kernel).
+// ---------------------------------------------------------------------------
+constexpr int WIDE_L = 16;
+constexpr int WIDE_EMB = 10;
+
+__global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restrict__ spk,
+                                                         const double* __restrict__ coef,
+                                                         NBodyArgs a) {
+  __shared__ LdsTables lds;
+  __shared__ __align__(16) double s_rec[256 / WIDE_L][WIDE_EMB + 1][44];
+  for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
+    const int r = k / 8, c = k % 8;
+    lds.rdiff[r][c] = RT.rdiff[r][c];
+    if (r == 0) {
+      lds.h[c] = RT.h[c];
+      lds.inv_h[c] = RT.inv_h[c];
+    }
+  }
+  __syncthreads();
+  slot_range(a);
+  const int64_t slot = a.slot_begin + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WIDE_L;
+  if (slot >= a.slot_end) return;
+  const int64_t obj = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
+  if (a.status[obj] != ST_OK) return;
+  const int l = threadIdx.x & (WIDE_L - 1);
+  const int g0 = (threadIdx.x & 63) & ~(WIDE_L - 1);
+  const double eps = a.epsilon;
+
+  double t = a.jd0[obj];
+  const double tf = a.jd_final[a.jd_final_stride ? obj : 0];
+  Vec3 x{a.out[0][obj], a.out[1][obj], a.out[2][obj]};
+  Vec3 v{a.out[3][obj], a.out[4][obj], a.out[5][obj]};
+  const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
+
+  // this lane's body (lanes 0..9) or segment (lane 10); lanes 11..15 none
+  const bool is_body = l < N_PLANETS;
+  const bool is_seg = l <= WIDE_EMB;
+  const BodyDev bd = spk->body[is_body ? l : 0];
+  const int32_t seg = l == WIDE_EMB ? spk->emb_seg : bd.seg;
+  const SegDev sg = spk->seg[seg];
+  const bool need_vel = is_body && bd.kind == 1;
+  const bool cache_rec = spk->all14 != 0 && sg.rsize == 44;
+  int32_t rec_cached = -1;
+  double* const rslot = &s_rec[threadIdx.x / WIDE_L][is_seg ? l : 0][0];
+
+  double h = 0.0, h_next = 0.0;
+  Vec3 a0{0, 0, 0};
+  double B[7][3], G[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
+  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
+  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
+
+  for (;;) {
+    const bool sub = phase == PH_SUB;
+    // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
+    const double s = lds.h[j];
+    double xe[3], ve[3];
+    {
+      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double sp = 1.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        sp *= s;
+        const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bw[d] += B[k][d] * wk;
+          bu[d] += B[k][d] * uk;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
+        double pv = vs[d] + s * h * (as[d] + bu[d]);
+        xe[d] = sub ? px : xs[d];
+        ve[d] = sub ? pv : vs[d];
+      }
+    }
+    const double te = sub ? t + s * h : t;
+
+    // ---- this lane's share of the force evaluation ----
+    n_force++;
+    int32_t lst = ST_OK;
+    Vec3 bp{0.0, 0.0, 0.0}, bv{0.0, 0.0, 0.0};
+    if (is_seg) {
+      if (te < sg.jd_start || te > sg.jd_end) lst = ST_DAF_LIMITS;
+      const double js = (te - 2451545.0) * 86400.0;
+      const int32_t ri = record_index(sg, js);
+      const double* __restrict__ grec = coef + sg.off + (int64_t)ri * sg.rsize;
+      double ir, xs;
+      Vec3 pp, dd{0.0, 0.0, 0.0};
+      if (cache_rec) {
+        if (ri != rec_cached) {
+          // the record, word 1 (the radius) as its reciprocal
+          const double2* __restrict__ src = reinterpret_cast<const double2*>(grec);
+#pragma unroll
+          for (int q = 0; q < 22; ++q) reinterpret_cast<double2*>(rslot)[q] = src[q];
+          rslot[1] = 1.0 / grec[1];
+          rec_cached = ri;
+        }
+        ir = rslot[1];
+        xs = (js - rslot[0]) * ir;
+        cheb_both_lds(rslot, xs, pp, dd);
+      } else {
+        ir = 1.0 / grec[1];
+        xs = (js - grec[0]) * ir;
+        if (spk->all14) cheb_both_n<14>(grec + 2, xs, pp, dd);
+        else cheb_both_dyn(grec + 2, sg.n_coef, xs, pp, dd);
+      }
+      const double vsc = 86400.0 * ir * INV_AU_KM;
+      bp = Vec3{pp.x * INV_AU_KM, pp.y * INV_AU_KM, pp.z * INV_AU_KM};
+      bv = need_vel ? Vec3{dd.x * vsc, dd.y * vsc, dd.z * vsc} : Vec3{0.0, 0.0, 0.0};
+    }
+    // the barycentre is read before the bodies (spk_accel), so its failure
+    // comes first; Earth and Moon are given relative to it
+    const int32_t emb_st = row_bcast_i<WIDE_EMB>(lst);
+    const Vec3 emb{row_bcast<WIDE_EMB>(bp.x), row_bcast<WIDE_EMB>(bp.y),
+                   row_bcast<WIDE_EMB>(bp.z)};
+    Vec3 acc{0.0, 0.0, 0.0};
+    if (is_body) {
+      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+      if (lst == ST_OK)
+        lst = add_body_at(bd, bp, bv, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
+                          ngp, acc);
+    }
+    int32_t st = emb_st, impact = 0;
+    {
+      const uint64_t bad = __ballot(is_body && lst != ST_OK);
+      const uint32_t gbad = (uint32_t)((bad >> g0) & 0xFFFFull);
+      if (st == ST_OK && gbad) {
+        const int first = __builtin_ctz(gbad);
+        st = __shfl(lst, g0 + first, 64);
+        if (st == ST_IMPACT) impact = spk->body[first].id;
+      }
+    }
+    const double ev[3] = {row_sum(acc.x), row_sum(acc.y), row_sum(acc.z)};
+
+    // ---- G column j (radau.rs:292-308), kept only on substeps ----
+    {
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double gk[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = lds.rdiff[j][z];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const bool put = sub && q == j - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
+      }
+    }
+    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
+    const bool es = sub && j == 7 && st == ST_OK;
+    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double acc2 = G[c][d];
+#pragma unroll
+        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
+        B[c][d] = es ? acc2 : B[c][d];
+      }
+    }
+    double fmx[3];
+    bool small = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      fmx[d] = fabs(ev[d]) + 1e-6;
+      small = small && fabs(B[6][d] - bp6[d]) < 1e-14 * fmx[d];
+    }
+    const bool conv = es && small;
+    // ---- accepted step (radau.rs:320-338) ----
+    {
+      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double bw = 0.0, bu = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          bw += B[k][d] * RTC.w[k];
+          bu += B[k][d] * RTC.u[k];
+        }
+        double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
+        double nv = vs[d] + h * (as[d] + bu);
+        xs[d] = conv ? nx : xs[d];
+        vs[d] = conv ? nv : vs[d];
+      }
+      x = Vec3{xs[0], xs[1], xs[2]};
+      v = Vec3{vs[0], vs[1], vs[2]};
+    }
+    if (conv) {
+      // next step size (radau.rs:339-342)
+      double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
+      double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
+      double ratio = root7(eps * fm / bm);
+      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
+      h_next = h * ratio;
+    }
+    t = conv ? t + h : t;
+    const bool take_a = st == ST_OK && !sub;
+    a0 = Vec3{take_a ? ev[0] : a0.x, take_a ? ev[1] : a0.y, take_a ? ev[2] : a0.z};
+
+    // ---- scalar control (radau.rs:160-218) ----
+    int32_t done = -1;
+    bool begin = false;
+    if (st != ST_OK) {
+      done = st;
+    } else if (phase == PH_INIT) {
+      if (fabs(tf - t) < 1e-10) {
+        done = ST_OK;
+      } else {
+        h = copysign(0.1, tf - t);
+        begin = true;
+      }
+    } else if (phase == PH_EXACT) {
+      h = h_next;
+      if (fabs(t - tf) < 1e-12) {
+        done = ST_OK;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    } else if (j < 7) {
+      j = j + 1;
+    } else if (conv) {
+      n_acc++;
+      phase = PH_EXACT;
+    } else if (sweep + 1 < 10) {
+      sweep++;
+      j = 1;
+      n_sweep++;
+    } else {
+      n_fail++;
+      fails++;
+      h *= 0.7;
+      if (fails > 10) {
+        done = ST_CONVERGENCE;
+      } else {
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    }
+    if (begin) {
+      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      sweep = 0;
+      j = 1;
+      n_sweep++;
+      phase = PH_SUB;
+    }
+    if (done >= 0) {
+      if (l == 0)
+        store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      return;
+    }
+  }
+}
+
 template <typename K>
 static int64_t resident_grid(K kernel, int block, size_t lds_bytes, int32_t n_cu) {
   int per_cu = 0;
@@ -1094,8 +1380,12 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     int64_t* split = p.mode == 0 && p.latency_k > 0 ? p.split : nullptr;
     const double lanes = (double)resident_grid(radau_nbody_kernel<false>, block, lds_bytes,
                                                p.n_cu) * block;
-    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off, split, lanes,
-                       p.latency_k);
+    // KH_LATENCY_SHARE scales the resident lane's share above which an
+    // object goes to the latency kernel (1 by default)
+    double share = 1.0;
+    if (const char* e = getenv("KH_LATENCY_SHARE")) share = atof(e) > 0.0 ? atof(e) : 1.0;
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off, split,
+                       lanes / share, p.latency_k);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off, perm);
     a.perm = perm;
   }
@@ -1152,12 +1442,16 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       const int64_t k_max = p.latency_k < p.n ? p.latency_k : p.n;
       (void)hipEventRecord(p.fork, stream);
       (void)hipStreamWaitEvent(p.stream2, p.fork, 0);
-      with_ast(p.n_ast, [&](auto ast) {
-        constexpr bool A = decltype(ast)::value;
-        hipLaunchKernelGGL((radau_nbody_kernel<false, true, false, A>),
-                           dim3((unsigned)((k_max + 63) / 64)), dim3(64), 0, p.stream2, spk,
-                           coef, al);
-      });
+      if (p.wide_latency && p.n_ast == 0)
+        hipLaunchKernelGGL(radau_wide_kernel, dim3((unsigned)((k_max * WIDE_L + 255) / 256)),
+                           dim3(256), 0, p.stream2, spk, coef, al);
+      else
+        with_ast(p.n_ast, [&](auto ast) {
+          constexpr bool A = decltype(ast)::value;
+          hipLaunchKernelGGL((radau_nbody_kernel<false, true, false, A>),
+                             dim3((unsigned)((k_max + 63) / 64)), dim3(64), 0, p.stream2, spk,
+                             coef, al);
+        });
       (void)hipEventRecord(p.join, p.stream2);
       a.split = p.split;
       a.part = 2;
