@@ -1062,6 +1062,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--epsilon", type=float, default=1e-6,
+                    help="Radau tolerance of configs 2 and 3 (kete's 1e-6 by default)")
     ap.add_argument("--no-sub", action="store_true",
                     help="config 2 only: skip the other workloads' sub-records")
     args = ap.parse_args()
@@ -1070,11 +1072,11 @@ def main():
                                    "(the propagation shards by object across GPUs)"}))
         sys.exit(2)
     if args.config == 2:
-        line = run_nbody(args)
+        line = run_nbody(args, epsilon=args.epsilon)
         if not args.no_sub:
             line["sub_records"] = run_sub_records(args)
     elif args.config == 3:
-        line = run_nbody(args, uneven=True)
+        line = run_nbody(args, uneven=True, epsilon=args.epsilon)
     elif args.config == 1:
         line = run_two_body(args)
     elif args.config == 0:
