@@ -1068,7 +1068,9 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
     if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
     const double js = (t - 2451545.0) * 86400.0;
     const int32_t ri = record_index(g, js);
-#ifdef KH_SCALAR_FIRST
+    // a wavefront-uniform record: its coefficients through scalar loads
+    // (SGPR operands of the sums; no LDS reads, fewer VGPRs): 72.3 -> 69.3
+    // ms per launch at 1e6 objects against the LDS window below
     const int32_t ru = __builtin_amdgcn_readfirstlane(ri);
     if (__all(ri == ru)) {
       const double* rec = coef + g.off + (int64_t)ru * 44;
@@ -1077,7 +1079,6 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                      (js - rec[0]) * ir, ir, x, v, exact, ng, acc, impact_id,
                                      n_ast);
     }
-#endif
     const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
     if (win > 0 && __all(loc >= 0 && loc < win)) {
       const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
