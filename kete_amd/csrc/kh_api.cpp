@@ -115,6 +115,7 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
     delete c;
     return fail(KH_E_HIP, "stream/event/queue allocation failed");
   }
+
   RadauTables t;
   fill_radau_tables(t);
   if (upload_radau_tables(t) != 0 || upload_grad_tables(t) != 0) {
@@ -757,15 +758,34 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   }
   p.n_cu = c->n_cu;
   if (const char* g = getenv("KH_GRID")) p.grid = atoi(g);
-  // plain launch on the shared record grid: the slowest objects run in the
-  // latency kernel beside the main launch, those whose cost outweighs a
-  // lane's share of the batch, at most a cap (KH_LATENCY_K sets it)
-  // The latency kernel: sixteen lanes per object for batches of uneven arcs
-  // (per-object final epochs; planets only), whose heavy tail it shortens
-  // (2e6 objects over 1-3650 d: 449 ms with up to 32 x n_cu objects in it,
-  // 484 ms with the one-lane variant); the one-lane variant, up to 8 x n_cu
-  // objects, for a common final epoch, where the main launch sets the time
-  // either way and the one-lane variant takes fewer SIMDs from it.
+  // tail hand-off of the plain launch (KH_PARK=0 turns it off): a wavefront
+  // down to KH_PARK_LANES live lanes (default 4, at most 4) parks the objects
+  // with more than KH_PARK_LEFT force evaluations to go (default 200, by
+  // their rate so far) and finishes them sixteen lanes each.  It catches the
+  // slow objects no cost key predicts (close approaches): 1e6 objects over a
+  // year, epsilon 1e-9: 471 -> 212 ms; epsilon 1e-6 with the one-lane
+  // latency kernel: 71.0 -> 69.9 ms, without it 67.5 ms.
+  bool park = false;
+  {
+    const char* pk = getenv("KH_PARK");
+    if (!me && !include_asteroids && !(pk && atoi(pk) == 0)) {
+      park = true;
+      p.park_lanes = 4;
+      p.park_min_left = 200.0;
+      if (const char* e = getenv("KH_PARK_LANES")) p.park_lanes = atoi(e);
+      if (const char* e = getenv("KH_PARK_LEFT")) p.park_min_left = atof(e);
+      p.park_lanes = p.park_lanes < 0 ? 0 : (p.park_lanes > 4 ? 4 : p.park_lanes);
+    }
+  }
+  // plain launch on the shared record grid: the slowest objects by the cost
+  // key run in the latency kernel beside the main launch, those whose cost
+  // outweighs a lane's share of the batch, at most a cap (KH_LATENCY_K sets
+  // it).  The latency kernel: sixteen lanes per object for batches of uneven
+  // arcs (per-object final epochs; planets only), whose heavy tail it
+  // shortens (2e6 objects over 1-3650 d: 449 ms with up to 32 x n_cu objects
+  // in it, 484 ms with the one-lane variant, 680 ms with none); the one-lane
+  // variant, up to 8 x n_cu objects, for a common final epoch without the
+  // tail hand-off (with it, the main launch alone is faster).
   // KH_LATENCY=wide or =ilp chooses.
   {
     const char* lv = getenv("KH_LATENCY");
@@ -773,7 +793,10 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     if (lv && std::string(lv) == "wide") wide = true;
     if (lv && std::string(lv) == "ilp") wide = false;
     p.wide_latency = wide && !include_asteroids;
-    p.latency_k = c->spk.shared_grid ? (int64_t)c->n_cu * (p.wide_latency ? 32 : 8) : 0;
+    p.latency_k = !c->spk.shared_grid ? 0
+                  : p.wide_latency    ? (int64_t)c->n_cu * 32
+                  : park              ? 0
+                                      : (int64_t)c->n_cu * 8;
   }
   if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;
   if (tuned) {

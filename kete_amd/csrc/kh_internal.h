@@ -88,6 +88,13 @@ struct NBodyLaunch {
   double* mout[6];
   // force list: 0 PLANETS, N_ASTEROIDS MASSES (include_asteroids)
   int32_t n_ast;
+  // tail hand-off of the plain planets-only launch (park_lanes = 0: off):
+  // when a wavefront is down to park_lanes (<= 4) live lanes, an object that
+  // by its progress so far has more than park_min_left force evaluations to
+  // go is parked at its next step boundary, and the wavefront finishes its
+  // parked objects on sixteen lanes each once its other lanes are done
+  int32_t park_lanes;
+  double park_min_left;
 };
 
 // Shared epochs of a multi-epoch propagation (device array, integration
@@ -100,6 +107,19 @@ struct MultiEpochs {
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.
+// An object parked by the plain N-body launch at a step boundary: the
+// integrator state a new step starts from (radau.rs:183-215 between two
+// calls of step): time, final time, the step about to be tried, position,
+// velocity, acceleration, the B table carried over as the predictor, and the
+// work counters so far.  B is [k][d].
+struct ParkRec {
+  int64_t obj;
+  int32_t n_force, n_sweep, n_acc, n_fail;
+  double t, tf, h;
+  double x[3], v[3], a[3];
+  double B[7][3];
+};
+
 constexpr size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket

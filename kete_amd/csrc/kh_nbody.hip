@@ -67,6 +67,9 @@ struct NBodyArgs {
   int32_t n_epochs;
   double* mout[6];
   int32_t n_ast;  // asteroids after the planets in the force list (0 or 5)
+  // tail hand-off of the plain planets-only launch (park_lanes = 0: off)
+  int32_t park_lanes;
+  double park_min_left;
 };
 
 // Last index of the run of epochs from e that moves away from t in one
@@ -389,6 +392,306 @@ struct LdsTables {
   double rdiff[8][8];
 };
 
+constexpr int WIDE_L = 16;
+constexpr int WIDE_EMB = 10;
+
+// Parked objects of the plain launch: up to PARK_SLOTS per wavefront (one
+// per sixteen-lane row), four wavefronts per block.
+constexpr int PARK_SLOTS = 64 / WIDE_L;
+__shared__ ParkRec kh_park_rec[4][PARK_SLOTS];
+__shared__ int32_t kh_park_n[4];
+
+// ---------------------------------------------------------------------------
+// One object on sixteen lanes (one DPP row), from `st` to the end of its arc:
+// lane l < 10 evaluates body l of kete's PLANETS list (gravity.rs:145-148)
+// -- its Chebyshev record, its Newton, GR and J2 terms and, for the Sun, the
+// non-grav term (add_body_at) -- and lane 10 the Earth-Moon barycentre
+// segment the Earth and Moon lanes add to their positions.  The row sums the
+// terms (row_sum: the same bits in every lane), and every lane of the row
+// then runs the same integrator arithmetic as the one-lane kernel on
+// identical values, so the row takes identical decisions without exchanging
+// them.  A force evaluation thus costs one body's terms and the integrator's
+// bookkeeping per lane instead of the whole force sum: the latency of a slow
+// object's serial chain drops, at a sixteenth of the one-lane kernel's
+// objects per wavefront.  Results differ from the one-lane kernel's only by
+// the order in which the body terms are summed.  Planets only.
+// fresh: start at st.t from st.x, st.v (the integrator's first exact
+// evaluation); otherwise st is a step boundary (ParkRec) and the object
+// continues with its first substep.  Records: WINDOW reads them from the
+// launch's LDS window when there (else from HBM); otherwise each lane keeps
+// its record in rslot (LDS, 44 doubles), reloaded when the record changes.
+// l: lane in the row, g0: the row's first lane in the wavefront.
+// ---------------------------------------------------------------------------
+template <bool WINDOW>
+__device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
+                                               const double* __restrict__ coef,
+                                               const NBodyArgs& a, const LdsTables& lds,
+                                               int64_t obj, const ParkRec& st0, bool fresh,
+                                               int l, int g0, double* rslot, int32_t winw) {
+  const double eps = a.epsilon;
+  double t = st0.t;
+  const double tf = st0.tf;
+  Vec3 x{st0.x[0], st0.x[1], st0.x[2]};
+  Vec3 v{st0.v[0], st0.v[1], st0.v[2]};
+  const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
+
+  // this lane's body (lanes 0..9) or segment (lane 10); lanes 11..15 none
+  const bool is_body = l < N_PLANETS;
+  const bool is_seg = l <= WIDE_EMB;
+  const BodyDev bd = spk->body[is_body ? l : 0];
+  const int32_t seg = l == WIDE_EMB ? spk->emb_seg : bd.seg;
+  const SegDev sg = spk->seg[seg];
+  const bool need_vel = is_body && bd.kind == 1;
+  const bool all14 = spk->all14 != 0;
+  const bool cache_rec = all14 && sg.rsize == 44;
+  int32_t rec_cached = -1;
+
+  double h = 0.0, h_next = 0.0;
+  Vec3 a0{0, 0, 0};
+  double B[7][3], G[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
+  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
+  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
+  if (!fresh) {
+    // the first substep of the step that starts at st0 (the state after
+    // `begin` in the one-lane kernel)
+    h = st0.h;
+    a0 = Vec3{st0.a[0], st0.a[1], st0.a[2]};
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) B[k][d] = st0.B[k][d];
+    phase = PH_SUB;
+    n_force = st0.n_force;
+    n_sweep = st0.n_sweep;
+    n_acc = st0.n_acc;
+    n_fail = st0.n_fail;
+  }
+
+  for (;;) {
+    const bool sub = phase == PH_SUB;
+    // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
+    const double s = lds.h[j];
+    double xe[3], ve[3];
+    {
+      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double sp = 1.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        sp *= s;
+        const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          bw[d] += B[k][d] * wk;
+          bu[d] += B[k][d] * uk;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
+        double pv = vs[d] + s * h * (as[d] + bu[d]);
+        xe[d] = sub ? px : xs[d];
+        ve[d] = sub ? pv : vs[d];
+      }
+    }
+    const double te = sub ? t + s * h : t;
+
+    // ---- this lane's share of the force evaluation ----
+    n_force++;
+    int32_t lst = ST_OK;
+    Vec3 bp{0.0, 0.0, 0.0}, bv{0.0, 0.0, 0.0};
+    if (is_seg) {
+      if (te < sg.jd_start || te > sg.jd_end) lst = ST_DAF_LIMITS;
+      const double js = (te - 2451545.0) * 86400.0;
+      const int32_t ri = record_index(sg, js);
+      const double* __restrict__ grec = coef + sg.off + (int64_t)ri * sg.rsize;
+      double ir, xs;
+      Vec3 pp, dd{0.0, 0.0, 0.0};
+      int32_t loc = -1;
+      if constexpr (WINDOW) loc = winw > 0 && all14 ? ri - kh_win_w0[seg] : -1;
+      if (WINDOW && loc >= 0 && loc < winw) {
+        // the record in the launch's LDS window (word 1 staged as 1 / radius)
+        const double* rec = kh_win_rec + (seg * winw + loc) * 44;
+        ir = rec[1];
+        xs = (js - rec[0]) * ir;
+        cheb_both_lds(rec, xs, pp, dd);
+      } else if (!WINDOW && cache_rec) {
+        if (ri != rec_cached) {
+          // the record, word 1 (the radius) as its reciprocal
+          const double2* __restrict__ src = reinterpret_cast<const double2*>(grec);
+#pragma unroll
+          for (int q = 0; q < 22; ++q) reinterpret_cast<double2*>(rslot)[q] = src[q];
+          rslot[1] = 1.0 / grec[1];
+          rec_cached = ri;
+        }
+        ir = rslot[1];
+        xs = (js - rslot[0]) * ir;
+        cheb_both_lds(rslot, xs, pp, dd);
+      } else {
+        ir = 1.0 / grec[1];
+        xs = (js - grec[0]) * ir;
+        if (all14) cheb_both_n<14>(grec + 2, xs, pp, dd);
+        else cheb_both_dyn(grec + 2, sg.n_coef, xs, pp, dd);
+      }
+      const double vsc = 86400.0 * ir * INV_AU_KM;
+      bp = Vec3{pp.x * INV_AU_KM, pp.y * INV_AU_KM, pp.z * INV_AU_KM};
+      bv = need_vel ? Vec3{dd.x * vsc, dd.y * vsc, dd.z * vsc} : Vec3{0.0, 0.0, 0.0};
+    }
+    // the barycentre is read before the bodies (spk_accel), so its failure
+    // comes first; Earth and Moon are given relative to it
+    const int32_t emb_st = row_bcast_i<WIDE_EMB>(lst);
+    const Vec3 emb{row_bcast<WIDE_EMB>(bp.x), row_bcast<WIDE_EMB>(bp.y),
+                   row_bcast<WIDE_EMB>(bp.z)};
+    Vec3 acc{0.0, 0.0, 0.0};
+    if (is_body) {
+      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
+      if (lst == ST_OK)
+        lst = add_body_at(bd, bp, bv, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
+                          ngp, acc);
+    }
+    int32_t st = emb_st, impact = 0;
+    {
+      const uint64_t bad = __ballot(is_body && lst != ST_OK);
+      const uint32_t gbad = (uint32_t)((bad >> g0) & 0xFFFFull);
+      if (st == ST_OK && gbad) {
+        const int first = __builtin_ctz(gbad);
+        st = __shfl(lst, g0 + first, 64);
+        if (st == ST_IMPACT) impact = spk->body[first].id;
+      }
+    }
+    const double ev[3] = {row_sum(acc.x), row_sum(acc.y), row_sum(acc.z)};
+
+    // ---- G column j (radau.rs:292-308), kept only on substeps ----
+    {
+      const double as[3] = {a0.x, a0.y, a0.z};
+      double gk[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
+#pragma unroll
+      for (int z = 1; z < 7; ++z) {
+        const double rd = lds.rdiff[j][z];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
+      }
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const bool put = sub && q == j - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
+      }
+    }
+    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
+    const bool es = sub && j == 7 && st == ST_OK;
+    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double acc2 = G[c][d];
+#pragma unroll
+        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
+        B[c][d] = es ? acc2 : B[c][d];
+      }
+    }
+    double fmx[3];
+    bool small = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      fmx[d] = fabs(ev[d]) + 1e-6;
+      small = small && fabs(B[6][d] - bp6[d]) < 1e-14 * fmx[d];
+    }
+    const bool conv = es && small;
+    // ---- accepted step (radau.rs:320-338) ----
+    {
+      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
+      const double as[3] = {a0.x, a0.y, a0.z};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double bw = 0.0, bu = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+          bw += B[k][d] * RTC.w[k];
+          bu += B[k][d] * RTC.u[k];
+        }
+        double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
+        double nv = vs[d] + h * (as[d] + bu);
+        xs[d] = conv ? nx : xs[d];
+        vs[d] = conv ? nv : vs[d];
+      }
+      x = Vec3{xs[0], xs[1], xs[2]};
+      v = Vec3{vs[0], vs[1], vs[2]};
+    }
+    if (conv) {
+      // next step size (radau.rs:339-342)
+      double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
+      double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
+      double ratio = root7(eps * fm / bm);
+      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
+      h_next = h * ratio;
+    }
+    t = conv ? t + h : t;
+    const bool take_a = st == ST_OK && !sub;
+    a0 = Vec3{take_a ? ev[0] : a0.x, take_a ? ev[1] : a0.y, take_a ? ev[2] : a0.z};
+
+    // ---- scalar control (radau.rs:160-218) ----
+    int32_t done = -1;
+    bool begin = false;
+    if (st != ST_OK) {
+      done = st;
+    } else if (phase == PH_INIT) {
+      if (fabs(tf - t) < 1e-10) {
+        done = ST_OK;
+      } else {
+        h = copysign(0.1, tf - t);
+        begin = true;
+      }
+    } else if (phase == PH_EXACT) {
+      h = h_next;
+      if (fabs(t - tf) < 1e-12) {
+        done = ST_OK;
+      } else {
+        fails = 0;
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    } else if (j < 7) {
+      j = j + 1;
+    } else if (conv) {
+      n_acc++;
+      phase = PH_EXACT;
+    } else if (sweep + 1 < 10) {
+      sweep++;
+      j = 1;
+      n_sweep++;
+    } else {
+      n_fail++;
+      fails++;
+      h *= 0.7;
+      if (fails > 10) {
+        done = ST_CONVERGENCE;
+      } else {
+        if (fabs(h) < 0.005) h = copysign(0.005, h);
+        begin = true;
+      }
+    }
+    if (begin) {
+      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      sweep = 0;
+      j = 1;
+      n_sweep++;
+      phase = PH_SUB;
+    }
+    if (done >= 0) {
+      if (l == 0)
+        store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      return;
+    }
+  }
+}
+
 // ILP: the latency variant for the slowest objects -- one wavefront per block
 // and per SIMD, the whole register file, the force evaluation arranged for
 // the shortest dependency chain (spk_accel_ilp), no LDS window.
@@ -450,8 +753,14 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   bool shortened = false;        // MULTI: the step was shortened to land on tf
   if (MULTI) tf = a.epochs[multi_run_end(a, 0, t)];
 
+  // the plain planets-only launch hands its wavefronts' tails on
+  constexpr bool PARKABLE = !PERSISTENT && !ILP && !MULTI && !AST;
+  if constexpr (PARKABLE)
+    if ((threadIdx.x & 63) == 0) kh_park_n[threadIdx.x >> 6] = 0;
   while (obj < a.n) {
     const bool sub = phase == PH_SUB;
+    // live lanes of the wavefront (every lane with an object is here)
+    const uint64_t live_mask = PARKABLE ? __ballot(1) : 0ull;
     // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
     const double s = lds.h[j];
     double xe[3], ve[3];
@@ -610,7 +919,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
 
     // ---- scalar control (radau.rs:160-218) ----
     int32_t done = -1;
-    bool begin = false;
+    bool begin = false, next_step = false;
     if (st != ST_OK) {
       done = st;  // Impact and DAFLimits abort the integration (radau.rs:207-208)
     } else if (phase == PH_INIT) {
@@ -628,6 +937,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
         fails = 0;
         if (fabs(h) < 0.005) h = copysign(0.005, h);
         begin = true;
+        next_step = true;
       }
     } else if (j < 7) {
       j = j + 1;
@@ -693,10 +1003,54 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       phase = PH_SUB;
     }
 
-    // ---- finished objects: store, then take the next one ----
-    if (done >= 0) {
+    // ---- tail hand-off: with few live lanes left in the wavefront, an
+    // object with much work to go leaves at the start of a step, its state
+    // parked in LDS, and the wavefront finishes the parked objects sixteen
+    // lanes each once its other lanes are done (the remaining work is
+    // estimated from the evaluations so far and the fraction of the arc
+    // covered) ----
+    bool parked = false;
+    if constexpr (PARKABLE) {
+      if (next_step && a.park_lanes > 0 && __popcll(live_mask) <= a.park_lanes) {
+        const double t0 = a.jd0[obj];
+        const double covered = (t - t0) / (tf - t0);
+        if (covered > 0.0 && (double)n_force * (1.0 - covered) > a.park_min_left * covered) {
+          const int w = threadIdx.x >> 6;
+          const int k = atomicAdd(&kh_park_n[w], 1);
+          if (k < PARK_SLOTS) {
+            ParkRec& r = kh_park_rec[w][k];
+            r.obj = obj;
+            r.n_force = n_force;
+            r.n_sweep = n_sweep;
+            r.n_acc = n_acc;
+            r.n_fail = n_fail;
+            r.t = t;
+            r.tf = tf;
+            r.h = h;
+            r.x[0] = x.x;
+            r.x[1] = x.y;
+            r.x[2] = x.z;
+            r.v[0] = v.x;
+            r.v[1] = v.y;
+            r.v[2] = v.z;
+            r.a[0] = a0.x;
+            r.a[1] = a0.y;
+            r.a[2] = a0.z;
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) r.B[q][d] = B[q][d];
+            parked = true;
+          }
+        }
+      }
+    }
+
+    // ---- finished (or parked) objects: store, then take the next one ----
+    if (done >= 0 || parked) {
       // SSB-equatorial result; the post-pass converts it back
-      store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      if (!parked)
+        store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
       int64_t nxt = a.n;
       if (PERSISTENT) nxt = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
       double nt, ntf;
@@ -713,11 +1067,22 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       ej = 0;
       n_force = n_sweep = n_acc = n_fail = 0;
     }
-    const bool fresh = done >= 0;
+    const bool fresh = done >= 0 || parked;
 #pragma unroll
     for (int k = 0; k < 7; ++k)
 #pragma unroll
       for (int d = 0; d < 3; ++d) B[k][d] = fresh ? 0.0 : B[k][d];
+  }
+  // ---- the wavefront's parked objects, sixteen lanes each ----
+  if constexpr (PARKABLE) {
+    const int w = threadIdx.x >> 6;
+    const int g = (threadIdx.x & 63) / WIDE_L;
+    const int np = kh_park_n[w];
+    if (g < np) {
+      const ParkRec& r = kh_park_rec[w][g];
+      wide_integrate<true>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),
+                           g * WIDE_L, nullptr, winw);
+    }
   }
 }
 
@@ -1045,9 +1410,12 @@ __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 // one-lane kernel's only by the order in which the body terms are summed.
 // Planets only (the asteroid list runs in the one-lane latency kernel).
 // ---------------------------------------------------------------------------
-constexpr int WIDE_L = 16;
-constexpr int WIDE_EMB = 10;
-
+// ---------------------------------------------------------------------------
+// Wide latency variant for the slowest objects (the head of the longest-
+// first order, on the second stream beside the main launch): four objects
+// per wavefront, each on sixteen lanes (wide_integrate), each lane's record
+// cached in LDS.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restrict__ spk,
                                                          const double* __restrict__ coef,
                                                          NBodyArgs a) {
@@ -1069,246 +1437,15 @@ __global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restric
   if (a.status[obj] != ST_OK) return;
   const int l = threadIdx.x & (WIDE_L - 1);
   const int g0 = (threadIdx.x & 63) & ~(WIDE_L - 1);
-  const double eps = a.epsilon;
-
-  double t = a.jd0[obj];
-  const double tf = a.jd_final[a.jd_final_stride ? obj : 0];
-  Vec3 x{a.out[0][obj], a.out[1][obj], a.out[2][obj]};
-  Vec3 v{a.out[3][obj], a.out[4][obj], a.out[5][obj]};
-  const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
-
-  // this lane's body (lanes 0..9) or segment (lane 10); lanes 11..15 none
-  const bool is_body = l < N_PLANETS;
-  const bool is_seg = l <= WIDE_EMB;
-  const BodyDev bd = spk->body[is_body ? l : 0];
-  const int32_t seg = l == WIDE_EMB ? spk->emb_seg : bd.seg;
-  const SegDev sg = spk->seg[seg];
-  const bool need_vel = is_body && bd.kind == 1;
-  const bool cache_rec = spk->all14 != 0 && sg.rsize == 44;
-  int32_t rec_cached = -1;
-  double* const rslot = &s_rec[threadIdx.x / WIDE_L][is_seg ? l : 0][0];
-
-  double h = 0.0, h_next = 0.0;
-  Vec3 a0{0, 0, 0};
-  double B[7][3], G[7][3];
-#pragma unroll
-  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
-  int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
-  int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
-
-  for (;;) {
-    const bool sub = phase == PH_SUB;
-    // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
-    const double s = lds.h[j];
-    double xe[3], ve[3];
-    {
-      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
-      const double as[3] = {a0.x, a0.y, a0.z};
-      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
-      double sp = 1.0;
-#pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        sp *= s;
-        const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          bw[d] += B[k][d] * wk;
-          bu[d] += B[k][d] * uk;
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
-        double pv = vs[d] + s * h * (as[d] + bu[d]);
-        xe[d] = sub ? px : xs[d];
-        ve[d] = sub ? pv : vs[d];
-      }
-    }
-    const double te = sub ? t + s * h : t;
-
-    // ---- this lane's share of the force evaluation ----
-    n_force++;
-    int32_t lst = ST_OK;
-    Vec3 bp{0.0, 0.0, 0.0}, bv{0.0, 0.0, 0.0};
-    if (is_seg) {
-      if (te < sg.jd_start || te > sg.jd_end) lst = ST_DAF_LIMITS;
-      const double js = (te - 2451545.0) * 86400.0;
-      const int32_t ri = record_index(sg, js);
-      const double* __restrict__ grec = coef + sg.off + (int64_t)ri * sg.rsize;
-      double ir, xs;
-      Vec3 pp, dd{0.0, 0.0, 0.0};
-      if (cache_rec) {
-        if (ri != rec_cached) {
-          // the record, word 1 (the radius) as its reciprocal
-          const double2* __restrict__ src = reinterpret_cast<const double2*>(grec);
-#pragma unroll
-          for (int q = 0; q < 22; ++q) reinterpret_cast<double2*>(rslot)[q] = src[q];
-          rslot[1] = 1.0 / grec[1];
-          rec_cached = ri;
-        }
-        ir = rslot[1];
-        xs = (js - rslot[0]) * ir;
-        cheb_both_lds(rslot, xs, pp, dd);
-      } else {
-        ir = 1.0 / grec[1];
-        xs = (js - grec[0]) * ir;
-        if (spk->all14) cheb_both_n<14>(grec + 2, xs, pp, dd);
-        else cheb_both_dyn(grec + 2, sg.n_coef, xs, pp, dd);
-      }
-      const double vsc = 86400.0 * ir * INV_AU_KM;
-      bp = Vec3{pp.x * INV_AU_KM, pp.y * INV_AU_KM, pp.z * INV_AU_KM};
-      bv = need_vel ? Vec3{dd.x * vsc, dd.y * vsc, dd.z * vsc} : Vec3{0.0, 0.0, 0.0};
-    }
-    // the barycentre is read before the bodies (spk_accel), so its failure
-    // comes first; Earth and Moon are given relative to it
-    const int32_t emb_st = row_bcast_i<WIDE_EMB>(lst);
-    const Vec3 emb{row_bcast<WIDE_EMB>(bp.x), row_bcast<WIDE_EMB>(bp.y),
-                   row_bcast<WIDE_EMB>(bp.z)};
-    Vec3 acc{0.0, 0.0, 0.0};
-    if (is_body) {
-      if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
-      if (lst == ST_OK)
-        lst = add_body_at(bd, bp, bv, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
-                          ngp, acc);
-    }
-    int32_t st = emb_st, impact = 0;
-    {
-      const uint64_t bad = __ballot(is_body && lst != ST_OK);
-      const uint32_t gbad = (uint32_t)((bad >> g0) & 0xFFFFull);
-      if (st == ST_OK && gbad) {
-        const int first = __builtin_ctz(gbad);
-        st = __shfl(lst, g0 + first, 64);
-        if (st == ST_IMPACT) impact = spk->body[first].id;
-      }
-    }
-    const double ev[3] = {row_sum(acc.x), row_sum(acc.y), row_sum(acc.z)};
-
-    // ---- G column j (radau.rs:292-308), kept only on substeps ----
-    {
-      const double as[3] = {a0.x, a0.y, a0.z};
-      double gk[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
-#pragma unroll
-      for (int z = 1; z < 7; ++z) {
-        const double rd = lds.rdiff[j][z];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
-      }
-#pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        const bool put = sub && q == j - 1;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
-      }
-    }
-    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
-    const bool es = sub && j == 7 && st == ST_OK;
-    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
-#pragma unroll
-    for (int c = 0; c < 7; ++c) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double acc2 = G[c][d];
-#pragma unroll
-        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
-        B[c][d] = es ? acc2 : B[c][d];
-      }
-    }
-    double fmx[3];
-    bool small = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      fmx[d] = fabs(ev[d]) + 1e-6;
-      small = small && fabs(B[6][d] - bp6[d]) < 1e-14 * fmx[d];
-    }
-    const bool conv = es && small;
-    // ---- accepted step (radau.rs:320-338) ----
-    {
-      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
-      const double as[3] = {a0.x, a0.y, a0.z};
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double bw = 0.0, bu = 0.0;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-          bw += B[k][d] * RTC.w[k];
-          bu += B[k][d] * RTC.u[k];
-        }
-        double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
-        double nv = vs[d] + h * (as[d] + bu);
-        xs[d] = conv ? nx : xs[d];
-        vs[d] = conv ? nv : vs[d];
-      }
-      x = Vec3{xs[0], xs[1], xs[2]};
-      v = Vec3{vs[0], vs[1], vs[2]};
-    }
-    if (conv) {
-      // next step size (radau.rs:339-342)
-      double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
-      double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
-      double ratio = root7(eps * fm / bm);
-      ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
-      h_next = h * ratio;
-    }
-    t = conv ? t + h : t;
-    const bool take_a = st == ST_OK && !sub;
-    a0 = Vec3{take_a ? ev[0] : a0.x, take_a ? ev[1] : a0.y, take_a ? ev[2] : a0.z};
-
-    // ---- scalar control (radau.rs:160-218) ----
-    int32_t done = -1;
-    bool begin = false;
-    if (st != ST_OK) {
-      done = st;
-    } else if (phase == PH_INIT) {
-      if (fabs(tf - t) < 1e-10) {
-        done = ST_OK;
-      } else {
-        h = copysign(0.1, tf - t);
-        begin = true;
-      }
-    } else if (phase == PH_EXACT) {
-      h = h_next;
-      if (fabs(t - tf) < 1e-12) {
-        done = ST_OK;
-      } else {
-        fails = 0;
-        if (fabs(h) < 0.005) h = copysign(0.005, h);
-        begin = true;
-      }
-    } else if (j < 7) {
-      j = j + 1;
-    } else if (conv) {
-      n_acc++;
-      phase = PH_EXACT;
-    } else if (sweep + 1 < 10) {
-      sweep++;
-      j = 1;
-      n_sweep++;
-    } else {
-      n_fail++;
-      fails++;
-      h *= 0.7;
-      if (fails > 10) {
-        done = ST_CONVERGENCE;
-      } else {
-        if (fabs(h) < 0.005) h = copysign(0.005, h);
-        begin = true;
-      }
-    }
-    if (begin) {
-      if (fabs(t - tf) <= fabs(h)) h = tf - t;
-      sweep = 0;
-      j = 1;
-      n_sweep++;
-      phase = PH_SUB;
-    }
-    if (done >= 0) {
-      if (l == 0)
-        store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
-      return;
-    }
+  ParkRec st0;
+  st0.t = a.jd0[obj];
+  st0.tf = a.jd_final[a.jd_final_stride ? obj : 0];
+  for (int k = 0; k < 3; ++k) {
+    st0.x[k] = a.out[k][obj];
+    st0.v[k] = a.out[3 + k][obj];
   }
+  double* const rslot = &s_rec[threadIdx.x / WIDE_L][l <= WIDE_EMB ? l : 0][0];
+  wide_integrate<false>(spk, coef, a, lds, obj, st0, true, l, g0, rslot, 0);
 }
 
 template <typename K>
@@ -1365,6 +1502,9 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.n_epochs = p.n_epochs;
   for (int k = 0; k < 6; ++k) a.mout[k] = p.mout[k];
   const bool multi = p.epochs != nullptr && p.n_epochs > 0;
+  // tail hand-off: plain launch, one final epoch per object, planets only
+  a.park_lanes = p.mode == 0 && !multi && p.n_ast == 0 ? p.park_lanes : 0;
+  a.park_min_left = p.park_min_left;
   if (a.range)
     hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range,
                        multi ? p.ep_lo : __builtin_inf(), multi ? p.ep_hi : 0.0);

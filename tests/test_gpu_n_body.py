@@ -66,12 +66,14 @@ def test_parity_one_year(ctx, oracle_spk, mix, mode):
 def test_plain_and_persistent_identical(ctx, mix, modes, monkeypatch):
     """The one-lane kernels (plain with the one-lane latency variant,
     persistent; three-lane plain and persistent) give bitwise-identical
-    results.  The sixteen-lane latency variant (the default for uneven arcs)
-    sums the body terms in another order: the objects it takes (here all of
-    them: a batch this small fits the latency kernel's cap) agree to
-    rounding."""
+    results (the plain launch's tail hand-off, which moves objects to
+    sixteen lanes, is off here).  The sixteen-lane latency variant (the
+    default for uneven arcs) sums the body terms in another order: the
+    objects it takes (here all of them: a batch this small fits the latency
+    kernel's cap) agree to rounding."""
     n = mix.shape[1]
     batch = ka.StateArray(np.full(n, JD0), mix)
+    monkeypatch.setenv("KH_PARK", "0")
     monkeypatch.setenv("KH_LATENCY", "ilp")
     a = ctx.n_body(batch, JD0 - 200.0, mode=modes[0])
     b = ctx.n_body(batch, JD0 - 200.0, mode=modes[1])
@@ -83,6 +85,49 @@ def test_plain_and_persistent_identical(ctx, mix, modes, monkeypatch):
         assert np.array_equal(w["status"], a["status"])
         d = np.abs(w["out"][:3] - a["out"][:3]).max(axis=0)
         assert d.max() < 1e-10
+
+
+def test_tail_handoff(ctx, oracle_spk, monkeypatch):
+    """The plain launch's tail hand-off: once a wavefront is down to four
+    live lanes, objects with more than 200 force evaluations to go (by their
+    rate so far) are parked at a step boundary and finished on sixteen lanes
+    each.  A wavefront of 60 main-belt objects on 40-day arcs and four NEOs
+    on 3-year arcs: the NEOs are parked after the others end.  They must
+    match the oracle as the one-lane path does, agree with the one-lane
+    path (KH_PARK=0) to rounding -- not bitwise: the sixteen lanes sum the
+    body terms in another order, which shows the hand-off ran -- keep their
+    work counters running across it, and leave the short arcs bitwise
+    alone."""
+    mb = syn.n_body_workload(n=60, neo_fraction=0.0, seed_index=11)
+    neo = syn.n_body_workload(n=4, neo_fraction=1.0, seed_index=12)
+    xyz = np.concatenate([mb, neo], axis=1)
+    n = xyz.shape[1]
+    jf = np.concatenate([np.full(60, JD0 + 40.0), np.full(4, JD0 + 3 * 365.25)])
+    batch = ka.StateArray(np.full(n, JD0), xyz)
+    monkeypatch.setenv("KH_LATENCY_K", "0")
+    monkeypatch.setenv("KH_PARK", "1")
+    g = ctx.n_body(batch, jf, counters=True)
+    monkeypatch.setenv("KH_PARK", "0")
+    p = ctx.n_body(batch, jf, counters=True)
+    r = oracle_spk.batch_n_body(JD0, xyz, jf)
+    assert np.all(g["status"] == 0) and np.all(p["status"] == 0)
+    assert np.array_equal(g["status"], r["status"])
+    for res in (g, p):
+        assert np.abs(res["out"][:3] - r["out"][:3]).max() < POS_TOL
+        assert np.abs(res["out"][3:] - r["out"][3:]).max() < VEL_TOL
+    # short arcs: untouched; long arcs: the sixteen-lane sums
+    assert np.array_equal(g["out"][:, :60], p["out"][:, :60])
+    d = np.abs(g["out"][:3, 60:] - p["out"][:3, 60:]).max(axis=0)
+    # (the oracle's FMA build differs from its default build by 1-4e-11 AU
+    # on these arcs)
+    assert np.all(d > 0.0) and d.max() < 2e-10, d
+    # counters continue across the hand-off: the same work to within the
+    # rounding-decided sweeps and step failures (the oracle's two builds
+    # differ by up to 9% in force evaluations on these arcs)
+    gc = g["counters"][60:].astype(np.int64)
+    pc = p["counters"][60:].astype(np.int64)
+    assert np.all(np.abs(gc[:, 2] - pc[:, 2]) <= 2), (gc, pc)
+    assert np.all(np.abs(gc[:, 0] / pc[:, 0] - 1) < 0.15), (gc, pc)
 
 
 def test_three_lane_error_semantics(ctx, oracle_spk, mix):
