@@ -782,10 +782,11 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // outweighs a lane's share of the batch, at most a cap (KH_LATENCY_K sets
   // it).  The latency kernel: sixteen lanes per object for batches of uneven
   // arcs (per-object final epochs; planets only), whose heavy tail it
-  // shortens (2e6 objects over 1-3650 d: 449 ms with up to 32 x n_cu objects
-  // in it, 484 ms with the one-lane variant, 680 ms with none); the one-lane
-  // variant, up to 8 x n_cu objects, for a common final epoch without the
-  // tail hand-off (with it, the main launch alone is faster).
+  // shortens (2e6 objects over 1-3650 d with the tail hand-off: 422 ms with
+  // up to 16 x n_cu objects in it, 434-439 ms with 8, 32 or 64 x n_cu, 682
+  // ms with the one-lane variant, 680 ms with none); the one-lane variant,
+  // up to 8 x n_cu objects, for a common final epoch without the tail
+  // hand-off (with it, the main launch alone is faster).
   // KH_LATENCY=wide or =ilp chooses.
   {
     const char* lv = getenv("KH_LATENCY");
@@ -794,7 +795,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     if (lv && std::string(lv) == "ilp") wide = false;
     p.wide_latency = wide && !include_asteroids;
     p.latency_k = !c->spk.shared_grid ? 0
-                  : p.wide_latency    ? (int64_t)c->n_cu * 32
+                  : p.wide_latency    ? (int64_t)c->n_cu * 16
                   : park              ? 0
                                       : (int64_t)c->n_cu * 8;
   }
@@ -1321,7 +1322,8 @@ int32_t kh_autotune_n_body(kh_ctx* c, int64_t n, const double* jd0, const double
     }
   }
   if (best.mode == 0 && shared)
-    for (int64_t lk : {(int64_t)0, (int64_t)c->n_cu * 2, (int64_t)c->n_cu * 32}) {
+    for (int64_t lk : {(int64_t)0, (int64_t)c->n_cu * 2, (int64_t)c->n_cu * 16,
+                       (int64_t)c->n_cu * 32}) {
       Cand cd = best;
       cd.latency_k = lk;
       consider(cd);
