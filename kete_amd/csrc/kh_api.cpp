@@ -765,11 +765,9 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // slow objects no cost key predicts (close approaches): 1e6 objects over a
   // year, epsilon 1e-9: 471 -> 212 ms; epsilon 1e-6 with the one-lane
   // latency kernel: 71.0 -> 69.9 ms, without it 67.5 ms.
-  bool park = false;
   {
     const char* pk = getenv("KH_PARK");
     if (!me && !include_asteroids && !(pk && atoi(pk) == 0)) {
-      park = true;
       p.park_lanes = 4;
       p.park_min_left = 200.0;
       if (const char* e = getenv("KH_PARK_LANES")) p.park_lanes = atoi(e);
@@ -785,8 +783,12 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // shortens (2e6 objects over 1-3650 d with the tail hand-off: 422 ms with
   // up to 16 x n_cu objects in it, 434-439 ms with 8, 32 or 64 x n_cu, 682
   // ms with the one-lane variant, 680 ms with none); the one-lane variant,
-  // up to 8 x n_cu objects, for a common final epoch without the tail
-  // hand-off (with it, the main launch alone is faster).
+  // up to 8 x n_cu objects, for per-object final epochs with
+  // include_asteroids.  For a common final epoch the main launch alone is
+  // faster: planets only, its tail hand-off takes the slow objects (67.5 vs
+  // 69.9 ms with the one-lane variant beside it); with include_asteroids the
+  // cost key's head is the eccentric NEOs, which the one-lane variant holds
+  // longer than the main launch takes (191 vs 168 ms).
   // KH_LATENCY=wide or =ilp chooses.
   {
     const char* lv = getenv("KH_LATENCY");
@@ -794,10 +796,9 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     if (lv && std::string(lv) == "wide") wide = true;
     if (lv && std::string(lv) == "ilp") wide = false;
     p.wide_latency = wide && !include_asteroids;
-    p.latency_k = !c->spk.shared_grid ? 0
-                  : p.wide_latency    ? (int64_t)c->n_cu * 16
-                  : park              ? 0
-                                      : (int64_t)c->n_cu * 8;
+    p.latency_k = !c->spk.shared_grid || jd_final_stride == 0 ? 0
+                  : p.wide_latency                             ? (int64_t)c->n_cu * 16
+                                                               : (int64_t)c->n_cu * 8;
   }
   if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;
   if (tuned) {

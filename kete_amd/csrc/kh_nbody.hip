@@ -276,10 +276,11 @@ __device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t slot, doubl
 // carry objects of similar cost, so few idle while their wavefront finishes.
 // The cost key is |span| times the mean of r^-1.5 (the local dynamical rate
 // the Radau steps follow) at nine points of the arc on the two-body orbit
-// through the input state; binned on a log scale (eighth-octave buckets) and
-// counting-sorted, highest first.  On the default workload it packs
-// wavefronts to 82% of their lanes' work where the mean motion alone (the
-// rate at the epoch's semi-major axis) packed them to 68%.  The order
+// through the input state, times exp(e^2); binned on a log scale
+// (eighth-octave buckets) and counting-sorted, highest first.  On the
+// default workload the nine points packed wavefronts to 82% of their lanes'
+// work where the mean motion alone (the rate at the epoch's semi-major
+// axis) packed them to 68%; the eccentricity factor takes it to 85%.  The order
 // changes which lane takes an object, never its result.
 // ---------------------------------------------------------------------------
 constexpr int ORDER_BUCKETS = 1024;
@@ -312,7 +313,18 @@ __device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
     }
     sum += 1.0 / (rk * sqrt(rk));
   }
-  const double key = fabs(span) * (sum / ORDER_POINTS);
+  // eccentric orbits cost more than the nine points show (the steps crowd
+  // around perihelion passages between them): times exp(e^2), e from the
+  // input state.  Sorting 131,072 objects of the 1e6-object one-year
+  // workload and 65,536 of the uneven-span one by their key and cutting the
+  // order into wavefronts of 64, the oracle's force-evaluation counts fill
+  // 84.8% and 79.8% of the wavefronts' lane time with it, 81.7% and 70.8%
+  // without (99.8% and 97.1% sorted by the counts themselves).
+  const Vec3 hv{x.y * v.z - x.z * v.y, x.z * v.x - x.x * v.z, x.x * v.y - x.y * v.x};
+  const Vec3 ev{(v.y * hv.z - v.z * hv.y) * (1.0 / GMS) - x.x / r0,
+                (v.z * hv.x - v.x * hv.z) * (1.0 / GMS) - x.y / r0,
+                (v.x * hv.y - v.y * hv.x) * (1.0 / GMS) - x.z / r0};
+  const double key = fabs(span) * (sum / ORDER_POINTS) * exp(fmin(dot(ev, ev), 4.0));
   if (!(key > 0.0) || !isfinite(key)) return 0;
   const int b = (int)floor(log2(key) * 8.0) + ORDER_BUCKETS / 2;
   return b < 1 ? 1 : (b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b);
