@@ -829,9 +829,18 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     // turns it off
     const char* ord = getenv("KH_ORDER");
     if (!(ord && atoi(ord) == 0)) {
-      if (order_scratch) p.order_buf = order_scratch;
-      else if (kh_grow(&c->obuf, &c->obuf_size, order_buf_bytes(n)) == KH_OK) p.order_buf = c->obuf;
-      else return KH_E_HIP;
+      // the allocating entry points also stage each object as one 64-byte
+      // record after the order (the no-alloc entry point's scratch holds the
+      // order only)
+      const size_t ob = align256(order_buf_bytes(n));
+      if (order_scratch) {
+        p.order_buf = order_scratch;
+      } else if (kh_grow(&c->obuf, &c->obuf_size, ob + stage_buf_bytes(n)) == KH_OK) {
+        p.order_buf = c->obuf;
+        p.stage = reinterpret_cast<double*>(static_cast<char*>(c->obuf) + ob);
+      } else {
+        return KH_E_HIP;
+      }
     }
   }
   if (launch_nbody(c->d_spk, c->d_coef, p, c->stream, &c->last_block, &c->last_grid,

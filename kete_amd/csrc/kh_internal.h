@@ -95,6 +95,9 @@ struct NBodyLaunch {
   // parked objects on sixteen lanes each once its other lanes are done
   int32_t park_lanes;
   double park_min_left;
+  // [n][8] doubles, 64-byte aligned: the object records of the plain launch
+  // over the longest-first order (stage_buf_bytes), or null
+  double* stage;
 };
 
 // Shared epochs of a multi-epoch propagation (device array, integration
@@ -121,6 +124,8 @@ struct ParkRec {
 };
 
 constexpr size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
+// Bytes of NBodyLaunch::stage for n objects (one 64-byte record each).
+constexpr size_t stage_buf_bytes(int64_t n) { return (size_t)n * 64; }
 
 // Pre-pass, integrator and post-pass on `stream`; ev0/ev1 (nullable) bracket
 // the integrator kernel alone.

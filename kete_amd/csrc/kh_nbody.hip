@@ -70,6 +70,15 @@ struct NBodyArgs {
   // tail hand-off of the plain planets-only launch (park_lanes = 0: off)
   int32_t park_lanes;
   double park_min_left;
+  // plain launch over the longest-first order: each object's state as one
+  // 64-byte record [n][8] in object order -- written by the pre-pass
+  // (x, v, jd0, final epoch; jd0 NaN for an object the pre-pass ended),
+  // read by the integrator through the permutation, overwritten with the
+  // result (x, v, impact epoch, status and impact id), turned back into the
+  // output columns by the post-pass -- so an object's input and its result
+  // each move as one cache line instead of seven to nine scattered words.
+  // null: the output columns carry the state.
+  double* stage;
 };
 
 // Last index of the run of epochs from e that moves away from t in one
@@ -148,6 +157,18 @@ __device__ __forceinline__ void store_result(const NBodyArgs& a, int64_t i, int3
                                              int4 cnt) {
   const double nan = __builtin_nan("");
   if (st != ST_OK) x = v = Vec3{nan, nan, nan};
+  if (a.stage != nullptr) {
+    // the object's record: x, v, impact epoch, status | impact id << 32
+    double2* r = reinterpret_cast<double2*>(a.stage + 8 * i);
+    const long long tag = (long long)(uint32_t)st |
+                          ((long long)(uint32_t)(st == ST_IMPACT ? impact : 0) << 32);
+    r[0] = double2{x.x, x.y};
+    r[1] = double2{x.z, v.x};
+    r[2] = double2{v.y, v.z};
+    r[3] = double2{st == ST_IMPACT ? impact_jd : nan, __longlong_as_double(tag)};
+    if (a.counters) reinterpret_cast<int4*>(a.counters)[i] = cnt;
+    return;
+  }
   a.out[0][i] = x.x;
   a.out[1][i] = x.y;
   a.out[2][i] = x.z;
@@ -176,7 +197,28 @@ __global__ void nbody_prepare_kernel(const SpkDev* __restrict__ spk,
   const bool finite = isfinite(t) && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) &&
                       isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
   int32_t st = finite ? to_ssb_equatorial(spk, coef, a, t, x, v) : ST_VALUE_ERROR;
-  if (valid) store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+  if (valid) {
+    if (a.stage != nullptr) {
+      // the object's record for the integrator (jd0 NaN: ended here); the
+      // columns carry only the objects that end here
+      const double nan = __builtin_nan("");
+      const double tf = a.jd_final[a.jd_final_stride ? i : 0];
+      double2* r = reinterpret_cast<double2*>(a.stage + 8 * i);
+      r[0] = double2{x.x, x.y};
+      r[1] = double2{x.z, v.x};
+      r[2] = double2{v.y, v.z};
+      r[3] = double2{st == ST_OK ? t : nan, tf};
+      if (st != ST_OK) {
+        NBodyArgs b = a;
+        b.stage = nullptr;
+        store_result(b, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+      } else {
+        a.status[i] = ST_OK;
+      }
+    } else {
+      store_result(a, i, st, x, v, 0, 0.0, int4{0, 0, 0, 0});
+    }
+  }
   if (!valid) st = ST_VALUE_ERROR;
   if (a.range != nullptr) {
     // epoch range of the batch (all epochs are positive JDs, so the IEEE bit
@@ -210,8 +252,33 @@ __global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n || a.status[i] != ST_OK) return;
   const double tf = a.jd_final[a.jd_final_stride ? i : 0];
-  Vec3 x{a.out[0][i], a.out[1][i], a.out[2][i]};
-  Vec3 v{a.out[3][i], a.out[4][i], a.out[5][i]};
+  Vec3 x, v;
+  if (a.stage != nullptr) {
+    // the integrator's result record; a failed integration keeps its NaNs
+    const double2* r = reinterpret_cast<const double2*>(a.stage + 8 * i);
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const long long tag = __double_as_longlong(r3.y);
+    const int32_t ist = (int32_t)(uint32_t)(tag & 0xffffffffll);
+    x = Vec3{r0.x, r0.y, r1.x};
+    v = Vec3{r1.y, r2.x, r2.y};
+    if (ist != ST_OK) {
+      a.out[0][i] = x.x;
+      a.out[1][i] = x.y;
+      a.out[2][i] = x.z;
+      a.out[3][i] = v.x;
+      a.out[4][i] = v.y;
+      a.out[5][i] = v.z;
+      a.status[i] = ist;
+      if (a.impact_id) a.impact_id[i] = (int64_t)(int32_t)(uint32_t)(tag >> 32);
+      if (a.impact_jd) a.impact_jd[i] = r3.x;
+      return;
+    }
+    if (a.impact_id) a.impact_id[i] = 0;
+    if (a.impact_jd) a.impact_jd[i] = __builtin_nan("");
+  } else {
+    x = Vec3{a.out[0][i], a.out[1][i], a.out[2][i]};
+    v = Vec3{a.out[3][i], a.out[4][i], a.out[5][i]};
+  }
   const int32_t st = from_ssb_equatorial(spk, coef, a, tf, x, v);
   if (st != ST_OK) {
     const double nan = __builtin_nan("");
@@ -257,7 +324,18 @@ __device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t slot, doubl
                                          double& tf, Vec3& x, Vec3& v) {
   while (slot < a.slot_end) {
     const int64_t i = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
-    if (a.status[i] == ST_OK) {
+    if (a.stage != nullptr) {
+      // one 64-byte record (jd0 NaN: the pre-pass ended the object)
+      const double2* r = reinterpret_cast<const double2*>(a.stage + 8 * i);
+      const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+      if (!isnan(r3.x)) {
+        t = r3.x;
+        tf = r3.y;
+        x = Vec3{r0.x, r0.y, r1.x};
+        v = Vec3{r1.y, r2.x, r2.y};
+        return i;
+      }
+    } else if (a.status[i] == ST_OK) {
       t = a.jd0[i];
       tf = a.jd_final[a.jd_final_stride ? i : 0];
       x = Vec3{a.out[0][i], a.out[1][i], a.out[2][i]};
@@ -1446,16 +1524,27 @@ __global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restric
   const int64_t slot = a.slot_begin + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / WIDE_L;
   if (slot >= a.slot_end) return;
   const int64_t obj = a.perm != nullptr ? (int64_t)a.perm[slot] : slot;
-  if (a.status[obj] != ST_OK) return;
+  ParkRec st0;
+  if (a.stage != nullptr) {
+    const double* r = a.stage + 8 * obj;
+    if (isnan(r[6])) return;
+    for (int k = 0; k < 3; ++k) {
+      st0.x[k] = r[k];
+      st0.v[k] = r[3 + k];
+    }
+    st0.t = r[6];
+    st0.tf = r[7];
+  } else {
+    if (a.status[obj] != ST_OK) return;
+    st0.t = a.jd0[obj];
+    st0.tf = a.jd_final[a.jd_final_stride ? obj : 0];
+    for (int k = 0; k < 3; ++k) {
+      st0.x[k] = a.out[k][obj];
+      st0.v[k] = a.out[3 + k][obj];
+    }
+  }
   const int l = threadIdx.x & (WIDE_L - 1);
   const int g0 = (threadIdx.x & 63) & ~(WIDE_L - 1);
-  ParkRec st0;
-  st0.t = a.jd0[obj];
-  st0.tf = a.jd_final[a.jd_final_stride ? obj : 0];
-  for (int k = 0; k < 3; ++k) {
-    st0.x[k] = a.out[k][obj];
-    st0.v[k] = a.out[3 + k][obj];
-  }
   double* const rslot = &s_rec[threadIdx.x / WIDE_L][l <= WIDE_EMB ? l : 0][0];
   wide_integrate<false>(spk, coef, a, lds, obj, st0, true, l, g0, rslot, 0);
 }
@@ -1517,6 +1606,11 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   // tail hand-off: plain launch, one final epoch per object, planets only
   a.park_lanes = p.mode == 0 && !multi && p.n_ast == 0 ? p.park_lanes : 0;
   a.park_min_left = p.park_min_left;
+  // object records: the plain launch over the longest-first order
+  a.stage = p.stage != nullptr && p.mode == 0 && !multi && p.order_buf != nullptr &&
+                    p.n < (int64_t)1 << 31
+                ? p.stage
+                : nullptr;
   if (a.range)
     hipLaunchKernelGGL(init_range, dim3(1), dim3(64), 0, stream, a.range,
                        multi ? p.ep_lo : __builtin_inf(), multi ? p.ep_hi : 0.0);
