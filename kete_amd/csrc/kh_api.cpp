@@ -110,7 +110,7 @@ int32_t kh_ctx_create(int32_t device, kh_ctx** out) {
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_queue, sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->d_counts, 4 * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&c->d_range, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_range, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->d_split, sizeof(int64_t)) != hipSuccess) {
     delete c;
     return fail(KH_E_HIP, "stream/event/queue allocation failed");

@@ -45,7 +45,7 @@ struct kh_ctx {
   void* pbuf = nullptr;  // partials: states at the epochs
   size_t pbuf_size = 0;
   unsigned long long* d_queue = nullptr;
-  unsigned long long* d_range = nullptr;  // epoch range of the last N-body batch [2]
+  unsigned long long* d_range = nullptr;  // epoch ranges of the last N-body batch [4]
   unsigned long long* d_counts = nullptr;  // FOV list counters [4]
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // FOV prefilter timing of the last kh_fov_state_check: event pairs around

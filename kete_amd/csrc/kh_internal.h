@@ -65,7 +65,7 @@ struct NBodyLaunch {
   int32_t mode;   // 0 plain, 1 persistent
   int32_t block;  // threads per block (0: default)
   int32_t grid;   // persistent grid size in blocks (0: default)
-  unsigned long long* range;  // [2] epoch-range scratch, or null: no LDS window
+  unsigned long long* range;  // [4] epoch-range scratch, or null: no LDS window
   int32_t window;             // records per segment staged in LDS
   int32_t n_seg;
   void* order_buf;  // scratch for the longest-first queue order (order_buf_bytes), or null
@@ -123,7 +123,7 @@ struct ParkRec {
   double B[7][3];
 };
 
-constexpr size_t order_buf_bytes(int64_t n) { return 2 * 1024 * 4 + (size_t)n * 6 + 256; }
+constexpr size_t order_buf_bytes(int64_t n) { return 2 * 2048 * 4 + (size_t)n * 6 + 256; }
 // Bytes of NBodyLaunch::stage for n objects (one 64-byte record each).
 constexpr size_t stage_buf_bytes(int64_t n) { return (size_t)n * 64; }
 

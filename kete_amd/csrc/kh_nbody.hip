@@ -52,7 +52,7 @@ struct NBodyArgs {
   int32_t* counters;  // [n][4] or null
   unsigned long long* queue;
   int64_t first_dynamic;  // objects below this are assigned by lane id
-  unsigned long long* range;  // [2]: bits of min and max epoch over the batch
+  unsigned long long* range;  // [4]: bits of min and max epoch over the batch, then of the starting epochs
   int32_t window;             // records per segment staged in LDS (0: none)
   const uint32_t* perm;       // queue slot -> object (longest first), or null
   int64_t slot_begin, slot_end;  // the queue slots this launch takes
@@ -226,24 +226,72 @@ __global__ void nbody_prepare_kernel(const SpkDev* __restrict__ spk,
     const double tf = a.jd_final[a.jd_final_stride ? i : 0];
     double lo = st == ST_OK ? fmin(t, tf) : __builtin_inf();
     double hi = st == ST_OK ? fmax(t, tf) : 0.0;
+    // and the range of the starting epochs (where the window is centred)
+    double slo = st == ST_OK ? t : __builtin_inf();
+    double shi = st == ST_OK ? t : 0.0;
     for (int off = 32; off > 0; off >>= 1) {
       lo = fmin(lo, __shfl_xor(lo, off));
       hi = fmax(hi, __shfl_xor(hi, off));
+      slo = fmin(slo, __shfl_xor(slo, off));
+      shi = fmax(shi, __shfl_xor(shi, off));
     }
-    if ((threadIdx.x & 63) == 0 && lo <= hi && lo > 0.0) {
-      atomicMin(&a.range[0], (unsigned long long)__double_as_longlong(lo));
-      atomicMax(&a.range[1], (unsigned long long)__double_as_longlong(hi));
+    // then over the block's wavefronts, so one lane per block updates the
+    // four words (same-address atomics serialise)
+    __shared__ double red[4][4];
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      red[w][0] = lo;
+      red[w][1] = hi;
+      red[w][2] = slo;
+      red[w][3] = shi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < nw; ++k) {
+        lo = fmin(lo, red[k][0]);
+        hi = fmax(hi, red[k][1]);
+        slo = fmin(slo, red[k][2]);
+        shi = fmax(shi, red[k][3]);
+      }
+      if (lo <= hi && lo > 0.0) {
+        atomicMin(&a.range[0], (unsigned long long)__double_as_longlong(lo));
+        atomicMax(&a.range[1], (unsigned long long)__double_as_longlong(hi));
+        atomicMin(&a.range[2], (unsigned long long)__double_as_longlong(slo));
+        atomicMax(&a.range[3], (unsigned long long)__double_as_longlong(shi));
+      }
     }
   }
 }
 
 // Epoch range before the pre-pass folds in the objects' own epochs: empty,
-// or [lo, hi] (the epochs of a multi-epoch propagation).
+// or [lo, hi] (the epochs of a multi-epoch propagation); r[2..3], the range
+// of the starting epochs, empty.
 __global__ void init_range(unsigned long long* r, double lo, double hi) {
   if (threadIdx.x == 0) {
     r[0] = (unsigned long long)__double_as_longlong(lo);
     r[1] = (unsigned long long)__double_as_longlong(hi);
+    r[2] = (unsigned long long)__double_as_longlong(__builtin_inf());
+    r[3] = 0ull;
   }
+}
+
+// First record of segment g in the launch's LDS window of a.window records:
+// the window covers the batch's epoch range from its start when the range
+// fits; otherwise it is centred on the starting epochs, where every arc
+// begins and the short ones stay (arcs in both directions from a common
+// epoch), within the range.
+__device__ __forceinline__ int32_t window_first_record(const NBodyArgs& a, const SegDev& g) {
+  const double lo = (__longlong_as_double((long long)a.range[0]) - 2451545.0) * 86400.0;
+  const double hi = (__longlong_as_double((long long)a.range[1]) - 2451545.0) * 86400.0;
+  const double slo = __longlong_as_double((long long)a.range[2]);
+  const double shi = __longlong_as_double((long long)a.range[3]);
+  const double len = (double)a.window * g.intlen;
+  double w = lo;
+  if (hi - lo > len && shi >= slo) {
+    const double c = (0.5 * (slo + shi) - 2451545.0) * 86400.0;
+    w = fmax(lo, fmin(c - 0.5 * len, hi - len));
+  }
+  return record_index(g, w);
 }
 
 // Post-pass: back to the input center and frame at jd_final (mod.rs:115-116).
@@ -361,7 +409,10 @@ __device__ __forceinline__ int64_t fetch(const NBodyArgs& a, int64_t slot, doubl
 // axis) packed them to 68%; the eccentricity factor takes it to 85%.  The order
 // changes which lane takes an object, never its result.
 // ---------------------------------------------------------------------------
-constexpr int ORDER_BUCKETS = 1024;
+// eighth-octave cost levels, each split by the direction of the arc, so that
+// the lanes of a wavefront also move through time the same way
+constexpr int ORDER_LEVELS = 1024;
+constexpr int ORDER_BUCKETS = 2 * ORDER_LEVELS;
 constexpr int ORDER_POINTS = 9;
 
 __device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
@@ -404,8 +455,9 @@ __device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
                 (v.x * hv.y - v.y * hv.x) * (1.0 / GMS) - x.z / r0};
   const double key = fabs(span) * (sum / ORDER_POINTS) * exp(fmin(dot(ev, ev), 4.0));
   if (!(key > 0.0) || !isfinite(key)) return 0;
-  const int b = (int)floor(log2(key) * 8.0) + ORDER_BUCKETS / 2;
-  return b < 1 ? 1 : (b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b);
+  int lv = (int)floor(log2(key) * 8.0) + ORDER_LEVELS / 2;
+  lv = lv < 1 ? 1 : (lv > ORDER_LEVELS - 1 ? ORDER_LEVELS - 1 : lv);
+  return 2 * lv + (span < 0.0 ? 1 : 0);
 }
 
 // Bucket of each object and the bucket histogram (privatised in LDS).
@@ -434,24 +486,68 @@ __global__ void order_zero_kernel(uint32_t* __restrict__ hist) {
 // latency kernel: the objects whose cost key exceeds the average key load of
 // one resident lane of the main launch (`lanes`), at most `cap` of them --
 // objects that would otherwise outlast everything else.
-__global__ void order_scan_kernel(const uint32_t* __restrict__ hist, uint32_t* __restrict__ off,
-                                  int64_t* __restrict__ split, double lanes, int64_t cap) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t run = 0;
-  double load = 0.0;
-  for (int b = ORDER_BUCKETS - 1; b >= 0; --b) {
-    off[b] = run;
-    run += hist[b];
-    if (b > 0) load += (double)hist[b] * exp2((b - ORDER_BUCKETS / 2 + 0.5) / 8.0);
+__global__ void __launch_bounds__(256) order_scan_kernel(const uint32_t* __restrict__ hist,
+                                                         uint32_t* __restrict__ off,
+                                                         int64_t* __restrict__ split,
+                                                         double lanes, int64_t cap) {
+  // 256 threads, each over eight consecutive buckets of the descending order
+  constexpr int PER = ORDER_BUCKETS / 256;
+  static_assert(ORDER_BUCKETS % 256 == 0, "buckets per thread");
+  __shared__ uint32_t s_cnt[256];
+  __shared__ double s_load[256];
+  __shared__ unsigned long long s_brk;
+  const int t = threadIdx.x;
+  uint32_t c[PER];
+  uint32_t tot = 0;
+  double ld = 0.0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int b = ORDER_BUCKETS - 1 - (t * PER + j);
+    c[j] = hist[b];
+    tot += c[j];
+    if (b > 1) ld += (double)c[j] * exp2(((b >> 1) - ORDER_LEVELS / 2 + 0.5) / 8.0);
+  }
+  s_cnt[t] = tot;
+  s_load[t] = ld;
+  __syncthreads();
+  if (t == 0) {
+    // exclusive prefix of the threads' counts, and the total key load
+    uint32_t run = 0;
+    double total = 0.0;
+    for (int k = 0; k < 256; ++k) {
+      const uint32_t v = s_cnt[k];
+      s_cnt[k] = run;
+      run += v;
+      total += s_load[k];
+    }
+    s_load[0] = total;
+    s_brk = ~0ull;
+  }
+  __syncthreads();
+  uint32_t run = s_cnt[t];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    off[ORDER_BUCKETS - 1 - (t * PER + j)] = run;
+    run += c[j];
   }
   if (split == nullptr) return;
-  const double per_lane = load / lanes;
-  int64_t k = 0;
-  for (int b = ORDER_BUCKETS - 1; b > 0; --b) {
-    if (exp2((b - ORDER_BUCKETS / 2) / 8.0) <= per_lane || k + (int64_t)hist[b] > cap) break;
-    k += hist[b];
+  // the head for the latency kernel: from the top, the buckets whose key
+  // exceeds a resident lane's share of the load, at most `cap` objects; the
+  // first bucket that fails (the lowest two always do) ends it
+  const double per_lane = s_load[0] / lanes;
+  run = s_cnt[t];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int pos = t * PER + j, b = ORDER_BUCKETS - 1 - pos;
+    if (b <= 1 || exp2(((b >> 1) - ORDER_LEVELS / 2) / 8.0) <= per_lane ||
+        (int64_t)run + (int64_t)c[j] > cap) {
+      atomicMin(&s_brk, ((unsigned long long)pos << 32) | run);
+      break;
+    }
+    run += c[j];
   }
-  *split = k;
+  __syncthreads();
+  if (t == 0) *split = (int64_t)(s_brk & 0xffffffffull);
 }
 
 // perm[position] = object; positions within a bucket are claimed per block.
@@ -806,13 +902,11 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       lds.inv_h[c] = RT.inv_h[c];
     }
   }
-  // Stage the records covering the batch's earliest epoch onwards: a window
-  // of a.window records per segment (a one-year arc needs 13 of 32 days).
+  // Stage a window of a.window records per segment (a one-year arc needs 13
+  // of 32 days), placed by window_first_record.
   if (a.window > 0) {
-    const double tlo = __longlong_as_double((long long)a.range[0]);
-    const double jlo = (tlo - 2451545.0) * 86400.0;
     if (threadIdx.x < spk->n_seg)
-      kh_win_w0[threadIdx.x] = record_index(spk->seg[threadIdx.x], jlo);
+      kh_win_w0[threadIdx.x] = window_first_record(a, spk->seg[threadIdx.x]);
     __syncthreads();
     const int per_seg = a.window * 44;
     for (int k = threadIdx.x; k < spk->n_seg * per_seg; k += blockDim.x) {
@@ -1239,10 +1333,8 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
     }
   }
   if (a.window > 0) {
-    const double tlo = __longlong_as_double((long long)a.range[0]);
-    const double jlo = (tlo - 2451545.0) * 86400.0;
     if (threadIdx.x < spk->n_seg)
-      kh_win_w0[threadIdx.x] = record_index(spk->seg[threadIdx.x], jlo);
+      kh_win_w0[threadIdx.x] = window_first_record(a, spk->seg[threadIdx.x]);
     __syncthreads();
     const int per_seg = a.window * 44;
     for (int k = threadIdx.x; k < spk->n_seg * per_seg; k += blockDim.x) {
@@ -1630,7 +1722,7 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     // object goes to the latency kernel (1 by default)
     double share = 1.0;
     if (const char* e = getenv("KH_LATENCY_SHARE")) share = atof(e) > 0.0 ? atof(e) : 1.0;
-    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(64), 0, stream, hist, off, split,
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, stream, hist, off, split,
                        lanes / share, p.latency_k);
     hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off, perm);
     a.perm = perm;
