@@ -780,9 +780,10 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // outweighs a lane's share of the batch, at most a cap (KH_LATENCY_K sets
   // it).  The latency kernel: sixteen lanes per object for batches of uneven
   // arcs (per-object final epochs; planets only), whose heavy tail it
-  // shortens (2e6 objects over 1-3650 d with the tail hand-off: 422 ms with
-  // up to 16 x n_cu objects in it, 434-439 ms with 8, 32 or 64 x n_cu, 682
-  // ms with the one-lane variant, 680 ms with none); the one-lane variant,
+  // shortens (2e6 objects over 1-3650 d with the tail hand-off: 316 ms with
+  // up to 32 x n_cu objects in it, 317-320 ms with 64-256 x n_cu (the key
+  // threshold then decides), 340 ms with 16, 379 ms with 8, 640 ms with
+  // none; more objects by a lower threshold, 600 ms); the one-lane variant,
   // up to 8 x n_cu objects, for per-object final epochs with
   // include_asteroids.  For a common final epoch the main launch alone is
   // faster: planets only, its tail hand-off takes the slow objects (67.5 vs
@@ -797,7 +798,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     if (lv && std::string(lv) == "ilp") wide = false;
     p.wide_latency = wide && !include_asteroids;
     p.latency_k = !c->spk.shared_grid || jd_final_stride == 0 ? 0
-                  : p.wide_latency                             ? (int64_t)c->n_cu * 16
+                  : p.wide_latency                             ? (int64_t)c->n_cu * 32
                                                                : (int64_t)c->n_cu * 8;
   }
   if (const char* k = getenv("KH_LATENCY_K")) p.latency_k = c->spk.shared_grid ? atoll(k) : 0;

@@ -1178,5 +1178,14 @@ __device__ __forceinline__ double row_sum(double v) {
   return v;
 }
 
+// v of lane K of this lane's quad (quad_perm [K,K,K,K])
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v) {
+  return dpp_row<K * 0x55>(v);
+}
+template <int K>
+__device__ __forceinline__ int quad_bcast_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
+}
 
 }  // namespace kh

@@ -593,14 +593,19 @@ __shared__ int32_t kh_park_n[4];
 // -- its Chebyshev record, its Newton, GR and J2 terms and, for the Sun, the
 // non-grav term (add_body_at) -- and lane 10 the Earth-Moon barycentre
 // segment the Earth and Moon lanes add to their positions.  The row sums the
-// terms (row_sum: the same bits in every lane), and every lane of the row
-// then runs the same integrator arithmetic as the one-lane kernel on
-// identical values, so the row takes identical decisions without exchanging
-// them.  A force evaluation thus costs one body's terms and the integrator's
-// bookkeeping per lane instead of the whole force sum: the latency of a slow
-// object's serial chain drops, at a sixteenth of the one-lane kernel's
-// objects per wavefront.  Results differ from the one-lane kernel's only by
-// the order in which the body terms are summed.  Planets only.
+// terms (row_sum: the same bits in every lane).  The integrator is split by
+// coordinate: lane c of each quad of the row (c < 3; lane 3 repeats
+// coordinate 2) holds x_c, v_c, a_c, B[7]_c and G[7]_c and runs the one-lane
+// kernel's arithmetic for that coordinate; the predicted position and
+// velocity reach every lane by quad broadcasts, and the convergence test and
+// the step-size maximum combine the quad's three coordinates.  The control
+// values (t, h, phase, counters) are the same in every lane, so the row takes
+// identical decisions without exchanging them.  A force evaluation thus
+// costs one body's terms and a third of the integrator's bookkeeping per
+// lane instead of the whole force sum: the latency of a slow object's serial
+// chain drops, at a sixteenth of the one-lane kernel's objects per wavefront.
+// Results differ from the one-lane kernel's only by the order in which the
+// body terms are summed.  Planets only.
 // fresh: start at st.t from st.x, st.v (the integrator's first exact
 // evaluation); otherwise st is a step boundary (ParkRec) and the object
 // continues with its first substep.  Records: WINDOW reads them from the
@@ -617,8 +622,14 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
   const double eps = a.epsilon;
   double t = st0.t;
   const double tf = st0.tf;
-  Vec3 x{st0.x[0], st0.x[1], st0.x[2]};
-  Vec3 v{st0.v[0], st0.v[1], st0.v[2]};
+  // this lane's coordinate of the integrator (lane 3 of each quad repeats
+  // coordinate 2, so quad reductions see only the three coordinates)
+  const int cd = (l & 3) < 3 ? (l & 3) : 2;
+  const auto pick = [cd](double c0, double c1, double c2) {
+    return cd == 0 ? c0 : (cd == 1 ? c1 : c2);
+  };
+  double xd = pick(st0.x[0], st0.x[1], st0.x[2]);
+  double vd = pick(st0.v[0], st0.v[1], st0.v[2]);
   const NonGravDev* ngp = (a.ng && a.ng[obj].kind != 0) ? &a.ng[obj] : nullptr;
 
   // this lane's body (lanes 0..9) or segment (lane 10); lanes 11..15 none
@@ -632,22 +643,19 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
   const bool cache_rec = all14 && sg.rsize == 44;
   int32_t rec_cached = -1;
 
-  double h = 0.0, h_next = 0.0;
-  Vec3 a0{0, 0, 0};
-  double B[7][3], G[7][3];
+  double h = 0.0, h_next = 0.0, ad = 0.0;
+  double B[7], G[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) B[k][0] = B[k][1] = B[k][2] = G[k][0] = G[k][1] = G[k][2] = 0.0;
+  for (int k = 0; k < 7; ++k) B[k] = G[k] = 0.0;
   int32_t phase = PH_INIT, j = 1, sweep = 0, fails = 0;
   int32_t n_force = 0, n_sweep = 0, n_acc = 0, n_fail = 0;
   if (!fresh) {
     // the first substep of the step that starts at st0 (the state after
     // `begin` in the one-lane kernel)
     h = st0.h;
-    a0 = Vec3{st0.a[0], st0.a[1], st0.a[2]};
+    ad = pick(st0.a[0], st0.a[1], st0.a[2]);
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) B[k][d] = st0.B[k][d];
+    for (int k = 0; k < 7; ++k) B[k] = pick(st0.B[k][0], st0.B[k][1], st0.B[k][2]);
     phase = PH_SUB;
     n_force = st0.n_force;
     n_sweep = st0.n_sweep;
@@ -657,32 +665,27 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
 
   for (;;) {
     const bool sub = phase == PH_SUB;
-    // ---- evaluation point: predictor at spacing j (radau.rs:252-278) ----
+    // ---- evaluation point: predictor at spacing j (radau.rs:252-278), this
+    // lane's coordinate, then the quad's three coordinates in every lane ----
     const double s = lds.h[j];
-    double xe[3], ve[3];
+    double xe_d, ve_d;
     {
-      const double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
-      const double as[3] = {a0.x, a0.y, a0.z};
-      double bw[3] = {0.0, 0.0, 0.0}, bu[3] = {0.0, 0.0, 0.0};
+      double bw = 0.0, bu = 0.0;
       double sp = 1.0;
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
         sp *= s;
         const double wk = sp * RTC.w[k], uk = sp * RTC.u[k];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          bw[d] += B[k][d] * wk;
-          bu[d] += B[k][d] * uk;
-        }
+        bw += B[k] * wk;
+        bu += B[k] * uk;
       }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double px = xs[d] + s * vs[d] * h + (s * s) * (h * h) * (as[d] / 2.0 + bw[d]);
-        double pv = vs[d] + s * h * (as[d] + bu[d]);
-        xe[d] = sub ? px : xs[d];
-        ve[d] = sub ? pv : vs[d];
-      }
+      const double px = xd + s * vd * h + (s * s) * (h * h) * (ad / 2.0 + bw);
+      const double pv = vd + s * h * (ad + bu);
+      xe_d = sub ? px : xd;
+      ve_d = sub ? pv : vd;
     }
+    const double xe[3] = {quad_bcast<0>(xe_d), quad_bcast<1>(xe_d), quad_bcast<2>(xe_d)};
+    const double ve[3] = {quad_bcast<0>(ve_d), quad_bcast<1>(ve_d), quad_bcast<2>(ve_d)};
     const double te = sub ? t + s * h : t;
 
     // ---- this lane's share of the force evaluation ----
@@ -749,78 +752,60 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
       }
     }
     const double ev[3] = {row_sum(acc.x), row_sum(acc.y), row_sum(acc.z)};
+    const double ev_d = pick(ev[0], ev[1], ev[2]);
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
     {
-      const double as[3] = {a0.x, a0.y, a0.z};
-      double gk[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) gk[d] = (ev[d] - as[d]) * lds.inv_h[j];
+      double gk = (ev_d - ad) * lds.inv_h[j];
 #pragma unroll
       for (int z = 1; z < 7; ++z) {
         const double rd = lds.rdiff[j][z];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gk[d] = z < j ? (gk[d] - G[z - 1][d]) * rd : gk[d];
+        gk = z < j ? (gk - G[z - 1]) * rd : gk;
       }
 #pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        const bool put = sub && q == j - 1;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) G[q][d] = put ? gk[d] : G[q][d];
-      }
+      for (int q = 0; q < 7; ++q) G[q] = sub && q == j - 1 ? gk : G[q];
     }
-    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319) ----
+    // ---- end of sweep: B = G*C and the convergence test (radau.rs:313-319),
+    // the test's three coordinates combined over the quad ----
     const bool es = sub && j == 7 && st == ST_OK;
-    const double bp6[3] = {B[6][0], B[6][1], B[6][2]};
+    const double bp6 = B[6];
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
+      double acc2 = G[c];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double acc2 = G[c][d];
-#pragma unroll
-        for (int k = c + 1; k < 7; ++k) acc2 += G[k][d] * RTC.c[k][c];
-        B[c][d] = es ? acc2 : B[c][d];
-      }
+      for (int k = c + 1; k < 7; ++k) acc2 += G[k] * RTC.c[k][c];
+      B[c] = es ? acc2 : B[c];
     }
-    double fmx[3];
-    bool small = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      fmx[d] = fabs(ev[d]) + 1e-6;
-      small = small && fabs(B[6][d] - bp6[d]) < 1e-14 * fmx[d];
-    }
+    const int32_t small_d = fabs(B[6] - bp6) < 1e-14 * (fabs(ev_d) + 1e-6) ? 1 : 0;
+    const bool small = (quad_bcast_i<0>(small_d) & quad_bcast_i<1>(small_d) &
+                        quad_bcast_i<2>(small_d)) != 0;
     const bool conv = es && small;
     // ---- accepted step (radau.rs:320-338) ----
     {
-      double xs[3] = {x.x, x.y, x.z}, vs[3] = {v.x, v.y, v.z};
-      const double as[3] = {a0.x, a0.y, a0.z};
+      double bw = 0.0, bu = 0.0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        double bw = 0.0, bu = 0.0;
-#pragma unroll
-        for (int k = 0; k < 7; ++k) {
-          bw += B[k][d] * RTC.w[k];
-          bu += B[k][d] * RTC.u[k];
-        }
-        double nx = xs[d] + (vs[d] * h + (h * h) * (as[d] * 0.5 + bw));
-        double nv = vs[d] + h * (as[d] + bu);
-        xs[d] = conv ? nx : xs[d];
-        vs[d] = conv ? nv : vs[d];
+      for (int k = 0; k < 7; ++k) {
+        bw += B[k] * RTC.w[k];
+        bu += B[k] * RTC.u[k];
       }
-      x = Vec3{xs[0], xs[1], xs[2]};
-      v = Vec3{vs[0], vs[1], vs[2]};
+      const double nx = xd + (vd * h + (h * h) * (ad * 0.5 + bw));
+      const double nv = vd + h * (ad + bu);
+      xd = conv ? nx : xd;
+      vd = conv ? nv : vd;
     }
     if (conv) {
       // next step size (radau.rs:339-342)
-      double fm = fmax(fmax(fmx[0], fmx[1]), fmx[2]);
-      double bm = fmax(fmax(fabs(B[6][0]), fabs(B[6][1])), fabs(B[6][2]));
+      const double fm =
+          fmax(fmax(fabs(ev[0]) + 1e-6, fabs(ev[1]) + 1e-6), fabs(ev[2]) + 1e-6);
+      const double b6 = fabs(B[6]);
+      const double bm = fmax(fmax(quad_bcast<0>(b6), quad_bcast<1>(b6)), quad_bcast<2>(b6));
       double ratio = root7(eps * fm / bm);
       ratio = ratio < 0.25 ? 0.25 : (ratio > 4.0 ? 4.0 : ratio);
       h_next = h * ratio;
     }
     t = conv ? t + h : t;
     const bool take_a = st == ST_OK && !sub;
-    a0 = Vec3{take_a ? ev[0] : a0.x, take_a ? ev[1] : a0.y, take_a ? ev[2] : a0.z};
+    ad = take_a ? ev_d : ad;
 
     // ---- scalar control (radau.rs:160-218) ----
     int32_t done = -1;
@@ -871,8 +856,10 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
       phase = PH_SUB;
     }
     if (done >= 0) {
+      const Vec3 xo{quad_bcast<0>(xd), quad_bcast<1>(xd), quad_bcast<2>(xd)};
+      const Vec3 vo{quad_bcast<0>(vd), quad_bcast<1>(vd), quad_bcast<2>(vd)};
       if (l == 0)
-        store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+        store_result(a, obj, done, xo, vo, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
       return;
     }
   }
@@ -1571,38 +1558,19 @@ __global__ void __launch_bounds__(256) radau3_nbody_kernel(const SpkDev* __restr
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 
-// Blocks of `kernel` that are resident at once on the whole device: the
-// persistent launch must not exceed it, or the blocks beyond it start only
-// after the first ones have drained the queue.
 // ---------------------------------------------------------------------------
 // Wide latency variant for the slowest objects (the head of the longest-
-// first order, on the second stream beside the main launch): sixteen lanes
-// per object, one DPP row, four objects per wavefront.  Lane l < 10
-// evaluates body l of kete's PLANETS list (gravity.rs:145-148) -- its
-// Chebyshev record cached in LDS, its Newton, GR and J2 terms and, for the
-// Sun, the non-grav term (add_body_at) -- and lane 10 the Earth-Moon
-// barycentre segment the Earth and Moon lanes add to their positions.  The
-// row sums the terms (row_sum: the same bits in every lane), and every lane
-// of the row then runs the same integrator arithmetic as the one-lane kernel
-// on identical values, so the row takes identical decisions without
-// exchanging them.  A force evaluation thus costs one body's terms and the
-// integrator's bookkeeping per lane instead of the whole force sum: the
-// latency of the slowest objects' serial chains drops, at a sixteenth of
-// the one-lane kernel's objects per wavefront.  Results differ from the
-// one-lane kernel's only by the order in which the body terms are summed.
-// Planets only (the asteroid list runs in the one-lane latency kernel).
+// first order, beside the main launch): four objects per wavefront, each on
+// sixteen lanes (wide_integrate), each lane's record cached in LDS.  One
+// wavefront per block and at most 256 VGPRs, so a SIMD holds one of these
+// waves beside a wave of the main launch (the main launch's blocks need 256
+// VGPRs per wave and the LDS window, a block here 16 KB of LDS).
 // ---------------------------------------------------------------------------
-// ---------------------------------------------------------------------------
-// Wide latency variant for the slowest objects (the head of the longest-
-// first order, on the second stream beside the main launch): four objects
-// per wavefront, each on sixteen lanes (wide_integrate), each lane's record
-// cached in LDS.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restrict__ spk,
-                                                         const double* __restrict__ coef,
-                                                         NBodyArgs a) {
+constexpr int WIDE_BLOCK = 64;
+__global__ void __launch_bounds__(WIDE_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8)))
+radau_wide_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef, NBodyArgs a) {
   __shared__ LdsTables lds;
-  __shared__ __align__(16) double s_rec[256 / WIDE_L][WIDE_EMB + 1][44];
+  __shared__ __align__(16) double s_rec[WIDE_BLOCK / WIDE_L][WIDE_EMB + 1][44];
   for (int k = threadIdx.x; k < 8 * 8; k += blockDim.x) {
     const int r = k / 8, c = k % 8;
     lds.rdiff[r][c] = RT.rdiff[r][c];
@@ -1641,6 +1609,9 @@ __global__ void __launch_bounds__(256) radau_wide_kernel(const SpkDev* __restric
   wide_integrate<false>(spk, coef, a, lds, obj, st0, true, l, g0, rslot, 0);
 }
 
+// Blocks of `kernel` that are resident at once on the whole device: the
+// persistent launch must not exceed it, or the blocks beyond it start only
+// after the first ones have drained the queue.
 template <typename K>
 static int64_t resident_grid(K kernel, int block, size_t lds_bytes, int32_t n_cu) {
   int per_cu = 0;
@@ -1770,6 +1741,14 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     // latency variant on the second stream, concurrently with the rest
     const bool hybrid = !multi && a.perm != nullptr && p.latency_k > 0 && p.split &&
                         p.stream2 && p.fork && p.join;
+    // the sixteen-lane latency kernel goes on the launch stream and the main
+    // kernel on the second one, behind the fork wait: the latency kernel's
+    // one-wavefront blocks are then dispatched first and hold their place on
+    // the SIMDs beside the main launch's (uneven arcs: 340 ms, 383 ms with
+    // the main kernel dispatched first); the one-lane variant goes on the
+    // second stream
+    const bool wide = p.wide_latency && p.n_ast == 0;
+    hipStream_t s_lat = stream, s_main = stream;
     if (ev0) (void)hipEventRecord(ev0, stream);
     if (hybrid) {
       // up to latency_k slots; the kernel reads how many from *split
@@ -1780,17 +1759,21 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       const int64_t k_max = p.latency_k < p.n ? p.latency_k : p.n;
       (void)hipEventRecord(p.fork, stream);
       (void)hipStreamWaitEvent(p.stream2, p.fork, 0);
-      if (p.wide_latency && p.n_ast == 0)
-        hipLaunchKernelGGL(radau_wide_kernel, dim3((unsigned)((k_max * WIDE_L + 255) / 256)),
-                           dim3(256), 0, p.stream2, spk, coef, al);
+      if (wide)
+        s_main = p.stream2;
+      else
+        s_lat = p.stream2;
+      if (wide)
+        hipLaunchKernelGGL(radau_wide_kernel,
+                           dim3((unsigned)((k_max * WIDE_L + WIDE_BLOCK - 1) / WIDE_BLOCK)),
+                           dim3(WIDE_BLOCK), 0, s_lat, spk, coef, al);
       else
         with_ast(p.n_ast, [&](auto ast) {
           constexpr bool A = decltype(ast)::value;
           hipLaunchKernelGGL((radau_nbody_kernel<false, true, false, A>),
-                             dim3((unsigned)((k_max + 63) / 64)), dim3(64), 0, p.stream2, spk,
+                             dim3((unsigned)((k_max + 63) / 64)), dim3(64), 0, s_lat, spk,
                              coef, al);
         });
-      (void)hipEventRecord(p.join, p.stream2);
       a.split = p.split;
       a.part = 2;
     }
@@ -1799,11 +1782,12 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       constexpr bool A = decltype(ast)::value;
       if (multi)
         hipLaunchKernelGGL((radau_nbody_kernel<false, false, true, A>), dim3((unsigned)grid),
-                           dim3(block), lds_bytes, stream, spk, coef, a);
+                           dim3(block), lds_bytes, s_main, spk, coef, a);
       else
         hipLaunchKernelGGL((radau_nbody_kernel<false, false, false, A>), dim3((unsigned)grid),
-                           dim3(block), lds_bytes, stream, spk, coef, a);
+                           dim3(block), lds_bytes, s_main, spk, coef, a);
     });
+    if (hybrid) (void)hipEventRecord(p.join, p.stream2);
     if (hybrid) (void)hipStreamWaitEvent(stream, p.join, 0);
     if (ev1) (void)hipEventRecord(ev1, stream);
   }

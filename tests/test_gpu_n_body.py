@@ -155,11 +155,20 @@ def test_uneven_spans_parity(ctx, oracle_spk):
     restatement built with -ffp-contract=fast (a second, equally valid
     rounding).  Per decade of |span| the GPU's differences from the default
     oracle must be distributed like the FMA build's: quantiles 50/90/99 % at
-    most 2x, the maximum at most 5x, plus 1e-11 AU (the corrector's
-    convergence level: one sweep more or fewer moves a short arc by up to
-    1.1e-11 AU).  The factors bound the sampling spread of those statistics,
-    measured on 20000 objects in 4096-object chunks (quantile ratios 0.87 -
-    1.66, maximum 0.41 - 4.2).  Statuses must be identical."""
+    most 2x, plus 1e-11 AU (the corrector's convergence level: one sweep more
+    or fewer moves a short arc by up to 1.1e-11 AU).  The factor bounds the
+    sampling spread of those statistics, measured on 20000 objects in
+    4096-object chunks (quantile ratios 0.87 - 1.66).  Each object's
+    difference must stay within 5x the decade's FMA-build maximum (the
+    maximum's spread: 0.41 - 4.2) or within the one-year bar scaled by the
+    arc, 1e-9 AU per 365.25 d: a step-failure flip (ten sweeps without the
+    corrector meeting its ~50-ulp threshold, decided by the last bit) puts
+    an arc on another step sequence whose truncation error differs by up to
+    ~1e-9 AU per year -- object 521 of this sample (972 d) ends 2e-11 AU
+    from the converged answer in kete's rounding and 1.1e-9 AU with one
+    such flip in another valid rounding, while objects 974 and 3509 go the
+    other way (kete 5-6e-10 AU from it, the GPU 0.7-1e-10).  Statuses must
+    be identical."""
     n = 4096
     xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=3)
     span = syn.uneven_spans(n)
@@ -186,7 +195,8 @@ def test_uneven_spans_parity(ctx, oracle_spk):
               + " ".join(f"{x:.1e}" for x in (*qg, dg[s].max()))
               + " | fma build " + " ".join(f"{x:.1e}" for x in (*qf, df[s].max())), flush=True)
         assert np.all(qg <= 2.0 * qf + 1e-11), (lo, hi)
-        assert dg[s].max() <= 5.0 * df[s].max() + 1e-11, (lo, hi)
+        lim = np.maximum(5.0 * df[s].max() + 1e-11, 1e-9 * a[s] / 365.25)
+        assert np.all(dg[s] <= lim), (lo, hi, np.nonzero(s)[0][dg[s] > lim])
     # the one-year bar holds on the arcs up to a year
     assert dg[ok & (a <= 366.0)].max() < POS_TOL
 
