@@ -831,14 +831,21 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
     const char* ord = getenv("KH_ORDER");
     if (!(ord && atoi(ord) == 0)) {
       // the allocating entry points also stage each object as one 64-byte
-      // record after the order (the no-alloc entry point's scratch holds the
-      // order only)
+      // record after the order, and per-object final epochs the radix sort's
+      // keys and scratch after those (the no-alloc entry point's scratch
+      // holds the order only: counting sort, no records)
       const size_t ob = align256(order_buf_bytes(n));
+      const size_t sb = align256(stage_buf_bytes(n));
+      const size_t rb = jd_final_stride != 0 && !me ? order_sort_temp_bytes(n) : 0;
       if (order_scratch) {
         p.order_buf = order_scratch;
-      } else if (kh_grow(&c->obuf, &c->obuf_size, ob + stage_buf_bytes(n)) == KH_OK) {
+      } else if (kh_grow(&c->obuf, &c->obuf_size, ob + sb + rb) == KH_OK) {
         p.order_buf = c->obuf;
         p.stage = reinterpret_cast<double*>(static_cast<char*>(c->obuf) + ob);
+        if (rb) {
+          p.sort_buf = static_cast<char*>(c->obuf) + ob + sb;
+          p.sort_bytes = rb;
+        }
       } else {
         return KH_E_HIP;
       }

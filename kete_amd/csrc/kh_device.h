@@ -919,7 +919,8 @@ __device__ __forceinline__ int32_t add_body(const SpkDev* __restrict__ spk,
 // slot is the record's position in the window (the same for every segment).
 // UNI: the wavefront's record is uniform and `base` points at its
 // coefficients in segment 0's numbering (coefficients then come through
-// scalar loads); otherwise they are read from the LDS window at `slot`.
+// scalar loads); otherwise they are read from the LDS window `base` at
+// `slot`.
 template <bool UNI>
 __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ spk,
                                                      const double* __restrict__ base,
@@ -930,7 +931,7 @@ __device__ __forceinline__ int32_t accel_shared_grid(const SpkDev* __restrict__ 
   const double vscale = 86400.0 * inv_radius * INV_AU_KM;
   auto coefs = [&](int s) -> const double* {
     if (UNI) return base + spk->seg[s].off;
-    return kh_win_rec + (s * win + slot) * 44 + 2;
+    return base + (s * win + slot) * 44 + 2;
   };
   Vec3 emb = cheb_pos_n<14>(coefs(spk->emb_seg), xs);
   emb = Vec3{emb.x * INV_AU_KM, emb.y * INV_AU_KM, emb.z * INV_AU_KM};
@@ -1058,11 +1059,16 @@ __device__ __forceinline__ int32_t spk_accel_ilp(const SpkDev* __restrict__ spk,
 // n_ast != 0, the five asteroids that complete MASSES.  The Earth-Moon
 // barycentre is evaluated once and shared by 399 and 301 (kete evaluates it
 // for each, with identical results).
+// win > 0: records in an LDS window of win records per segment -- the
+// block's (kh_win_rec, first records kh_win_w0) or, with wwin, a wavefront's
+// own window wwin whose first record is ww0 in every segment (shared grid
+// only; lanes outside it take the per-body path without a window).
 __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                              const double* __restrict__ coef, double t, Vec3 x,
                                              Vec3 v, bool exact, const NonGravDev* ng, Vec3& acc,
                                              int32_t& impact_id, int32_t win = 0,
-                                             int32_t n_ast = 0) {
+                                             int32_t n_ast = 0, const double* wwin = nullptr,
+                                             int32_t ww0 = 0) {
   if (spk->shared_grid) {
     const SegDev& g = spk->seg[spk->emb_seg];
     if (t < g.jd_start || t > g.jd_end) return ST_DAF_LIMITS;
@@ -1079,14 +1085,16 @@ __device__ __forceinline__ int32_t spk_accel(const SpkDev* __restrict__ spk,
                                      (js - rec[0]) * ir, ir, x, v, exact, ng, acc, impact_id,
                                      n_ast);
     }
-    const int32_t loc = win > 0 ? ri - kh_win_w0[spk->emb_seg] : -1;
+    const int32_t loc = win > 0 ? ri - (wwin ? ww0 : kh_win_w0[spk->emb_seg]) : -1;
     if (win > 0 && __all(loc >= 0 && loc < win)) {
-      const double* r0 = kh_win_rec + (spk->emb_seg * win + loc) * 44;
+      const double* wr = wwin ? wwin : kh_win_rec;
+      const double* r0 = wr + (spk->emb_seg * win + loc) * 44;
       const double ir = r0[1];  // staged as 1 / radius
-      return accel_shared_grid<false>(spk, nullptr, loc, win, t, (js - r0[0]) * ir, ir, x, v,
-                                      exact, ng, acc, impact_id, n_ast);
+      return accel_shared_grid<false>(spk, wr, loc, win, t, (js - r0[0]) * ir, ir, x, v, exact,
+                                      ng, acc, impact_id, n_ast);
     }
   }
+  if (wwin) win = 0;
   Vec3 a{0.0, 0.0, 0.0};
   Vec3 emb;
   int32_t st = seg_pos(spk, coef, spk->emb_seg, t, emb, win);

@@ -98,6 +98,11 @@ struct NBodyLaunch {
   // [n][8] doubles, 64-byte aligned: the object records of the plain launch
   // over the longest-first order (stage_buf_bytes), or null
   double* stage;
+  // order_sort_temp_bytes(n) bytes: the order is radix-sorted by cost bucket
+  // and arc length (per-object final epochs), or null: counting-sorted by
+  // cost bucket
+  void* sort_buf;
+  size_t sort_bytes;
 };
 
 // Shared epochs of a multi-epoch propagation (device array, integration
@@ -109,7 +114,6 @@ struct MultiEpochs {
   double* out[6];
 };
 
-// Bytes of NBodyLaunch::order_buf for n objects.
 // An object parked by the plain N-body launch at a step boundary: the
 // integrator state a new step starts from (radau.rs:183-215 between two
 // calls of step): time, final time, the step about to be tried, position,
@@ -123,7 +127,18 @@ struct ParkRec {
   double B[7][3];
 };
 
+// Bytes of NBodyLaunch::order_buf for n objects.
 constexpr size_t order_buf_bytes(int64_t n) { return 2 * 2048 * 4 + (size_t)n * 6 + 256; }
+// Radix sort of the order (kh_sort.hip): NBodyLaunch::sort_buf holds the keys
+// and object indices the order pre-pass writes, the sorted keys and hipcub's
+// scratch, order_sort_temp_bytes(n) in all.
+constexpr size_t order_sort_span(int64_t n) { return ((size_t)n * 4 + 255) & ~(size_t)255; }
+inline uint32_t* order_sort_keys(void* buf, int64_t n) { return static_cast<uint32_t*>(buf); }
+inline uint32_t* order_sort_vals(void* buf, int64_t n) {
+  return reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + order_sort_span(n));
+}
+size_t order_sort_temp_bytes(int64_t n);
+int32_t launch_order_sort(int64_t n, void* buf, size_t bytes, uint32_t* perm, hipStream_t s);
 // Bytes of NBodyLaunch::stage for n objects (one 64-byte record each).
 constexpr size_t stage_buf_bytes(int64_t n) { return (size_t)n * 64; }
 

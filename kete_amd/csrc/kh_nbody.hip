@@ -294,6 +294,65 @@ __device__ __forceinline__ int32_t window_first_record(const NBodyArgs& a, const
   return record_index(g, w);
 }
 
+// Wavefront windows.  When the batch's epochs span more than the block's LDS
+// window covers (long arcs), each wavefront of the plain launch keeps a
+// window of its own -- ww_records(a) consecutive records of every segment,
+// its share of the block's LDS -- and moves it along with its lanes: when a
+// force evaluation has lanes outside it, the window moves to start at the
+// record of the lanes' earliest step start (latest, for arcs run backwards)
+// if that is further along.  Step starts only advance, so the window only
+// moves forward and each record is staged once; evaluations with lanes
+// outside the window take the per-body path from HBM.  Shared record grids
+// only (one record index for every segment).
+__device__ __forceinline__ int32_t ww_records(const NBodyArgs& a) {
+  return a.window * 64 / (int32_t)blockDim.x;
+}
+
+__device__ __forceinline__ bool ww_mode(const NBodyArgs& a, const SpkDev* __restrict__ spk) {
+  if (a.window <= 0 || !spk->shared_grid || !spk->all14 || ww_records(a) < 2) return false;
+  const SegDev& g = spk->seg[spk->emb_seg];
+  if (g.n_rec < ww_records(a)) return false;
+  const double lo = (__longlong_as_double((long long)a.range[0]) - 2451545.0) * 86400.0;
+  const double hi = (__longlong_as_double((long long)a.range[1]) - 2451545.0) * 86400.0;
+  return hi - lo > (double)a.window * g.intlen;
+}
+
+// Stage records [w0, w0 + wr) of every segment into the wavefront's window
+// (layout [seg][wr][44], word 1 as its reciprocal), spread over the active
+// lanes eight loads at a time; the wavefront's later LDS reads see the stores
+// (LDS operations of a wavefront complete in order).
+__device__ __forceinline__ void ww_stage(const SpkDev* __restrict__ spk,
+                                         const double* __restrict__ coef, double* wrec,
+                                         int32_t w0, int32_t wr) {
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act);
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+  const int per_seg = wr * 44;
+  const int total = spk->n_seg * per_seg;
+  constexpr int U = 8;
+  for (int base = rank; base < total; base += U * nact) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = base + u * nact;
+      if (k < total) {
+        const int sg = k / per_seg, rem = k - sg * per_seg;
+        v[u] = coef[spk->seg[sg].off + (int64_t)w0 * 44 + rem];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * nact < total) wrec[base + u * nact] = v[u];
+  }
+  for (int k = rank; k < spk->n_seg * wr; k += nact) {
+    const int sg = k / wr, r = k - sg * wr;
+    wrec[k * 44 + 1] = 1.0 / coef[spk->seg[sg].off + (int64_t)(w0 + r) * 44 + 1];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Post-pass: back to the input center and frame at jd_final (mod.rs:115-116).
 __global__ void nbody_finish_kernel(const SpkDev* __restrict__ spk,
                                     const double* __restrict__ coef, NBodyArgs a) {
@@ -460,9 +519,12 @@ __device__ __forceinline__ int32_t order_bucket(const NBodyArgs& a, int64_t i) {
   return 2 * lv + (span < 0.0 ? 1 : 0);
 }
 
-// Bucket of each object and the bucket histogram (privatised in LDS).
+// Bucket of each object and the bucket histogram (privatised in LDS).  With
+// skey, also the radix-sort key of each object (bucket above the arc length,
+// log2 |span| in 1/65536 octaves) and its index.
 __global__ void order_hist_kernel(NBodyArgs a, uint16_t* __restrict__ bkt,
-                                  uint32_t* __restrict__ hist) {
+                                  uint32_t* __restrict__ hist, uint32_t* __restrict__ skey,
+                                  uint32_t* __restrict__ sval) {
   __shared__ uint32_t lh[ORDER_BUCKETS];
   for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x) lh[k] = 0;
   __syncthreads();
@@ -471,6 +533,13 @@ __global__ void order_hist_kernel(NBodyArgs a, uint16_t* __restrict__ bkt,
     const int32_t b = order_bucket(a, i);
     bkt[i] = (uint16_t)b;
     atomicAdd(&lh[b], 1u);
+    if (skey != nullptr) {
+      const double sp = fabs(a.jd_final[a.jd_final_stride ? i : 0] - a.jd0[i]);
+      const double q = (log2(sp) + 16.0) * 65536.0;
+      const uint32_t sk = q > 0.0 ? (q < 2097151.0 ? (uint32_t)q : 2097151u) : 0u;
+      skey[i] = ((uint32_t)b << 21) | sk;
+      sval[i] = (uint32_t)i;
+    }
   }
   __syncthreads();
   for (int k = threadIdx.x; k < ORDER_BUCKETS; k += blockDim.x)
@@ -609,8 +678,9 @@ __shared__ int32_t kh_park_n[4];
 // fresh: start at st.t from st.x, st.v (the integrator's first exact
 // evaluation); otherwise st is a step boundary (ParkRec) and the object
 // continues with its first substep.  Records: WINDOW reads them from the
-// launch's LDS window when there (else from HBM); otherwise each lane keeps
-// its record in rslot (LDS, 44 doubles), reloaded when the record changes.
+// LDS window when there (the block's, or with wwin the wavefront's, whose
+// first record is ww0; else from HBM); otherwise each lane keeps its record
+// in rslot (LDS, 44 doubles), reloaded when the record changes.
 // l: lane in the row, g0: the row's first lane in the wavefront.
 // ---------------------------------------------------------------------------
 template <bool WINDOW>
@@ -618,7 +688,8 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
                                                const double* __restrict__ coef,
                                                const NBodyArgs& a, const LdsTables& lds,
                                                int64_t obj, const ParkRec& st0, bool fresh,
-                                               int l, int g0, double* rslot, int32_t winw) {
+                                               int l, int g0, double* rslot, int32_t winw,
+                                               const double* wwin, int32_t ww0) {
   const double eps = a.epsilon;
   double t = st0.t;
   const double tf = st0.tf;
@@ -700,10 +771,11 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
       double ir, xs;
       Vec3 pp, dd{0.0, 0.0, 0.0};
       int32_t loc = -1;
-      if constexpr (WINDOW) loc = winw > 0 && all14 ? ri - kh_win_w0[seg] : -1;
+      if constexpr (WINDOW) loc = winw > 0 && all14 ? ri - (wwin ? ww0 : kh_win_w0[seg]) : -1;
       if (WINDOW && loc >= 0 && loc < winw) {
-        // the record in the launch's LDS window (word 1 staged as 1 / radius)
-        const double* rec = kh_win_rec + (seg * winw + loc) * 44;
+        // the record in the LDS window (the block's, or the wavefront's
+        // wwin; word 1 staged as 1 / radius)
+        const double* rec = (wwin ? wwin : kh_win_rec) + (seg * winw + loc) * 44;
         ir = rec[1];
         xs = (js - rec[0]) * ir;
         cheb_both_lds(rec, xs, pp, dd);
@@ -875,7 +947,12 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
 // AST: the force list is MASSES (the planets and five asteroids) instead of
 // PLANETS; a separate instantiation, so the planets-only kernels carry no
 // asteroid code.
-template <bool PERSISTENT, bool ILP = false, bool MULTI = false, bool AST = false>
+// WW: the plain planets-only launch of a batch with per-object final
+// epochs, whose wavefronts keep windows of their own when its epochs span
+// more than the block window (ww_mode); a separate instantiation, so the
+// common-epoch kernel carries no such code.
+template <bool PERSISTENT, bool ILP = false, bool MULTI = false, bool AST = false,
+          bool WW = false>
 __global__ void __launch_bounds__(ILP ? 64 : 256)
 __attribute__((amdgpu_waves_per_eu(ILP ? 1 : 2, ILP ? 1 : 8)))
 radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coef,
@@ -890,8 +967,12 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     }
   }
   // Stage a window of a.window records per segment (a one-year arc needs 13
-  // of 32 days), placed by window_first_record.
-  if (a.window > 0) {
+  // of 32 days), placed by window_first_record -- or, for the plain launch
+  // over a longer range of epochs, give each wavefront a window of its own
+  // (ww_mode), staged as its lanes go.
+  constexpr bool WWABLE = WW && !PERSISTENT && !ILP && !MULTI;
+  const bool ww = WWABLE && ww_mode(a, spk);
+  if (a.window > 0 && !ww) {
     if (threadIdx.x < spk->n_seg)
       kh_win_w0[threadIdx.x] = window_first_record(a, spk->seg[threadIdx.x]);
     __syncthreads();
@@ -906,7 +987,9 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     }
   }
   __syncthreads();
-  const int32_t winw = a.window;
+  const int32_t winw = ww ? ww_records(a) : a.window;
+  double* const wwin = ww ? kh_win_rec + (threadIdx.x >> 6) * (spk->n_seg * winw * 44) : nullptr;
+  int32_t ww0 = -(1 << 30);  // first record of the wavefront's window (none yet)
 
   const double eps = a.epsilon;
   slot_range(a);
@@ -961,6 +1044,24 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     }
     const double te = sub ? t + s * h : t;
 
+    // ---- the wavefront's window follows its lanes ----
+    if constexpr (WWABLE) {
+      if (ww) {
+        const SegDev& g = spk->seg[spk->emb_seg];
+        const int32_t ri = record_index(g, (te - 2451545.0) * 86400.0);
+        if (__ballot(ri < ww0 || ri >= ww0 + winw) != 0ull) {
+          const int32_t rs = record_index(g, (t - 2451545.0) * 86400.0);
+          const bool fwd = __builtin_amdgcn_readfirstlane(tf >= t ? 1 : 0) != 0;
+          int32_t w = fwd ? __ockl_wfred_min_i32(rs) : __ockl_wfred_max_i32(rs) - winw + 1;
+          w = w < 0 ? 0 : (w > g.n_rec - winw ? g.n_rec - winw : w);
+          if (ww0 < 0 || (fwd ? w > ww0 : w < ww0)) {
+            ww0 = w;
+            ww_stage(spk, coef, wwin, w, winw);
+          }
+        }
+      }
+    }
+
     // ---- the one force evaluation of this iteration ----
     Vec3 acc;
     int32_t impact = 0;
@@ -972,7 +1073,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
                          !sub, ngp, acc, impact, AST ? N_ASTEROIDS : 0);
     else
       st = spk_accel(spk, coef, te, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
-                     ngp, acc, impact, winw, AST ? N_ASTEROIDS : 0);
+                     ngp, acc, impact, winw, AST ? N_ASTEROIDS : 0, wwin, ww0);
     const double ev[3] = {acc.x, acc.y, acc.z};
 
     // ---- G column j (radau.rs:292-308), kept only on substeps ----
@@ -1252,7 +1353,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     if (g < np) {
       const ParkRec& r = kh_park_rec[w][g];
       wide_integrate<true>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),
-                           g * WIDE_L, nullptr, winw);
+                           g * WIDE_L, nullptr, winw, wwin, ww0);
     }
   }
 }
@@ -1606,7 +1707,7 @@ radau_wide_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ coe
   const int l = threadIdx.x & (WIDE_L - 1);
   const int g0 = (threadIdx.x & 63) & ~(WIDE_L - 1);
   double* const rslot = &s_rec[threadIdx.x / WIDE_L][l <= WIDE_EMB ? l : 0][0];
-  wide_integrate<false>(spk, coef, a, lds, obj, st0, true, l, g0, rslot, 0);
+  wide_integrate<false>(spk, coef, a, lds, obj, st0, true, l, g0, rslot, 0, nullptr, 0);
 }
 
 // Blocks of `kernel` that are resident at once on the whole device: the
@@ -1685,7 +1786,12 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     uint32_t* perm = off + ORDER_BUCKETS;
     uint16_t* bkt = reinterpret_cast<uint16_t*>(perm + p.n);
     hipLaunchKernelGGL(order_zero_kernel, dim3(1), dim3(256), 0, stream, hist);
-    hipLaunchKernelGGL(order_hist_kernel, dim3(g1), dim3(256), 0, stream, a, bkt, hist);
+    // per-object final epochs with a sort buffer: radix sort by cost bucket
+    // and arc length (kh_sort.hip) instead of the counting sort's scatter
+    const bool rsort = p.sort_buf != nullptr && p.jd_final_stride != 0 && !multi;
+    hipLaunchKernelGGL(order_hist_kernel, dim3(g1), dim3(256), 0, stream, a, bkt, hist,
+                       rsort ? order_sort_keys(p.sort_buf, p.n) : nullptr,
+                       rsort ? order_sort_vals(p.sort_buf, p.n) : nullptr);
     int64_t* split = p.mode == 0 && p.latency_k > 0 ? p.split : nullptr;
     const double lanes = (double)resident_grid(radau_nbody_kernel<false>, block, lds_bytes,
                                                p.n_cu) * block;
@@ -1695,7 +1801,12 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
     if (const char* e = getenv("KH_LATENCY_SHARE")) share = atof(e) > 0.0 ? atof(e) : 1.0;
     hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(256), 0, stream, hist, off, split,
                        lanes / share, p.latency_k);
-    hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off, perm);
+    if (rsort) {
+      if (launch_order_sort(p.n, p.sort_buf, p.sort_bytes, perm, stream)) return -1;
+    } else {
+      hipLaunchKernelGGL(order_scatter_kernel, dim3(g1), dim3(256), 0, stream, p.n, bkt, off,
+                         perm);
+    }
     a.perm = perm;
   }
   int64_t grid;
@@ -1783,6 +1894,9 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
       if (multi)
         hipLaunchKernelGGL((radau_nbody_kernel<false, false, true, A>), dim3((unsigned)grid),
                            dim3(block), lds_bytes, s_main, spk, coef, a);
+      else if (p.jd_final_stride != 0 && !A)
+        hipLaunchKernelGGL((radau_nbody_kernel<false, false, false, false, true>),
+                           dim3((unsigned)grid), dim3(block), lds_bytes, s_main, spk, coef, a);
       else
         hipLaunchKernelGGL((radau_nbody_kernel<false, false, false, A>), dim3((unsigned)grid),
                            dim3(block), lds_bytes, s_main, spk, coef, a);
