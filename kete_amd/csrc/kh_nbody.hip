@@ -655,6 +655,9 @@ constexpr int WIDE_EMB = 10;
 constexpr int PARK_SLOTS = 64 / WIDE_L;
 __shared__ ParkRec kh_park_rec[4][PARK_SLOTS];
 __shared__ int32_t kh_park_n[4];
+// first record of each wavefront's window (wavefront windows): the value the
+// window holds, for lanes that left the integration loop before its last move
+__shared__ int32_t kh_ww_w0[4];
 
 // ---------------------------------------------------------------------------
 // One object on sixteen lanes (one DPP row), from `st` to the end of its arc:
@@ -1010,7 +1013,10 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   // the plain planets-only launch hands its wavefronts' tails on
   constexpr bool PARKABLE = !PERSISTENT && !ILP && !MULTI && !AST;
   if constexpr (PARKABLE)
-    if ((threadIdx.x & 63) == 0) kh_park_n[threadIdx.x >> 6] = 0;
+    if ((threadIdx.x & 63) == 0) {
+      kh_park_n[threadIdx.x >> 6] = 0;
+      kh_ww_w0[threadIdx.x >> 6] = ww0;
+    }
   while (obj < a.n) {
     const bool sub = phase == PH_SUB;
     // live lanes of the wavefront (every lane with an object is here)
@@ -1056,6 +1062,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
           w = w < 0 ? 0 : (w > g.n_rec - winw ? g.n_rec - winw : w);
           if (ww0 < 0 || (fwd ? w > ww0 : w < ww0)) {
             ww0 = w;
+            kh_ww_w0[threadIdx.x >> 6] = w;
             ww_stage(spk, coef, wwin, w, winw);
           }
         }
@@ -1350,6 +1357,9 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     const int w = threadIdx.x >> 6;
     const int g = (threadIdx.x & 63) / WIDE_L;
     const int np = kh_park_n[w];
+    // the window as the wavefront's last move left it (a lane that ended
+    // earlier holds an older first record)
+    if (WWABLE && ww) ww0 = kh_ww_w0[w];
     if (g < np) {
       const ParkRec& r = kh_park_rec[w][g];
       wide_integrate<true>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),

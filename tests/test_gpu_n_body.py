@@ -144,14 +144,19 @@ def test_three_lane_error_semantics(ctx, oracle_spk, mix):
     assert np.all(g["status"] == 3)
 
 
-def test_uneven_spans_parity(ctx, oracle_spk):
+@pytest.mark.parametrize("launch", ["default", "main_kernel"])
+def test_uneven_spans_parity(ctx, oracle_spk, launch, monkeypatch):
     """Config 3 at its own spans: 4096 objects, |span| log-uniform in [1,
-    3650] d with random signs.  The GPU and the oracle run the same
-    algorithm with different roundings (FMA contraction, operation order),
-    and the adaptive Radau decisions (step sizes from a noisy b6, the
-    corrector's sweep count) turn last-ulp differences into integration-
-    error-sized ones, which planetary encounters on the 10-year NEO arcs
-    amplify.  The yardstick is therefore the oracle against itself: the same
+    3650] d with random signs, in two launches: the default one (a batch
+    this small goes to the sixteen-lane latency kernel almost whole) and the
+    main kernel alone (KH_LATENCY_K=0), whose wavefronts move their LDS
+    windows along with their lanes and hand their tails to sixteen-lane rows
+    that read the window where the wavefront left it.  The GPU and the
+    oracle run the same algorithm with different roundings (FMA
+    contraction, operation order), and the adaptive Radau decisions (step
+    sizes from a noisy b6, the corrector's sweep count) turn last-ulp
+    differences into integration-error-sized ones, which planetary
+    encounters on the 10-year NEO arcs amplify.  The yardstick is therefore the oracle against itself: the same
     restatement built with -ffp-contract=fast (a second, equally valid
     rounding).  Per decade of |span| the GPU's differences from the default
     oracle must be distributed like the FMA build's: quantiles 50/90/99 % at
@@ -169,6 +174,8 @@ def test_uneven_spans_parity(ctx, oracle_spk):
     such flip in another valid rounding, while objects 974 and 3509 go the
     other way (kete 5-6e-10 AU from it, the GPU 0.7-1e-10).  Statuses must
     be identical."""
+    if launch == "main_kernel":
+        monkeypatch.setenv("KH_LATENCY_K", "0")
     n = 4096
     xyz = syn.n_body_workload(n=n, neo_fraction=0.1, seed_index=3)
     span = syn.uneven_spans(n)
