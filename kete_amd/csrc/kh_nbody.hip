@@ -1325,11 +1325,13 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       }
     }
 
-    // ---- finished (or parked) objects: store, then take the next one ----
+    // ---- finished (or parked) objects: store, then take the next one (the
+    // plain launch has one object per lane: the lane is done) ----
     if (done >= 0 || parked) {
       // SSB-equatorial result; the post-pass converts it back
       if (!parked)
         store_result(a, obj, done, x, v, impact, te, int4{n_force, n_sweep, n_acc, n_fail});
+      if constexpr (!PERSISTENT) break;
       int64_t nxt = a.n;
       if (PERSISTENT) nxt = a.first_dynamic + (int64_t)atomicAdd(a.queue, 1ull);
       double nt, ntf;
@@ -1346,11 +1348,13 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
       ej = 0;
       n_force = n_sweep = n_acc = n_fail = 0;
     }
-    const bool fresh = done >= 0 || parked;
+    if constexpr (PERSISTENT) {
+      const bool fresh = done >= 0 || parked;
 #pragma unroll
-    for (int k = 0; k < 7; ++k)
+      for (int k = 0; k < 7; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) B[k][d] = fresh ? 0.0 : B[k][d];
+        for (int d = 0; d < 3; ++d) B[k][d] = fresh ? 0.0 : B[k][d];
+    }
   }
   // ---- the wavefront's parked objects, sixteen lanes each ----
   if constexpr (PARKABLE) {
