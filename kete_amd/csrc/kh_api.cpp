@@ -760,16 +760,20 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   if (const char* g = getenv("KH_GRID")) p.grid = atoi(g);
   // tail hand-off of the plain launch (KH_PARK=0 turns it off): a wavefront
   // down to KH_PARK_LANES live lanes (default 4, at most 4) parks the objects
-  // with more than KH_PARK_LEFT force evaluations to go (default 200, by
-  // their rate so far) and finishes them sixteen lanes each.  It catches the
-  // slow objects no cost key predicts (close approaches): 1e6 objects over a
-  // year, epsilon 1e-9: 471 -> 212 ms; epsilon 1e-6 with the one-lane
-  // latency kernel: 71.0 -> 69.9 ms, without it 67.5 ms.
+  // with more than KH_PARK_LEFT force evaluations to go (by their rate so
+  // far; default 0: every object still running) and finishes them sixteen
+  // lanes each.  It catches the slow objects no cost key predicts (close
+  // approaches): 1e6 objects over a year, epsilon 1e-9: 471 -> 212 ms
+  // (149-150 ms with the rows split by coordinate); epsilon 1e-6 with the
+  // one-lane latency kernel: 71.0 -> 69.9 ms, without it 67.5 ms.  With the
+  // split rows the threshold measured 62.1-62.6 / 61.1-61.3 / 60.6-60.2 /
+  // 59.8-60.0 ms at 200 / 50 / 10 / 0 evaluations (epsilon 1e-6), 148-150
+  // ms at 1e-9 for all of 0-200, 400: 192 ms.
   {
     const char* pk = getenv("KH_PARK");
     if (!me && !include_asteroids && !(pk && atoi(pk) == 0)) {
       p.park_lanes = 4;
-      p.park_min_left = 200.0;
+      p.park_min_left = 0.0;
       if (const char* e = getenv("KH_PARK_LANES")) p.park_lanes = atoi(e);
       if (const char* e = getenv("KH_PARK_LEFT")) p.park_min_left = atof(e);
       p.park_lanes = p.park_lanes < 0 ? 0 : (p.park_lanes > 4 ? 4 : p.park_lanes);

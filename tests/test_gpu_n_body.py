@@ -88,16 +88,17 @@ def test_plain_and_persistent_identical(ctx, mix, modes, monkeypatch):
 
 
 def test_tail_handoff(ctx, oracle_spk, monkeypatch):
-    """The plain launch's tail hand-off: once a wavefront is down to four
-    live lanes, objects with more than 200 force evaluations to go (by their
-    rate so far) are parked at a step boundary and finished on sixteen lanes
-    each.  A wavefront of 60 main-belt objects on 40-day arcs and four NEOs
-    on 3-year arcs: the NEOs are parked after the others end.  They must
-    match the oracle as the one-lane path does, agree with the one-lane
-    path (KH_PARK=0) to rounding -- not bitwise: the sixteen lanes sum the
-    body terms in another order, which shows the hand-off ran -- keep their
-    work counters running across it, and leave the short arcs bitwise
-    alone."""
+    """The plain launch's tail hand-off: once a wavefront is down to four live
+    lanes, objects with more than KH_PARK_LEFT force evaluations to go (by
+    their rate so far; 200 here, so that the short arcs stay on their lanes
+    -- the default, 0, parks every object still running) are parked at a
+    step boundary and finished on sixteen lanes each.  A wavefront of 60
+    main-belt objects on 40-day arcs and four NEOs on 3-year arcs: the NEOs
+    are parked after the others end.  They must match the oracle as the
+    one-lane path does, agree with the one-lane path (KH_PARK=0) to rounding
+    -- not bitwise: the sixteen lanes sum the body terms in another order,
+    which shows the hand-off ran -- keep their work counters running across
+    it, and leave the short arcs bitwise alone."""
     mb = syn.n_body_workload(n=60, neo_fraction=0.0, seed_index=11)
     neo = syn.n_body_workload(n=4, neo_fraction=1.0, seed_index=12)
     xyz = np.concatenate([mb, neo], axis=1)
@@ -106,6 +107,7 @@ def test_tail_handoff(ctx, oracle_spk, monkeypatch):
     batch = ka.StateArray(np.full(n, JD0), xyz)
     monkeypatch.setenv("KH_LATENCY_K", "0")
     monkeypatch.setenv("KH_PARK", "1")
+    monkeypatch.setenv("KH_PARK_LEFT", "200")
     g = ctx.n_body(batch, jf, counters=True)
     monkeypatch.setenv("KH_PARK", "0")
     p = ctx.n_body(batch, jf, counters=True)
@@ -146,31 +148,31 @@ def test_three_lane_error_semantics(ctx, oracle_spk, mix):
 
 @pytest.mark.parametrize("launch", ["default", "main_kernel"])
 def test_uneven_spans_parity(ctx, oracle_spk, launch, monkeypatch):
-    """Config 3 at its own spans: 4096 objects, |span| log-uniform in [1,
-    3650] d with random signs, in two launches: the default one (a batch
-    this small goes to the sixteen-lane latency kernel almost whole) and the
-    main kernel alone (KH_LATENCY_K=0), whose wavefronts move their LDS
-    windows along with their lanes and hand their tails to sixteen-lane rows
-    that read the window where the wavefront left it.  The GPU and the
-    oracle run the same algorithm with different roundings (FMA
-    contraction, operation order), and the adaptive Radau decisions (step
-    sizes from a noisy b6, the corrector's sweep count) turn last-ulp
-    differences into integration-error-sized ones, which planetary
-    encounters on the 10-year NEO arcs amplify.  The yardstick is therefore the oracle against itself: the same
-    restatement built with -ffp-contract=fast (a second, equally valid
-    rounding).  Per decade of |span| the GPU's differences from the default
-    oracle must be distributed like the FMA build's: quantiles 50/90/99 % at
-    most 2x, plus 1e-11 AU (the corrector's convergence level: one sweep more
-    or fewer moves a short arc by up to 1.1e-11 AU).  The factor bounds the
-    sampling spread of those statistics, measured on 20000 objects in
-    4096-object chunks (quantile ratios 0.87 - 1.66).  Each object's
-    difference must stay within 5x the decade's FMA-build maximum (the
-    maximum's spread: 0.41 - 4.2) or within the one-year bar scaled by the
-    arc, 1e-9 AU per 365.25 d: a step-failure flip (ten sweeps without the
-    corrector meeting its ~50-ulp threshold, decided by the last bit) puts
-    an arc on another step sequence whose truncation error differs by up to
-    ~1e-9 AU per year -- object 521 of this sample (972 d) ends 2e-11 AU
-    from the converged answer in kete's rounding and 1.1e-9 AU with one
+    """Config 3 at its own spans: 4096 objects, |span| log-uniform in [1, 3650] d
+    with random signs, in two launches: the default one (a batch this small
+    goes to the sixteen-lane latency kernel almost whole) and the main
+    kernel alone (KH_LATENCY_K=0), whose wavefronts move their LDS windows
+    along with their lanes and hand their tails to sixteen-lane rows that
+    read the window where the wavefront left it.  The GPU and the oracle run
+    the same algorithm with different roundings (FMA contraction, operation
+    order), and the adaptive Radau decisions (step sizes from a noisy b6,
+    the corrector's sweep count) turn last-ulp differences into
+    integration-error-sized ones, which planetary encounters on the 10-year
+    NEO arcs amplify.  The yardstick is therefore the oracle against itself:
+    the same restatement built with -ffp-contract=fast (a second, equally
+    valid rounding).  Per decade of |span| the GPU's differences from the
+    default oracle must be distributed like the FMA build's: quantiles
+    50/90/99 % at most 2x, plus 1e-11 AU (the corrector's convergence level:
+    one sweep more or fewer moves a short arc by up to 1.1e-11 AU).  The
+    factor bounds the sampling spread of those statistics, measured on 20000
+    objects in 4096-object chunks (quantile ratios 0.87 - 1.66).  Each
+    object's difference must stay within 5x the decade's FMA-build maximum
+    (the maximum's spread: 0.41 - 4.2) or within the one-year bar scaled by
+    the arc, 1e-9 AU per 365.25 d: a step-failure flip (ten sweeps without
+    the corrector meeting its ~50-ulp threshold, decided by the last bit)
+    puts an arc on another step sequence whose truncation error differs by
+    up to ~1e-9 AU per year -- object 521 of this sample (972 d) ends 2e-11
+    AU from the converged answer in kete's rounding and 1.1e-9 AU with one
     such flip in another valid rounding, while objects 974 and 3509 go the
     other way (kete 5-6e-10 AU from it, the GPU 0.7-1e-10).  Statuses must
     be identical."""
