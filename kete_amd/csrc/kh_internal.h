@@ -88,7 +88,7 @@ struct NBodyLaunch {
   double* mout[6];
   // force list: 0 PLANETS, N_ASTEROIDS MASSES (include_asteroids)
   int32_t n_ast;
-  // tail hand-off of the plain planets-only launch (park_lanes = 0: off):
+  // tail hand-off of the plain launch (park_lanes = 0: off):
   // when a wavefront is down to park_lanes (<= 4) live lanes, an object that
   // by its progress so far has more than park_min_left force evaluations to
   // go is parked at its next step boundary, and the wavefront finishes its

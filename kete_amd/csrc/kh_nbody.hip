@@ -677,7 +677,8 @@ __shared__ int32_t kh_ww_w0[4];
 // lane instead of the whole force sum: the latency of a slow object's serial
 // chain drops, at a sixteenth of the one-lane kernel's objects per wavefront.
 // Results differ from the one-lane kernel's only by the order in which the
-// body terms are summed.  Planets only.
+// body terms are summed.  With AST, lanes 11..15 add the five MASSES
+// asteroids.
 // fresh: start at st.t from st.x, st.v (the integrator's first exact
 // evaluation); otherwise st is a step boundary (ParkRec) and the object
 // continues with its first substep.  Records: WINDOW reads them from the
@@ -686,7 +687,7 @@ __shared__ int32_t kh_ww_w0[4];
 // in rslot (LDS, 44 doubles), reloaded when the record changes.
 // l: lane in the row, g0: the row's first lane in the wavefront.
 // ---------------------------------------------------------------------------
-template <bool WINDOW>
+template <bool WINDOW, bool AST = false>
 __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
                                                const double* __restrict__ coef,
                                                const NBodyArgs& a, const LdsTables& lds,
@@ -810,20 +811,43 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
     const Vec3 emb{row_bcast<WIDE_EMB>(bp.x), row_bcast<WIDE_EMB>(bp.y),
                    row_bcast<WIDE_EMB>(bp.z)};
     Vec3 acc{0.0, 0.0, 0.0};
+    // with the asteroids (MASSES), lanes 11..15 take asteroid l - 11: its
+    // Type 21 state, plus the Sun's (lane 0's body) for a Sun-centred
+    // segment, and its Newtonian term (add_asteroids)
+    const bool is_ast = AST && l > WIDE_EMB && l <= WIDE_EMB + N_ASTEROIDS;
+    Vec3 sun{0.0, 0.0, 0.0};
+    if constexpr (AST) sun = Vec3{row_bcast<0>(bp.x), row_bcast<0>(bp.y), row_bcast<0>(bp.z)};
     if (is_body) {
       if (bd.emb) bp = Vec3{bp.x + emb.x, bp.y + emb.y, bp.z + emb.z};
       if (lst == ST_OK)
         lst = add_body_at(bd, bp, bv, Vec3{xe[0], xe[1], xe[2]}, Vec3{ve[0], ve[1], ve[2]}, !sub,
                           ngp, acc);
     }
+    if constexpr (AST) {
+      if (is_ast) {
+        const AstDev& ad = spk->ast[l - WIDE_EMB - 1];
+        const double js = (te - 2451545.0) * 86400.0;
+        const T21Ref ref = t21_find(spk, ad.seg_begin, ad.seg_end, te, js);
+        if (!ref.found) {
+          lst = ST_DAF_LIMITS;
+        } else {
+          Vec3 p, unused;
+          t21_state_at<false>(spk, ref, js, p, unused);
+          if (ref.sun_centred) p = Vec3{p.x + sun.x, p.y + sun.y, p.z + sun.z};
+          lst = add_body_at(ad.bd, p, Vec3{0.0, 0.0, 0.0}, Vec3{xe[0], xe[1], xe[2]},
+                            Vec3{ve[0], ve[1], ve[2]}, !sub, nullptr, acc);
+        }
+      }
+    }
     int32_t st = emb_st, impact = 0;
     {
-      const uint64_t bad = __ballot(is_body && lst != ST_OK);
+      const uint64_t bad = __ballot((is_body || is_ast) && lst != ST_OK);
       const uint32_t gbad = (uint32_t)((bad >> g0) & 0xFFFFull);
       if (st == ST_OK && gbad) {
         const int first = __builtin_ctz(gbad);
         st = __shfl(lst, g0 + first, 64);
-        if (st == ST_IMPACT) impact = spk->body[first].id;
+        if (st == ST_IMPACT)
+          impact = first < N_PLANETS ? spk->body[first].id : spk->ast[first - WIDE_EMB - 1].bd.id;
       }
     }
     const double ev[3] = {row_sum(acc.x), row_sum(acc.y), row_sum(acc.z)};
@@ -1010,8 +1034,8 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   bool shortened = false;        // MULTI: the step was shortened to land on tf
   if (MULTI) tf = a.epochs[multi_run_end(a, 0, t)];
 
-  // the plain planets-only launch hands its wavefronts' tails on
-  constexpr bool PARKABLE = !PERSISTENT && !ILP && !MULTI && !AST;
+  // the plain launch hands its wavefronts' tails on
+  constexpr bool PARKABLE = !PERSISTENT && !ILP && !MULTI;
   if constexpr (PARKABLE)
     if ((threadIdx.x & 63) == 0) {
       kh_park_n[threadIdx.x >> 6] = 0;
@@ -1366,8 +1390,8 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     if (WWABLE && ww) ww0 = kh_ww_w0[w];
     if (g < np) {
       const ParkRec& r = kh_park_rec[w][g];
-      wide_integrate<true>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),
-                           g * WIDE_L, nullptr, winw, wwin, ww0);
+      wide_integrate<true, AST>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),
+                                g * WIDE_L, nullptr, winw, wwin, ww0);
     }
   }
 }
@@ -1781,8 +1805,8 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.n_epochs = p.n_epochs;
   for (int k = 0; k < 6; ++k) a.mout[k] = p.mout[k];
   const bool multi = p.epochs != nullptr && p.n_epochs > 0;
-  // tail hand-off: plain launch, one final epoch per object, planets only
-  a.park_lanes = p.mode == 0 && !multi && p.n_ast == 0 ? p.park_lanes : 0;
+  // tail hand-off: plain launch, one final epoch per object
+  a.park_lanes = p.mode == 0 && !multi ? p.park_lanes : 0;
   a.park_min_left = p.park_min_left;
   // object records: the plain launch over the longest-first order
   a.stage = p.stage != nullptr && p.mode == 0 && !multi && p.order_buf != nullptr &&
