@@ -676,9 +676,9 @@ __shared__ int32_t kh_ww_w0[4];
 // costs one body's terms and a third of the integrator's bookkeeping per
 // lane instead of the whole force sum: the latency of a slow object's serial
 // chain drops, at a sixteenth of the one-lane kernel's objects per wavefront.
-// Results differ from the one-lane kernel's only by the order in which the
-// body terms are summed.  With AST, lanes 11..15 add the five MASSES
-// asteroids.
+// Results differ from the one-lane kernel's only by rounding (the order in
+// which the body terms are summed, the compiler's FMA contraction).  With
+// AST, lanes 11..15 add the five MASSES asteroids.
 // fresh: start at st.t from st.x, st.v (the integrator's first exact
 // evaluation); otherwise st is a step boundary (ParkRec) and the object
 // continues with its first substep.  Records: WINDOW reads them from the
