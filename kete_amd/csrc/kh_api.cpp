@@ -771,7 +771,7 @@ static int32_t nbody_dev(kh_ctx* c, int64_t n, const double* jd0, const double* 
   // ms at 1e-9 for all of 0-200, 400: 192 ms.
   {
     const char* pk = getenv("KH_PARK");
-    if (!me && !(pk && atoi(pk) == 0)) {
+    if (!(pk && atoi(pk) == 0)) {
       p.park_lanes = 4;
       p.park_min_left = 0.0;
       if (const char* e = getenv("KH_PARK_LANES")) p.park_lanes = atoi(e);

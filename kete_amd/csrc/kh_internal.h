@@ -125,6 +125,10 @@ struct ParkRec {
   double t, tf, h;
   double x[3], v[3], a[3];
   double B[7][3];
+  // multi-epoch propagation: the next epoch to store, whether the step was
+  // shortened to land on the run's end, the step proposed before that
+  int32_t ej, shortened;
+  double h_want;
 };
 
 // Bytes of NBodyLaunch::order_buf for n objects.

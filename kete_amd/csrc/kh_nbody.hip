@@ -687,7 +687,16 @@ __shared__ int32_t kh_ww_w0[4];
 // in rslot (LDS, 44 doubles), reloaded when the record changes.
 // l: lane in the row, g0: the row's first lane in the wavefront.
 // ---------------------------------------------------------------------------
-template <bool WINDOW, bool AST = false>
+// Output column c (< 6) of a multi-epoch propagation, selected without
+// indexing the argument array by a lane-dependent value.
+__device__ __forceinline__ double* mout_col(const NBodyArgs& a, int c) {
+  double* r = a.mout[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) r = c == k ? a.mout[k] : r;
+  return r;
+}
+
+template <bool WINDOW, bool AST = false, bool MULTI = false>
 __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
                                                const double* __restrict__ coef,
                                                const NBodyArgs& a, const LdsTables& lds,
@@ -696,7 +705,10 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
                                                const double* wwin, int32_t ww0) {
   const double eps = a.epsilon;
   double t = st0.t;
-  const double tf = st0.tf;
+  double tf = st0.tf;
+  int32_t ej = 0;          // MULTI: the next epoch to store
+  double h_want = 0.0;     // MULTI: the step proposed before shortening
+  bool shortened = false;  // MULTI: the step was shortened to land on tf
   // this lane's coordinate of the integrator (lane 3 of each quad repeats
   // coordinate 2, so quad reductions see only the three coordinates)
   const int cd = (l & 3) < 3 ? (l & 3) : 2;
@@ -736,6 +748,11 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
     n_sweep = st0.n_sweep;
     n_acc = st0.n_acc;
     n_fail = st0.n_fail;
+    if constexpr (MULTI) {
+      ej = st0.ej;
+      h_want = st0.h_want;
+      shortened = st0.shortened != 0;
+    }
   }
 
   for (;;) {
@@ -889,6 +906,30 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
       }
       const double nx = xd + (vd * h + (h * h) * (ad * 0.5 + bw));
       const double nv = vd + h * (ad + bu);
+      if constexpr (MULTI) {
+        // epochs strictly inside the accepted step: the step's interpolant
+        // at s = (e - t)/h, this lane's coordinate (stored by lanes 0..2)
+        while (conv && ej < a.n_epochs) {
+          const double e = a.epochs[ej];
+          const double se = (e - t) / h;
+          if (!(se > 0.0 && se < 1.0) || fabs(e - (t + h)) < 1e-12) break;
+          double bw2 = 0.0, bu2 = 0.0, sp = 1.0;
+#pragma unroll
+          for (int k = 0; k < 7; ++k) {
+            sp *= se;
+            bw2 += B[k] * (sp * RTC.w[k]);
+            bu2 += B[k] * (sp * RTC.u[k]);
+          }
+          const double ox = xd + se * vd * h + (se * se) * (h * h) * (ad / 2.0 + bw2);
+          const double ov = vd + se * h * (ad + bu2);
+          if (l < 3) {
+            const int64_t q = (int64_t)ej * a.n + obj;
+            mout_col(a, l)[q] = ox;
+            mout_col(a, 3 + l)[q] = ov;
+          }
+          ++ej;
+        }
+      }
       xd = conv ? nx : xd;
       vd = conv ? nv : vd;
     }
@@ -947,8 +988,40 @@ __device__ __forceinline__ void wide_integrate(const SpkDev* __restrict__ spk,
         begin = true;
       }
     }
+    if constexpr (MULTI) {
+      // the epochs at the current time, after an accepted step and at the
+      // end of a run; then on to the end of the next run (as the one-lane
+      // kernel)
+      if (done == ST_OK || (conv && done < 0)) {
+        while (ej < a.n_epochs && fabs(a.epochs[ej] - t) < 1e-12) {
+          if (l < 3) {
+            const int64_t q = (int64_t)ej * a.n + obj;
+            mout_col(a, l)[q] = xd;
+            mout_col(a, 3 + l)[q] = vd;
+          }
+          ++ej;
+        }
+      }
+      if (done == ST_OK) {
+        if (ej < a.n_epochs) {
+          tf = a.epochs[multi_run_end(a, ej, t)];
+          const double hc = phase == PH_INIT ? 0.1 : (shortened ? h_want : h);
+          h = copysign(fmax(fabs(hc), 0.005), tf - t);
+          fails = 0;
+          done = -1;
+          begin = true;
+        }
+      } else if (done > 0) {
+        // a failing epoch and the later ones are undefined
+        const double nan = __builtin_nan("");
+        if (l < 6)
+          for (int32_t e = ej; e < a.n_epochs; ++e) mout_col(a, l)[(int64_t)e * a.n + obj] = nan;
+      }
+    }
     if (begin) {
-      if (fabs(t - tf) <= fabs(h)) h = tf - t;
+      if (MULTI) h_want = h;
+      shortened = fabs(t - tf) <= fabs(h);
+      if (shortened) h = tf - t;
       sweep = 0;
       j = 1;
       n_sweep++;
@@ -1035,7 +1108,7 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
   if (MULTI) tf = a.epochs[multi_run_end(a, 0, t)];
 
   // the plain launch hands its wavefronts' tails on
-  constexpr bool PARKABLE = !PERSISTENT && !ILP && !MULTI;
+  constexpr bool PARKABLE = !PERSISTENT && !ILP;
   if constexpr (PARKABLE)
     if ((threadIdx.x & 63) == 0) {
       kh_park_n[threadIdx.x >> 6] = 0;
@@ -1343,6 +1416,9 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
             for (int q = 0; q < 7; ++q)
 #pragma unroll
               for (int d = 0; d < 3; ++d) r.B[q][d] = B[q][d];
+            r.ej = ej;
+            r.shortened = shortened ? 1 : 0;
+            r.h_want = h_want;
             parked = true;
           }
         }
@@ -1390,8 +1466,9 @@ radau_nbody_kernel(const SpkDev* __restrict__ spk, const double* __restrict__ co
     if (WWABLE && ww) ww0 = kh_ww_w0[w];
     if (g < np) {
       const ParkRec& r = kh_park_rec[w][g];
-      wide_integrate<true, AST>(spk, coef, a, lds, r.obj, r, false, threadIdx.x & (WIDE_L - 1),
-                                g * WIDE_L, nullptr, winw, wwin, ww0);
+      wide_integrate<true, AST, MULTI>(spk, coef, a, lds, r.obj, r, false,
+                                       threadIdx.x & (WIDE_L - 1), g * WIDE_L, nullptr, winw,
+                                       wwin, ww0);
     }
   }
 }
@@ -1805,8 +1882,8 @@ int32_t launch_nbody(const SpkDev* spk, const double* coef, const NBodyLaunch& p
   a.n_epochs = p.n_epochs;
   for (int k = 0; k < 6; ++k) a.mout[k] = p.mout[k];
   const bool multi = p.epochs != nullptr && p.n_epochs > 0;
-  // tail hand-off: plain launch, one final epoch per object
-  a.park_lanes = p.mode == 0 && !multi ? p.park_lanes : 0;
+  // tail hand-off: the plain launch
+  a.park_lanes = p.mode == 0 ? p.park_lanes : 0;
   a.park_min_left = p.park_min_left;
   // object records: the plain launch over the longest-first order
   a.stage = p.stage != nullptr && p.mode == 0 && !multi && p.order_buf != nullptr &&
